@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X k-means Lloyd engine.
+
+One step = one full Lloyd training (fixed iteration count) over a resident
+synthetic Gaussian-mixture matrix, through the C ABI with device pointers.
+value = point-iterations/s = n * iterations * steps / elapsed.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fp32_c2]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("k-means point-iters/s + %roofline, bf16+refine vs fp32 path; p50/p99 job latency")
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+
+WORKLOADS = {
+    # Plain FP32 Lloyd (drop-in descriptor path): n=1e7, d=64, k=256, 256 Gaussians,
+    # centers U[-20,20]^d, per-cluster sigma U[0.5,2], random-row init, 20 iterations
+    "fp32_c2": dict(n=10_000_000, d=64, k=256, iters=20, center_range=20.0, sigma=(0.5, 2.0),
+                    anisotropic=False, bf16=False),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_data(torch, n, d, k, center_range, sigma, anisotropic, seed, device):
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    centers = (torch.rand((k, d), generator=g, device=device, dtype=torch.float64) * 2 - 1) * center_range
+    if anisotropic:
+        scale = 10.0 ** (torch.rand((k, d), generator=g, device=device, dtype=torch.float64) * 2 - 1)
+    else:
+        s = torch.rand((k, 1), generator=g, device=device, dtype=torch.float64) * (sigma[1] - sigma[0]) + sigma[0]
+        scale = s.expand(k, d)
+    x = torch.empty((n, d), device=device, dtype=torch.float32)
+    step = 1 << 20
+    for r0 in range(0, n, step):
+        r1 = min(n, r0 + step)
+        lab = torch.randint(0, k, (r1 - r0,), generator=g, device=device)
+        noise = torch.randn((r1 - r0, d), generator=g, device=device, dtype=torch.float32)
+        x[r0:r1] = (centers[lab] + noise.double() * scale[lab]).float()
+    perm = torch.randperm(n, generator=g, device=device)[:k]
+    c0 = x[perm].clone()
+    return x, c0
+
+
+def cpu_baseline(x_host, c0_host, iters, budget_s=20.0):
+    """The oracle's restatement of the reference CPU Lloyd, timed on a bounded
+    sample of the workload (first rows, same initial centroids)."""
+    from oracle import oracle as O
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    n = x_host.shape[0]
+    sample = min(n, 4096)
+    t0 = time.perf_counter()
+    O.daal_lloyd(x_host[:sample], c0_host, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+    dt = time.perf_counter() - t0
+    # size the sample for about budget_s seconds of work (one iteration)
+    sample = int(min(n, max(sample, sample * budget_s / max(dt, 1e-3))))
+    sample = max(sample, c0_host.shape[0])
+    t0 = time.perf_counter()
+    O.daal_lloyd(x_host[:sample], c0_host, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": sample * 1 / dt, "unit": "point-iterations/s", "cores": cores,
+            "kind": "port",
+            "sample": "first %d rows of the workload, 1 Lloyd iteration, oracle DAAL-Lloyd "
+                      "restatement (float32, %d threads, %.1f s)" % (sample, cores, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="fp32_c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the row-sharded design is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from onedal_amd.engine import Engine, KM_OPT_TIMING
+
+    wl = WORKLOADS[args.workload]
+    n, d, k, iters = wl["n"], wl["d"], wl["k"], wl["iters"]
+    dev = torch.device("cuda:0")
+    log("generating %s: n=%d d=%d k=%d" % (args.workload, n, d, k))
+    x, c0 = make_data(torch, n, d, k, wl["center_range"], wl["sigma"], wl["anisotropic"], 0, dev)
+    torch.cuda.synchronize()
+    eng = Engine(0)
+    eng.set_option(KM_OPT_TIMING, 1)
+    c_out = torch.empty((k, d), dtype=torch.float32, device=dev)
+    l_out = torch.empty(n, dtype=torch.int32, device=dev)
+    for w in range(args.warmup):
+        eng.train_device(x, c0, iters, 0.0, bf16=wl["bf16"], out=(c_out, l_out))
+        log("warmup %d done" % w)
+    torch.cuda.synchronize()
+    times = []
+    stats = []
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, obj, it = eng.train_device(x, c0, iters, 0.0, bf16=wl["bf16"], out=(c_out, l_out))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        stats.append(eng.stats())
+        log("step %d: %.3f s, iterations %d, objective %.6e, refined rows %d/%d" % (
+            s, times[-1], it, obj, stats[-1]["refined_candidate_rows"], stats[-1]["refined_full_rows"]))
+    elapsed = time.perf_counter() - t_all
+    pit = n * iters * args.steps / elapsed
+
+    # roofline of the dominant kernel (assignment): algorithmic 2*k*d flops per row
+    launches = sum(s["assign_launches"] for s in stats)
+    kern_ms = sum(s["assign_kernel_ms"] for s in stats)
+    avg_ms = kern_ms / max(1, launches)
+    flops_per_launch = 2.0 * k * d * n
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK["bf16_mfma_tflops"] if wl["bf16"] else PEAK["fp32_mfma_tflops"]
+    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "kernel": "assign (fused distance GEMM + argmin)", "avg_launch_ms": round(avg_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        xs = x[: min(n, 400_000)].cpu().numpy()
+        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters)
+
+    out = {
+        "metric": METRIC,
+        "value": pit,
+        "unit": "point-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "d": d, "k": k, "iterations": iters,
+                   "path": "bf16+refine" if wl["bf16"] else "fp32",
+                   "init": "random distinct rows (seed 0)"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "loop_ms_per_step": sum(s["iterations_ms"] for s in stats) / len(stats),
+        "refined_rows_per_step": stats[-1]["refined_candidate_rows"] + stats[-1]["refined_full_rows"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+/* kmeans_hip.h -- C ABI of the MI355X k-means Lloyd engine (libkmeans_hip.so).
+ *
+ * Drop-in boundary for oneDAL's dense Lloyd GPU backend.  Each entry point
+ * replaces one reference interface:
+ *
+ *   km_lloyd_train_f32 / _f64  <->  oneapi::dal::kmeans::backend::
+ *       train_kernel_gpu<Float, method::lloyd_dense, task::clustering>::operator()
+ *       (cpp/oneapi/dal/algo/kmeans/backend/gpu/train_kernel.hpp:24-29,
+ *        implemented at train_kernel_lloyd_dense_dpc.cpp:76-207)
+ *   km_lloyd_infer_f32 / _f64  <->  infer_kernel_gpu<Float, method::lloyd_dense, ...>
+ *       (backend/gpu/infer_kernel.hpp:24-29, infer_kernel_dpc.cpp:36-107)
+ *   km_context_create/destroy  <->  dal::backend::context_gpu + sycl::queue
+ *       (cpp/oneapi/dal/backend/dispatcher.hpp:118-132): one context = one
+ *       device + one in-order HIP stream.
+ *
+ * Plain pointers and sizes only; no exception crosses the ABI.  Status codes
+ * map one-to-one onto the oneapi::dal exception types
+ * (cpp/oneapi/dal/exceptions.hpp:28-100).  Preconditions are those of
+ * cpp/oneapi/dal/algo/kmeans/detail/train_ops.hpp:40-61 and infer_ops.hpp:40-56
+ * and are checked on the host, in the same order, before any launch.
+ *
+ * Arithmetic contract: labels are argmin_j of the exact-form distance
+ * sum_f (x_f - c_jf)^2 in the input precision (ties to the lowest j); centroid
+ * sums and the objective are accumulated in double in a fixed order, so results
+ * do not depend on timing, and the FP32 and BF16 paths return identical labels.
+ */
+#ifndef ONEDAL_AMD_KMEANS_HIP_H
+#define ONEDAL_AMD_KMEANS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct km_ctx km_ctx;
+
+typedef enum {
+    KM_OK = 0,
+    KM_DOMAIN_ERROR = 1,      /* oneapi::dal::domain_error */
+    KM_INVALID_ARGUMENT = 2,  /* oneapi::dal::invalid_argument */
+    KM_DEVICE_BAD_ALLOC = 3,  /* oneapi::dal::device_bad_alloc */
+    KM_HOST_BAD_ALLOC = 4,    /* oneapi::dal::host_bad_alloc */
+    KM_UNIMPLEMENTED = 5,     /* oneapi::dal::unimplemented */
+    KM_INTERNAL_ERROR = 6     /* oneapi::dal::internal_error (HIP runtime failure) */
+} km_status;
+
+/* call flags */
+#define KM_FLAG_EXACT_OBJECTIVE 0x1u /* infer: also return the objective
+                                        (result_options::compute_exact_objective_function) */
+#define KM_FLAG_BF16_REFINE 0x2u     /* distances on BF16 matrix cores, near-ties
+                                        re-scored in the input precision */
+#define KM_FLAG_DEVICE_POINTERS 0x4u /* X, C0/C and every output are device pointers
+                                        on the context's device */
+
+/* context options (km_context_set_option) */
+#define KM_OPT_ASSIGN_PATH 1 /* 0 auto, 1 generic tile kernel, 2 FP32 matrix core */
+#define KM_OPT_TIMING 2      /* 1: time the assignment kernels with HIP events */
+
+typedef struct {
+    int64_t assign_launches;      /* main assignment kernel launches in the last call */
+    double assign_kernel_ms;      /* their summed duration (KM_OPT_TIMING only) */
+    double iterations_ms;         /* Lloyd loop wall time on the stream (KM_OPT_TIMING only) */
+    int64_t refined_candidate_rows; /* rows settled by the candidate re-score */
+    int64_t refined_full_rows;      /* rows re-scored against every centroid */
+    int64_t empty_cluster_events;   /* iterations that refilled empty clusters */
+    int32_t assign_path;            /* path used: 1 generic, 2 FP32 MFMA, 3 BF16 MFMA */
+    int32_t reserved;
+} km_stats;
+
+/* Create a context on HIP device `device` with its own stream. */
+km_status km_context_create(int device, km_ctx** out);
+km_status km_context_destroy(km_ctx* ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char* km_last_error(const km_ctx* ctx);
+km_status km_context_set_option(km_ctx* ctx, int option, int64_t value);
+km_status km_get_stats(const km_ctx* ctx, km_stats* out);
+
+/* Lloyd training.  X: n x d row-major; C0: k x d row-major initial centroids.
+ * Outputs: C_out k x d, labels_out n (int32), *obj_out objective (sum of squared
+ * distances to the final centroids), *iters_out iterations run.
+ * Stop rule of the oneDAL GPU driver (train_kernel_lloyd_dense_dpc.cpp:147-168):
+ * after iteration t, stop if acc_thr > 0 and obj_t + acc_thr > obj_{t-1}. */
+km_status km_lloyd_train_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, const float* C0,
+                             int64_t k, int64_t max_iter, double acc_thr, uint32_t flags,
+                             float* C_out, int32_t* labels_out, double* obj_out,
+                             int64_t* iters_out);
+km_status km_lloyd_train_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C0,
+                             int64_t k, int64_t max_iter, double acc_thr, uint32_t flags,
+                             double* C_out, int32_t* labels_out, double* obj_out,
+                             int64_t* iters_out);
+
+/* Lloyd inference: labels of X against centroids C (k x d); objective when
+ * KM_FLAG_EXACT_OBJECTIVE is set (obj_out may be NULL otherwise). */
+km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, const float* C,
+                             int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
+km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
+                             int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ONEDAL_AMD_KMEANS_HIP_H */

@@ -1,0 +1,314 @@
+// km_assign.hpp -- fused distance + argmin kernels (assign_clusters).
+//
+// Replaces oneDAL's assign_clusters = oneMKL gemm (D = -2 X C^T, written to HBM)
+// + select_min_distance + complete_closest_distances
+// (cpp/oneapi/dal/algo/kmeans/backend/gpu/kernels_fp_impl.hpp:127-275, 416-439).
+// Here the n x k distance matrix never leaves registers: each lane keeps a
+// running top-2 (value, index) over the centroids it sees (a "slot"), and a row
+// is settled by the certification rule below.  Rows the rule cannot settle go
+// to the exact re-score kernels in km_refine.hpp.
+//
+// Certification.  v_j is the approximate value of ||c_j||^2 - 2 x.c_j and
+// band(x) bounds 2*(|v_j - V_j| + |D_j - dist_j|) (km_common.hpp).  With
+// T = min_j v_j + band(x), every centroid whose exact-form distance D_j can be
+// minimal has v_j <= T.  A slot holds its two smallest v; entries it dropped
+// are >= its second value.  So if every slot's second value exceeds T, the
+// set S = {slot minima <= T} contains every possible minimiser: |S| == 1 settles
+// the row, 2 <= |S| <= MAXC sends S to the candidate re-score, anything else
+// (a slot with two entries inside the band) sends the row to the full re-score.
+#pragma once
+
+#include "km_common.hpp"
+
+namespace km {
+
+template <typename F>
+struct top2 {
+    F m1, m2;
+    int i1, i2;
+    __device__ __forceinline__ void init() {
+        m1 = fp_traits<F>::inf;
+        m2 = fp_traits<F>::inf;
+        i1 = -1;
+        i2 = -1;
+    }
+    __device__ __forceinline__ void push(F v, int j) {
+        if (v < m1) {
+            m2 = m1;
+            i2 = i1;
+            m1 = v;
+            i1 = j;
+        } else if (v < m2) {
+            m2 = v;
+            i2 = j;
+        }
+    }
+};
+
+// Settle one row from its slots (see the file comment).
+template <typename F>
+__device__ __forceinline__ void decide_row(int64_t row, const top2<F>* slots, int nslots,
+                                           double band, int32_t* __restrict__ labels,
+                                           const refine_lists& L) {
+    F best = fp_traits<F>::inf;
+    int bi = -1;
+    for (int s = 0; s < nslots; ++s) {
+        if (slots[s].m1 < best || (slots[s].m1 == best && slots[s].i1 >= 0 && slots[s].i1 < bi)) {
+            best = slots[s].m1;
+            bi = slots[s].i1;
+        }
+    }
+    // 4.44e-16*|best| covers the rounding of this sum in double
+    const double T = double(best) + band + fabs(double(best)) * 4.44e-16;
+    bool cert = (bi >= 0) && (T < double(fp_traits<F>::inf));
+    int cnt = 0;
+    int cand[MAXC];
+    for (int s = 0; s < nslots; ++s) {
+        if (double(slots[s].m2) <= T) cert = false;
+        if (slots[s].i1 >= 0 && double(slots[s].m1) <= T) {
+            if (cnt < MAXC) cand[cnt] = slots[s].i1;
+            cnt++;
+        }
+    }
+    if (cert && cnt == 1) {
+        labels[row] = bi;
+    } else if (cert && cnt <= MAXC) {
+        const int e = atomicAdd(&L.counters[0], 1);
+        L.cand_rows[e] = int32_t(row);
+        L.cand_count[e] = cnt;
+        for (int c = 0; c < cnt; ++c) L.cand_ids[int64_t(e) * MAXC + c] = cand[c];
+    } else {
+        const int e = atomicAdd(&L.counters[1], 1);
+        L.full_rows[e] = int32_t(row);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic tile kernel (any d, any k; F = float or double).  256 threads, 64
+// rows x 64 centroids per step, K chunks of 16 features staged in LDS, 4 x 4
+// register micro-tile per thread.  Each row gets 16 slots (one per centroid
+// column group).  FP64 vector FMA runs at the same rate as FP64 MFMA on this
+// chip, so this is also the FP64 production kernel.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void assign_generic_kernel(
+    const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C, int k,
+    const F* __restrict__ cn, const float* __restrict__ xnorm_up, const double* __restrict__ cmax_p,
+    band_consts bc, int32_t* __restrict__ labels, refine_lists L) {
+    constexpr int TN = 64, TM = 64, KC = 16, PAD = 4;
+    __shared__ __align__(16) F Xs[KC][TN + PAD];
+    __shared__ __align__(16) F Cs[KC][TM + PAD];
+    __shared__ top2<F> slots[TN][16];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15;   // centroid group
+    const int ty = tid >> 4;   // row group
+    const double cmax = *cmax_p;
+    const int64_t ntiles = (n + TN - 1) / TN;
+
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t n0 = tile * TN;
+        top2<F> st[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) st[i].init();
+
+        for (int m0 = 0; m0 < k; m0 += TM) {
+            F acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = F(0);
+
+            for (int k0 = 0; k0 < d; k0 += KC) {
+                // stage X[n0:n0+64, k0:k0+16] and C[m0:m0+64, k0:k0+16] transposed
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = tid + r * 256;
+                    const int f = e & 15, p = e >> 4;
+                    const int64_t row = n0 + p;
+                    const int col = k0 + f;
+                    Xs[f][p] = (row < n && col < d) ? X[row * d + col] : F(0);
+                    const int cj = m0 + p;
+                    Cs[f][p] = (cj < k && col < d) ? C[int64_t(cj) * d + col] : F(0);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int f = 0; f < KC; ++f) {
+                    F a[4], b[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        a[i] = Xs[f][ty * 4 + i];
+                        b[i] = Cs[f][tx * 4 + i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int cj = m0 + tx * 4 + j;
+                if (cj < k) {
+                    const F c2 = cn[cj];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) st[i].push(fma(F(-2), acc[i][j], c2), cj);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) slots[ty * 4 + i][tx] = st[i];
+        __syncthreads();
+        if (tid < TN) {
+            const int64_t row = n0 + tid;
+            if (row < n) {
+                const double band = band_of(bc, double(xnorm_up[row]), cmax);
+                decide_row<F>(row, slots[tid], 16, band, labels, L);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 matrix-core kernel: v_mfma_f32_32x32x2_f32, d <= DP (DP in {32,64,128,256}).
+//
+// Workgroup = 4 waves; each wave owns NB blocks of 32 rows, held in VGPRs for
+// the whole pass (B operand), and streams every centroid through LDS in blocks
+// of 32 (A operand).  Lane l = 32*h + r: h selects the feature half
+// [h*DP/2, (h+1)*DP/2) so one MFMA step s multiplies feature s of half 0 and
+// feature DP/2+s of half 1; the 32x32 f32 accumulator of block (M, N) puts
+// D[m][n] with n = l % 32, m = 8*(i/4) + 4*h + (i%4) in register i.  A lane
+// therefore sees 16 of the 32 centroids of each block for a single row: each
+// row has two slots (h = 0, 1).
+// ---------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int DP, int NB, bool VEC>
+__global__ __launch_bounds__(256) void assign_mfma_f32_kernel(
+    const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Cpad, int k,
+    const float* __restrict__ cn, const float* __restrict__ xnorm_up,
+    const double* __restrict__ cmax_p, band_consts bc, int32_t* __restrict__ labels,
+    refine_lists L) {
+    constexpr int HALF = DP / 2;
+    constexpr int CST = DP + 4;            // LDS row stride (floats)
+    constexpr int ROWS_PER_WAVE = 32 * NB;
+    constexpr int ROWS_PER_WG = 4 * ROWS_PER_WAVE;
+    __shared__ __align__(16) float Cs[2][32 * CST];
+    __shared__ __align__(16) float Cn[2][32];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int r32 = lane & 31;
+    const int h = lane >> 5;
+    const double cmax = *cmax_p;
+    const int mblocks = (k + 31) / 32;
+    const int64_t ntiles = (n + ROWS_PER_WG - 1) / ROWS_PER_WG;
+
+    // stage centroid block mb (32 x DP floats, zero padded rows past k) into buffer b
+    auto stage = [&](int mb, int b) {
+        constexpr int F4_PER_ROW = DP / 4;
+        constexpr int TOTAL = 32 * F4_PER_ROW;
+        for (int e = tid; e < TOTAL; e += 256) {
+            const int row = e / F4_PER_ROW, c4 = e % F4_PER_ROW;
+            const int cj = mb * 32 + row;
+            floatx4 v = {0.f, 0.f, 0.f, 0.f};
+            if (cj < k) v = *reinterpret_cast<const floatx4*>(Cpad + int64_t(cj) * DP + c4 * 4);
+            *reinterpret_cast<floatx4*>(&Cs[b][row * CST + c4 * 4]) = v;
+        }
+        if (tid < 32) {
+            const int cj = mb * 32 + tid;
+            Cn[b][tid] = cj < k ? cn[cj] : fp_traits<float>::inf;
+        }
+    };
+
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t wrow0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
+        // B operand: this lane's half-row of each of its NB rows
+        float xr[NB][HALF];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int64_t row = wrow0 + nb * 32 + r32;
+            const bool ok = row < n;
+            const float* src = X + (ok ? row : 0) * int64_t(d);
+            if (VEC) {
+#pragma unroll
+                for (int s = 0; s < HALF; s += 4) {
+                    const int f = h * HALF + s;
+                    floatx4 v = {0.f, 0.f, 0.f, 0.f};
+                    if (ok && f < d) v = *reinterpret_cast<const floatx4*>(src + f);
+                    xr[nb][s] = v[0];
+                    xr[nb][s + 1] = v[1];
+                    xr[nb][s + 2] = v[2];
+                    xr[nb][s + 3] = v[3];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < HALF; ++s) {
+                    const int f = h * HALF + s;
+                    xr[nb][s] = (ok && f < d) ? src[f] : 0.f;
+                }
+            }
+        }
+        top2<float> st[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) st[nb].init();
+
+        __syncthreads();
+        stage(0, 0);
+        __syncthreads();
+        for (int mb = 0; mb < mblocks; ++mb) {
+            const int b = mb & 1;
+            if (mb + 1 < mblocks) stage(mb + 1, b ^ 1);
+            floatx16 acc[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[nb][i] = 0.f;
+            const float* arow = &Cs[b][r32 * CST + h * HALF];
+#pragma unroll
+            for (int s = 0; s < HALF; s += 4) {
+                const floatx4 a4 = *reinterpret_cast<const floatx4*>(arow + s);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q], xr[nb][s + q],
+                                                                       acc[nb], 0, 0, 0);
+            }
+            // epilogue: rows m = 8*(i/4) + 4*h + (i%4) of this centroid block
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const floatx4 c4 = *reinterpret_cast<const floatx4*>(&Cn[b][8 * q + 4 * h]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int cj = mb * 32 + 8 * q + 4 * h + t;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        st[nb].push(__builtin_fmaf(-2.f, acc[nb][4 * q + t], c4[t]), cj);
+                }
+            }
+            __syncthreads();
+        }
+        // settle rows: lane r32 (h = 0) merges its slot with lane r32 + 32
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            top2<float> sl[2];
+            sl[0] = st[nb];
+            sl[1].m1 = __shfl_xor(st[nb].m1, 32);
+            sl[1].m2 = __shfl_xor(st[nb].m2, 32);
+            sl[1].i1 = __shfl_xor(st[nb].i1, 32);
+            sl[1].i2 = __shfl_xor(st[nb].i2, 32);
+            const int64_t row = wrow0 + nb * 32 + r32;
+            if (h == 0 && row < n) {
+                const double band = band_of(bc, double(xnorm_up[row]), cmax);
+                decide_row<float>(row, sl, 2, band, labels, L);
+            }
+        }
+    }
+}
+
+}  // namespace km

@@ -1,0 +1,744 @@
+// km_engine.hip -- host driver and C ABI of the MI355X k-means Lloyd engine.
+//
+// Replaces train_kernel_gpu<F, lloyd_dense> (cpp/oneapi/dal/algo/kmeans/backend/
+// gpu/train_kernel_lloyd_dense_dpc.cpp:76-207) with its cluster_updater
+// (cluster_updater.hpp:82-179) and infer_kernel_gpu (infer_kernel_dpc.cpp:36-107).
+//
+// One Lloyd iteration on the context's stream:
+//   centroid_prep -> assign (+ exact re-score) -> count -> scan -> stable sort of
+//   row ids by label -> chunk sums (FP64) -> merge/divide [-> objective]
+//   -> one small D2H (empty-cluster count, objective) and a stream sync
+//   [-> empty-cluster refill].
+// The reference syncs twice per iteration (cluster_updater.hpp:135-136 and
+// kernels_integral_dpc.cpp:148); this driver syncs once.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <type_traits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/onedal_amd/kmeans_hip.h"
+#include "km_assign.hpp"
+#include "km_common.hpp"
+#include "km_refine.hpp"
+#include "km_update.hpp"
+
+namespace km {
+
+struct status_error : std::runtime_error {
+    km_status status;
+    status_error(km_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define KM_HIP(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            throw km::status_error(e_ == hipErrorOutOfMemory ? KM_DEVICE_BAD_ALLOC             \
+                                                         : KM_INTERNAL_ERROR,              \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));         \
+        }                                                                                  \
+    } while (0)
+
+#define KM_LAUNCH_CHECK() KM_HIP(hipGetLastError())
+
+// grow-only device buffer
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <typename T>
+    T* get(size_t count) {
+        const size_t need = std::max<size_t>(count * sizeof(T), 256);
+        if (need > bytes) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            hipError_t e = hipMalloc(&p, need);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw status_error(KM_DEVICE_BAD_ALLOC,
+                                   "device allocation of " + std::to_string(need) + " bytes failed");
+            }
+            bytes = need;
+        }
+        return static_cast<T*>(p);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace km
+
+struct km_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    km_stats stats{};
+    int assign_path_opt = 0;
+    bool timing = false;
+
+    // workspace
+    km::dbuf x_dev, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
+        lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
+        sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
+        c_out_dev;
+    int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+
+    hipEvent_t event() {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e;
+            KM_HIP(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_used++];
+    }
+};
+
+namespace km {
+
+static int grid_for(const km_ctx* ctx, int64_t work_items, int per_block, int waves_per_cu = 8) {
+    const int64_t want = (work_items + per_block - 1) / per_block;
+    const int64_t cap = int64_t(ctx->cu_count) * waves_per_cu;
+    return int(std::max<int64_t>(1, std::min(want, cap)));
+}
+
+template <typename T>
+static void fill(km_ctx* ctx, T* p, int64_t count, T value) {
+    const int grid = grid_for(ctx, count, 256, 4);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(grid), dim3(256), 0, ctx->stream, p, count, value);
+    KM_LAUNCH_CHECK();
+}
+
+template <typename F>
+struct work {
+    // sizes
+    int64_t n = 0;
+    int d = 0, k = 0, dpad = 0;
+    // device views
+    const F* X = nullptr;
+    float* xnorm = nullptr;
+    int32_t* labels = nullptr;
+    refine_lists L{};
+    F* cn = nullptr;
+    F* cpad = nullptr;
+    unsigned long long* cmax_bits = nullptr;
+};
+
+static int pad_dim(int d) {
+    int p = 32;
+    while (p < d) p <<= 1;
+    return p;
+}
+
+template <typename F>
+static int choose_path(const km_ctx* ctx, int d) {
+    if (std::is_same<F, double>::value) return 1;
+    if (ctx->assign_path_opt == 1) return 1;
+    if (d <= 256) return 2;
+    return 1;
+}
+
+template <typename F>
+static band_consts make_band(int d) {
+    const double u = fp_traits<F>::u;
+    const double ua = fp_traits<F>::u_acc;
+    band_consts b;
+    const double ga = gamma_m(double(d), ua);
+    b.a_xc = 2.0 * ga + 2.01 * u;
+    b.a_cc = 2.01 * u + gamma_m(double(d), std::is_same<F, double>::value ? ua : 2.220446049250313e-16);
+    b.g_canon = gamma_m(double(d + 2), u);
+    return b;
+}
+
+// per-centroid prep: ||c||^2, padded copy, max ||c||
+template <typename F>
+static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, bool need_pad) {
+    fill<unsigned long long>(ctx, w.cmax_bits, 1, 0ull);
+    const int grid = grid_for(ctx, int64_t(w.k) * 64, 256, 4);
+    hipLaunchKernelGGL(centroid_prep_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, C, w.k, w.d,
+                       w.dpad, w.cn, need_pad ? w.cpad : (F*)nullptr, w.cmax_bits);
+    KM_LAUNCH_CHECK();
+}
+
+template <int DP, int NB>
+static void launch_mfma_f32(km_ctx* ctx, work<float>& w, const band_consts& bc) {
+    constexpr int ROWS_PER_WG = 4 * 32 * NB;
+    const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
+    const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
+    const bool vec = (w.d % 4) == 0 && (reinterpret_cast<uintptr_t>(w.X) % 16) == 0;
+    if (vec)
+        hipLaunchKernelGGL((assign_mfma_f32_kernel<DP, NB, true>), dim3(grid), dim3(256), 0,
+                           ctx->stream, w.X, w.n, w.d, w.cpad, w.k, w.cn, w.xnorm,
+                           (const double*)w.cmax_bits, bc, w.labels, w.L);
+    else
+        hipLaunchKernelGGL((assign_mfma_f32_kernel<DP, NB, false>), dim3(grid), dim3(256), 0,
+                           ctx->stream, w.X, w.n, w.d, w.cpad, w.k, w.cn, w.xnorm,
+                           (const double*)w.cmax_bits, bc, w.labels, w.L);
+    KM_LAUNCH_CHECK();
+}
+
+// assignment of every row against w.X/C (centroid_prep already done for C)
+template <typename F>
+static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
+    fill<int32_t>(ctx, w.L.counters, 2, 0);
+    const band_consts bc = make_band<F>(w.d);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        e0 = ctx->event();
+        e1 = ctx->event();
+        KM_HIP(hipEventRecord(e0, ctx->stream));
+    }
+    if constexpr (std::is_same<F, float>::value) {
+        if (path == 2) {
+            switch (w.dpad) {
+                case 32: launch_mfma_f32<32, 2>(ctx, w, bc); break;
+                case 64: launch_mfma_f32<64, 2>(ctx, w, bc); break;
+                case 128: launch_mfma_f32<128, 1>(ctx, w, bc); break;
+                default: launch_mfma_f32<256, 1>(ctx, w, bc); break;
+            }
+        }
+    }
+    if (path == 1) {
+        const int64_t tiles = (w.n + 63) / 64;
+        const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 4)));
+        hipLaunchKernelGGL(assign_generic_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, w.X, w.n,
+                           w.d, C, w.k, w.cn, w.xnorm, (const double*)w.cmax_bits, bc, w.labels, w.L);
+        KM_LAUNCH_CHECK();
+    }
+    if (ctx->timing) {
+        KM_HIP(hipEventRecord(e1, ctx->stream));
+    }
+    ctx->stats.assign_launches++;
+    const int rgrid = ctx->cu_count * 4;
+    hipLaunchKernelGGL(refine_candidates_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d,
+                       C, w.L, w.labels);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(refine_full_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d, C,
+                       w.k, w.L, w.labels);
+    KM_LAUNCH_CHECK();
+}
+
+// objective of the current labels against C; optional per-row d64 output
+template <typename F>
+static double* objective(km_ctx* ctx, work<F>& w, const F* C, double* d64_out) {
+    const int64_t nchunks = (w.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+    double* chunks = ctx->obj_chunks.get<double>(nchunks);
+    double* out = ctx->obj_dev.get<double>(1);
+    const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
+    hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, w.X, w.n,
+                       w.d, C, w.labels, d64_out, chunks);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sum_sequential_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks, nchunks, out);
+    KM_LAUNCH_CHECK();
+    return out;
+}
+
+template <typename F>
+static void launch_reduce(km_ctx* ctx, work<F>& w, int32_t* perm, int64_t* offsets,
+                          int32_t* chunk_base, int32_t* meta, double* part) {
+    const int64_t max_chunks = w.n / REDUCE_CHUNK + w.k + 1;
+    const int grid = grid_for(ctx, max_chunks * 64, 256, 8);
+    const int cpl = w.d <= 64 ? 1 : (w.d + 63) / 64;
+#define KM_RED(C_)                                                                                  \
+    hipLaunchKernelGGL((reduce_chunks_kernel<F, C_>), dim3(grid), dim3(256), 0, ctx->stream, w.X,   \
+                       w.d, w.k, perm, offsets, chunk_base, meta, part)
+    if (cpl <= 1) KM_RED(1);
+    else if (cpl <= 2) KM_RED(2);
+    else if (cpl <= 4) KM_RED(4);
+    else if (cpl <= 8) KM_RED(8);
+    else if (cpl <= 16) KM_RED(16);
+    else if (cpl <= 32) KM_RED(32);
+    else if (cpl <= 64) KM_RED(64);
+    else throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
+#undef KM_RED
+    KM_LAUNCH_CHECK();
+}
+
+// stable grouping of row ids by label
+static void sort_by_label(km_ctx* ctx, const int32_t* labels, int64_t n, int k, int32_t* perm) {
+    unsigned bits = 1;
+    while ((1ull << bits) < (unsigned long long)k) bits++;
+    int32_t* keys_out = ctx->keys_out.get<int32_t>(n);
+    rocprim::counting_iterator<int32_t> ids(0);
+    size_t tmp_bytes = 0;
+    KM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, labels, keys_out, ids, perm,
+                                     (unsigned int)n, 0, bits, ctx->stream));
+    void* tmp = ctx->sort_tmp.get<char>(tmp_bytes);
+    KM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, labels, keys_out, ids, perm, (unsigned int)n, 0,
+                                     bits, ctx->stream));
+}
+
+// Refill empty clusters: the m rows with the largest d64 (ties: lowest row) go,
+// in that order, to the empty clusters in ascending index order.  Returns the
+// objective correction (sum of their d64, subtracted in order by the caller).
+template <typename F>
+static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m,
+                         int32_t* empty_list, double* obj_host, bool have_obj) {
+    double* d64 = ctx->d64.get<double>(w.n);
+    double* objp = objective<F>(ctx, w, Ccur, d64);
+    // radix select of the m-th largest key, 8 bits at a time from the top
+    unsigned int* hist = ctx->hist.get<unsigned int>(256);
+    unsigned long long prefix = 0, mask = 0;
+    int64_t remaining = m;
+    std::vector<unsigned int> h(256);
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        fill<unsigned int>(ctx, hist, 256, 0u);
+        const int grid = grid_for(ctx, w.n, 256, 4);
+        hipLaunchKernelGGL(digit_hist_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, w.n,
+                           prefix, mask, shift, hist);
+        KM_LAUNCH_CHECK();
+        KM_HIP(hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        int64_t cum = 0;
+        int digit = 0;
+        for (int dg = 255; dg >= 0; --dg) {
+            if (cum + h[dg] >= remaining) {
+                digit = dg;
+                remaining -= cum;
+                break;
+            }
+            cum += h[dg];
+        }
+        prefix |= (unsigned long long)digit << shift;
+        mask |= 0xFFull << shift;
+    }
+    // prefix = key of the m-th largest; `remaining` rows equal to it are taken
+    int32_t* gt = ctx->cand_gt.get<int32_t>(m);
+    int32_t* eq = ctx->cand_eq.get<int32_t>(w.n);
+    int32_t* cc = ctx->cand_counters.get<int32_t>(2);
+    fill<int32_t>(ctx, cc, 2, 0);
+    const int grid = grid_for(ctx, w.n, 256, 4);
+    hipLaunchKernelGGL(collect_candidates_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, w.n,
+                       prefix, gt, eq, cc);
+    KM_LAUNCH_CHECK();
+    int32_t counts[2];
+    KM_HIP(hipMemcpyAsync(counts, cc, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t n_gt = counts[0], n_eq = counts[1];
+    if (n_gt + remaining != m || remaining > n_eq)
+        throw status_error(KM_INTERNAL_ERROR, "empty-cluster candidate selection is inconsistent");
+    std::vector<int32_t> vgt(n_gt), veq(n_eq);
+    if (n_gt) KM_HIP(hipMemcpyAsync(vgt.data(), gt, n_gt * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (n_eq) KM_HIP(hipMemcpyAsync(veq.data(), eq, n_eq * 4, hipMemcpyDeviceToHost, ctx->stream));
+    double* gt_d = ctx->obj_chunks.get<double>(std::max<int64_t>(n_gt, (w.n + OBJ_CHUNK - 1) / OBJ_CHUNK));
+    std::vector<double> dgt(n_gt);
+    if (n_gt) {
+        hipLaunchKernelGGL(gather_f64_kernel, dim3(grid_for(ctx, n_gt, 256, 1)), dim3(256), 0,
+                           ctx->stream, d64, gt, n_gt, gt_d);
+        KM_LAUNCH_CHECK();
+        KM_HIP(hipMemcpyAsync(dgt.data(), gt_d, n_gt * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<std::pair<double, int32_t>> order(n_gt);
+    for (int64_t i = 0; i < n_gt; ++i) order[i] = {dgt[i], vgt[i]};
+    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
+        if (a.first != b.first) return a.first > b.first;
+        return a.second < b.second;
+    });
+    std::sort(veq.begin(), veq.end());
+    double thr_val;
+    {
+        const unsigned long long pk = prefix;
+        std::memcpy(&thr_val, &pk, sizeof(double));
+    }
+    std::vector<int32_t> cand(m);
+    std::vector<double> cand_d(m);
+    for (int64_t i = 0; i < n_gt; ++i) {
+        cand[i] = order[i].second;
+        cand_d[i] = order[i].first;
+    }
+    for (int64_t i = 0; i < remaining; ++i) {
+        cand[n_gt + i] = veq[i];
+        cand_d[n_gt + i] = thr_val;
+    }
+    int32_t* cand_dev = ctx->cand_dev.get<int32_t>(m);
+    KM_HIP(hipMemcpyAsync(cand_dev, cand.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+    const int fgrid = grid_for(ctx, int64_t(m) * w.d, 256, 4);
+    hipLaunchKernelGGL(fill_empty_kernel<F>, dim3(fgrid), dim3(256), 0, ctx->stream, w.X, w.d,
+                       empty_list, cand_dev, m, Cnext);
+    KM_LAUNCH_CHECK();
+    if (!have_obj) {
+        KM_HIP(hipMemcpyAsync(obj_host, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int e = 0; e < m; ++e) *obj_host = *obj_host - cand_d[e];
+    ctx->stats.empty_cluster_events++;
+}
+
+template <typename F>
+static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, int k) {
+    w.n = n;
+    w.d = d;
+    w.k = k;
+    w.dpad = pad_dim(d);
+    w.X = Xd;
+    w.xnorm = ctx->xnorm.get<float>(n);
+    w.labels = ctx->labels.get<int32_t>(n);
+    w.L.cand_rows = ctx->lists_rows.get<int32_t>(n);
+    w.L.cand_count = ctx->lists_cnt.get<int32_t>(n);
+    w.L.cand_ids = ctx->lists_ids.get<int32_t>(n * int64_t(MAXC));
+    w.L.full_rows = ctx->lists_full.get<int32_t>(n);
+    w.L.counters = ctx->lists_counters.get<int32_t>(2);
+    w.cn = ctx->cn.get<F>(k);
+    w.cpad = ctx->cpad.get<F>(int64_t(k) * w.dpad);
+    w.cmax_bits = ctx->cmax.get<unsigned long long>(1);
+    // every label stays a valid cluster id from here on
+    fill<int32_t>(ctx, w.labels, n, 0);
+    const int grid = grid_for(ctx, n * 64, 256, 8);
+    hipLaunchKernelGGL(row_norm_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, Xd, n, d, w.xnorm);
+    KM_LAUNCH_CHECK();
+}
+
+static void collect_refine_stats(km_ctx* ctx, const int32_t* host_counters) {
+    ctx->stats.refined_candidate_rows += host_counters[0];
+    ctx->stats.refined_full_rows += host_counters[1];
+}
+
+template <typename F>
+static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0, int64_t k64,
+                  int64_t max_iter, double thr, uint32_t flags, F* C_out, int32_t* labels_out,
+                  double* obj_out, int64_t* iters_out) {
+    const int d = int(d64_), k = int(k64);
+    const bool dev = flags & KM_FLAG_DEVICE_POINTERS;
+    ctx->stats = km_stats{};
+    const F* Xd = X;
+    if (!dev) {
+        F* xb = ctx->x_dev.get<F>(n * d);
+        KM_HIP(hipMemcpyAsync(xb, X, sizeof(F) * n * d, hipMemcpyHostToDevice, ctx->stream));
+        Xd = xb;
+    }
+    work<F> w;
+    setup_work<F>(ctx, w, Xd, n, d, k);
+    const int path = choose_path<F>(ctx, d);
+    ctx->stats.assign_path = path;
+    F* Ccur = ctx->c_cur.get<F>(int64_t(k) * d);
+    F* Cnext = ctx->c_next.get<F>(int64_t(k) * d);
+    KM_HIP(hipMemcpyAsync(Ccur, C0, sizeof(F) * k * d,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    int32_t* counts = ctx->counts.get<int32_t>(k);
+    int64_t* offsets = ctx->offsets.get<int64_t>(k + 1);
+    int32_t* chunk_base = ctx->chunk_base.get<int32_t>(k + 1);
+    int32_t* empty_list = ctx->empty_list.get<int32_t>(k);
+    int32_t* meta = ctx->meta.get<int32_t>(2);
+    int32_t* perm = ctx->perm.get<int32_t>(n);
+    double* part = ctx->part.get<double>((n / REDUCE_CHUNK + k + 1) * int64_t(d));
+    int32_t* pin = ctx->pinned;
+    const bool need_obj = thr > 0;
+
+    hipEvent_t loop0 = nullptr, loop1 = nullptr;
+    ctx->ev_used = 0;
+    if (ctx->timing) {
+        loop0 = ctx->event();
+        loop1 = ctx->event();
+        KM_HIP(hipEventRecord(loop0, ctx->stream));
+    }
+    double prev = double(std::numeric_limits<F>::max());
+    int64_t iter = 0;
+    for (iter = 0; iter < max_iter; ++iter) {
+        centroid_prep<F>(ctx, w, Ccur, path == 2);
+        assign<F>(ctx, w, Ccur, path);
+        fill<int32_t>(ctx, counts, k, 0);
+        {
+            const bool lds = k <= 16384;
+            const int grid = grid_for(ctx, n, 256, 4);
+            hipLaunchKernelGGL(count_kernel, dim3(grid), dim3(256), lds ? k * sizeof(int32_t) : 0,
+                               ctx->stream, w.labels, n, k, counts, lds ? 1 : 0);
+            KM_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(scan_clusters_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, k,
+                           offsets, chunk_base, empty_list, meta);
+        KM_LAUNCH_CHECK();
+        sort_by_label(ctx, w.labels, n, k, perm);
+        launch_reduce<F>(ctx, w, perm, offsets, chunk_base, meta, part);
+        {
+            const int grid = grid_for(ctx, int64_t(k) * d, 256, 4);
+            hipLaunchKernelGGL(finalize_centroids_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream,
+                               part, counts, chunk_base, k, d, Cnext);
+            KM_LAUNCH_CHECK();
+        }
+        double* objp = nullptr;
+        if (need_obj) objp = objective<F>(ctx, w, Ccur, nullptr);
+        KM_HIP(hipMemcpyAsync(pin, meta, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        if (need_obj)
+            KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        collect_refine_stats(ctx, pin + 2);
+        const int m = pin[0];
+        double obj = 0.0;
+        if (need_obj) std::memcpy(&obj, pin + 4, sizeof(double));
+        if (m > 0) handle_empty<F>(ctx, w, Ccur, Cnext, m, empty_list, &obj, need_obj);
+        std::swap(Ccur, Cnext);
+        if (need_obj) {
+            if (obj + thr > prev) {
+                iter++;
+                break;
+            }
+            prev = obj;
+        }
+    }
+    // final assignment with the final centroids and the objective
+    centroid_prep<F>(ctx, w, Ccur, path == 2);
+    assign<F>(ctx, w, Ccur, path);
+    double* objp = objective<F>(ctx, w, Ccur, nullptr);
+    if (ctx->timing) KM_HIP(hipEventRecord(loop1, ctx->stream));
+    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    const hipMemcpyKind out_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    KM_HIP(hipMemcpyAsync(C_out, Ccur, sizeof(F) * k * d, out_kind, ctx->stream));
+    KM_HIP(hipMemcpyAsync(labels_out, w.labels, sizeof(int32_t) * n, out_kind, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    collect_refine_stats(ctx, pin + 2);
+    double obj;
+    std::memcpy(&obj, pin + 4, sizeof(double));
+    if (obj_out) *obj_out = obj;
+    if (iters_out) *iters_out = iter;
+    if (ctx->timing) {
+        float ms = 0.f;
+        KM_HIP(hipEventElapsedTime(&ms, loop0, loop1));
+        ctx->stats.iterations_ms = ms;
+        double tot = 0.0;
+        for (size_t i = 2; i + 1 < ctx->ev_used; i += 2) {
+            float a = 0.f;
+            KM_HIP(hipEventElapsedTime(&a, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
+            tot += a;
+        }
+        ctx->stats.assign_kernel_ms = tot;
+    }
+}
+
+template <typename F>
+static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, int64_t k64,
+                  uint32_t flags, int32_t* labels_out, double* obj_out) {
+    const int d = int(d64_), k = int(k64);
+    const bool dev = flags & KM_FLAG_DEVICE_POINTERS;
+    ctx->stats = km_stats{};
+    ctx->ev_used = 0;
+    const F* Xd = X;
+    if (!dev) {
+        F* xb = ctx->x_dev.get<F>(n * d);
+        KM_HIP(hipMemcpyAsync(xb, X, sizeof(F) * n * d, hipMemcpyHostToDevice, ctx->stream));
+        Xd = xb;
+    }
+    work<F> w;
+    setup_work<F>(ctx, w, Xd, n, d, k);
+    const int path = choose_path<F>(ctx, d);
+    ctx->stats.assign_path = path;
+    F* Cd = ctx->c_cur.get<F>(int64_t(k) * d);
+    KM_HIP(hipMemcpyAsync(Cd, C, sizeof(F) * k * d,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    centroid_prep<F>(ctx, w, Cd, path == 2);
+    assign<F>(ctx, w, Cd, path);
+    int32_t* pin = ctx->pinned;
+    const bool want_obj = flags & KM_FLAG_EXACT_OBJECTIVE;
+    if (want_obj) {
+        double* objp = objective<F>(ctx, w, Cd, nullptr);
+        KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KM_HIP(hipMemcpyAsync(labels_out, w.labels, sizeof(int32_t) * n,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    collect_refine_stats(ctx, pin + 2);
+    if (want_obj && obj_out) std::memcpy(obj_out, pin + 4, sizeof(double));
+}
+
+// --- precondition checks (reference order: train_ops.hpp:40-61, common.cpp:82-115) ---
+static void check_train_args(const void* X, int64_t n, int64_t d, const void* C0, int64_t k,
+                             int64_t max_iter, double thr, const void* C_out, const void* labels) {
+    if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
+    if (k > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Cluster count is greater than max int32 value");
+    if (max_iter < 0) throw status_error(KM_DOMAIN_ERROR, "Max iteration count lower than zero");
+    if (thr < 0.0) throw status_error(KM_DOMAIN_ERROR, "Accuracy_threshold is lower than zero");
+    if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
+    if (n > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Row count is greater than max int32 value");
+    if (k > n) throw status_error(KM_INVALID_ARGUMENT, "Cluster count exceeds data row count");
+    if (C0 == nullptr)
+        throw status_error(KM_UNIMPLEMENTED,
+                           "Initial centroids are required: default k-means++ initialisation is "
+                           "not part of this engine");
+    if (C_out == nullptr || labels == nullptr)
+        throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
+}
+
+static void check_infer_args(const void* X, int64_t n, int64_t d, const void* C, int64_t k,
+                             uint32_t flags, const void* labels, const void* obj) {
+    if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
+    if (k > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Cluster count is greater than max int32 value");
+    if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
+    if (C == nullptr) throw status_error(KM_DOMAIN_ERROR, "Input model centroids are empty");
+    if (n > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Row count is greater than max int32 value");
+    if (labels == nullptr) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    if ((flags & KM_FLAG_EXACT_OBJECTIVE) && obj == nullptr)
+        throw status_error(KM_INVALID_ARGUMENT, "Objective output is required");
+    if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
+}
+
+template <typename Fn>
+static km_status guarded(km_ctx* ctx, Fn fn) {
+    if (!ctx) return KM_INVALID_ARGUMENT;
+    try {
+        KM_HIP(hipSetDevice(ctx->device));
+        fn();
+        ctx->last_error.clear();
+        return KM_OK;
+    } catch (const status_error& e) {
+        ctx->last_error = e.what();
+        if (e.status == KM_INTERNAL_ERROR || e.status == KM_DEVICE_BAD_ALLOC) {
+            // leave the stream idle so the context stays usable
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipGetLastError();
+        }
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "host allocation failed";
+        return KM_HOST_BAD_ALLOC;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return KM_INTERNAL_ERROR;
+    }
+}
+
+}  // namespace km
+
+extern "C" {
+
+km_status km_context_create(int device, km_ctx** out) {
+    if (!out) return KM_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return KM_INTERNAL_ERROR;
+    }
+    if (device < 0 || device >= count) return KM_INVALID_ARGUMENT;
+    km_ctx* ctx = new (std::nothrow) km_ctx();
+    if (!ctx) return KM_HOST_BAD_ALLOC;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return KM_INTERNAL_ERROR;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    void* pin = nullptr;
+    if (hipHostMalloc(&pin, 64, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return KM_HOST_BAD_ALLOC;
+    }
+    ctx->pinned = static_cast<int32_t*>(pin);
+    *out = ctx;
+    return KM_OK;
+}
+
+km_status km_context_destroy(km_ctx* ctx) {
+    if (!ctx) return KM_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    km::dbuf* bufs[] = {&ctx->x_dev, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
+                        &ctx->xnorm, &ctx->labels, &ctx->lists_rows, &ctx->lists_cnt,
+                        &ctx->lists_ids, &ctx->lists_full, &ctx->lists_counters, &ctx->counts,
+                        &ctx->offsets, &ctx->chunk_base, &ctx->empty_list, &ctx->meta, &ctx->perm,
+                        &ctx->keys_out, &ctx->sort_tmp, &ctx->part, &ctx->d64, &ctx->obj_chunks,
+                        &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
+                        &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev};
+    for (auto* b : bufs) b->release();
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KM_OK;
+}
+
+const char* km_last_error(const km_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+km_status km_context_set_option(km_ctx* ctx, int option, int64_t value) {
+    if (!ctx) return KM_INVALID_ARGUMENT;
+    switch (option) {
+        case KM_OPT_ASSIGN_PATH:
+            if (value < 0 || value > 2) return KM_INVALID_ARGUMENT;
+            ctx->assign_path_opt = int(value);
+            return KM_OK;
+        case KM_OPT_TIMING:
+            ctx->timing = value != 0;
+            return KM_OK;
+        default:
+            return KM_INVALID_ARGUMENT;
+    }
+}
+
+km_status km_get_stats(const km_ctx* ctx, km_stats* out) {
+    if (!ctx || !out) return KM_INVALID_ARGUMENT;
+    *out = ctx->stats;
+    return KM_OK;
+}
+
+km_status km_lloyd_train_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, const float* C0,
+                             int64_t k, int64_t max_iter, double acc_thr, uint32_t flags,
+                             float* C_out, int32_t* labels_out, double* obj_out,
+                             int64_t* iters_out) {
+    return km::guarded(ctx, [&] {
+        km::check_train_args(X, n, d, C0, k, max_iter, acc_thr, C_out, labels_out);
+        km::train<float>(ctx, X, n, d, C0, k, max_iter, acc_thr, flags, C_out, labels_out, obj_out,
+                         iters_out);
+    });
+}
+
+km_status km_lloyd_train_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C0,
+                             int64_t k, int64_t max_iter, double acc_thr, uint32_t flags,
+                             double* C_out, int32_t* labels_out, double* obj_out,
+                             int64_t* iters_out) {
+    return km::guarded(ctx, [&] {
+        km::check_train_args(X, n, d, C0, k, max_iter, acc_thr, C_out, labels_out);
+        km::train<double>(ctx, X, n, d, C0, k, max_iter, acc_thr, flags, C_out, labels_out,
+                          obj_out, iters_out);
+    });
+}
+
+km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, const float* C,
+                             int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out) {
+    return km::guarded(ctx, [&] {
+        km::check_infer_args(X, n, d, C, k, flags, labels_out, obj_out);
+        km::infer<float>(ctx, X, n, d, C, k, flags, labels_out, obj_out);
+    });
+}
+
+km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
+                             int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out) {
+    return km::guarded(ctx, [&] {
+        km::check_infer_args(X, n, d, C, k, flags, labels_out, obj_out);
+        km::infer<double>(ctx, X, n, d, C, k, flags, labels_out, obj_out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,370 @@
+// km_update.hpp -- centroid update, objective and empty-cluster kernels.
+//
+// Replaces, on the oneDAL side:
+//   count_clusters / count_empty_clusters   kernels_integral_dpc.cpp:45-149
+//   partial_reduce_centroids                kernels_fp_impl.hpp:327-376
+//   merge_reduce_centroids                  kernels_fp_impl.hpp:277-325
+//   compute_objective_function              kernels_fp_impl.hpp:378-400
+//   compute_squares (centroids)             kernels_fp_impl.hpp:402-414
+//   find_candidates / fill_empty_clusters   empty_cluster_handling_dpc.cpp:26-150, 402-455
+// The reference accumulates centroid sums in F with per-part copies walked by
+// 128 subgroups; here rows are grouped by label (stable, ascending row index),
+// each cluster's member list is cut into REDUCE_CHUNK chunks summed by one wave
+// in double, and the chunk sums are merged in chunk order.  The order is fixed,
+// so the result is identical run to run and identical to the oracle's
+// restatement of it.
+#pragma once
+
+#include "km_common.hpp"
+
+namespace km {
+
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ p, int64_t count, T value) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// row norms: ||x_i|| rounded up to float (used only by the error bound)
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void row_norm_kernel(const F* __restrict__ X, int64_t n, int d,
+                                                       float* __restrict__ norm_up) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
+    const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+    for (int64_t row = wave; row < n; row += nwaves) {
+        double s = 0.0;
+        for (int f = lane; f < d; f += 64) {
+            const double v = double(X[row * d + f]);
+            s = fma(v, v, s);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) {
+            const double r = sqrt(s) * (1.0 + 1e-12);
+            float f = float(r);
+            if (double(f) < r) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
+            norm_up[row] = f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// centroid preparation for one assignment pass: ||c_j||^2 in F (from a double
+// sum), max ||c_j|| (rounded up) and a zero-padded copy for the matrix-core
+// kernel.  One wave per centroid.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void centroid_prep_kernel(const F* __restrict__ C, int k, int d,
+                                                            int dpad, F* __restrict__ cn,
+                                                            F* __restrict__ cpad,
+                                                            unsigned long long* __restrict__ cmax_bits) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int j = wave; j < k; j += nwaves) {
+        double s = 0.0;
+        for (int f = lane; f < d; f += 64) {
+            const double v = double(C[int64_t(j) * d + f]);
+            s = fma(v, v, s);
+        }
+        if (cpad)
+            for (int f = lane; f < dpad; f += 64) cpad[int64_t(j) * dpad + f] = f < d ? C[int64_t(j) * d + f] : F(0);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) {
+            cn[j] = F(s);
+            const double r = sqrt(s) * (1.0 + 1e-12);
+            atomicMax(cmax_bits, (unsigned long long)__double_as_longlong(r));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// label histogram (count_clusters)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void count_kernel(const int32_t* __restrict__ labels, int64_t n,
+                                                    int k, int32_t* __restrict__ counts,
+                                                    int use_lds) {
+    extern __shared__ int32_t hist[];
+    if (use_lds) {
+        for (int j = threadIdx.x; j < k; j += blockDim.x) hist[j] = 0;
+        __syncthreads();
+    }
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int j = labels[i];
+        if (use_lds)
+            atomicAdd(&hist[j], 1);
+        else
+            atomicAdd(&counts[j], 1);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < k; j += blockDim.x)
+            if (hist[j]) atomicAdd(&counts[j], hist[j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// single-workgroup scan over clusters: member offsets, chunk offsets and the
+// ascending list of empty clusters.  meta[0] = #empty, meta[1] = #chunks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void scan_clusters_kernel(const int32_t* __restrict__ counts,
+                                                             int k, int64_t* __restrict__ offsets,
+                                                             int32_t* __restrict__ chunk_base,
+                                                             int32_t* __restrict__ empty_list,
+                                                             int32_t* __restrict__ meta) {
+    __shared__ int64_t s_off[1024];
+    __shared__ int32_t s_chk[1024];
+    __shared__ int32_t s_emp[1024];
+    __shared__ int64_t carry_off;
+    __shared__ int32_t carry_chk, carry_emp;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        carry_off = 0;
+        carry_chk = 0;
+        carry_emp = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < k; base += 1024) {
+        const int j = base + t;
+        const int32_t c = j < k ? counts[j] : 0;
+        s_off[t] = c;
+        s_chk[t] = j < k ? (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK : 0;
+        s_emp[t] = (j < k && c == 0) ? 1 : 0;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int64_t a = 0;
+            int32_t b = 0, e = 0;
+            if (t >= off) {
+                a = s_off[t - off];
+                b = s_chk[t - off];
+                e = s_emp[t - off];
+            }
+            __syncthreads();
+            s_off[t] += a;
+            s_chk[t] += b;
+            s_emp[t] += e;
+            __syncthreads();
+        }
+        if (j < k) {
+            const int64_t incl_off = s_off[t];
+            offsets[j] = carry_off + incl_off - c;
+            chunk_base[j] = carry_chk + s_chk[t] - (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK;
+            if (c == 0) empty_list[carry_emp + s_emp[t] - 1] = j;
+        }
+        __syncthreads();
+        if (t == 1023) {
+            carry_off += s_off[1023];
+            carry_chk += s_chk[1023];
+            carry_emp += s_emp[1023];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        offsets[k] = carry_off;
+        chunk_base[k] = carry_chk;
+        meta[0] = carry_emp;
+        meta[1] = carry_chk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chunk sums.  One wave per chunk of REDUCE_CHUNK members of one cluster.
+// G = reduce_groups(d) interleaved partial sums (member m of the chunk goes to
+// partial m % G); lanes g*d + f (d <= 32) or f, f+64, ... (d > 32) own columns.
+// CPL = columns per lane (d > 32 only).
+// ---------------------------------------------------------------------------
+template <typename F, int CPL>
+__global__ __launch_bounds__(256) void reduce_chunks_kernel(
+    const F* __restrict__ X, int d, int k, const int32_t* __restrict__ perm,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ chunk_base,
+    const int32_t* __restrict__ meta, double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int total = meta[1];
+    const int G = reduce_groups(d);
+    const bool small = d <= 32;
+    const int g = small ? lane / d : 0;
+    const int f0 = small ? lane % d : lane;
+    const bool active = small ? (g < G) : true;
+    for (int q = wave; q < total; q += nwaves) {
+        // cluster owning chunk q: last j with chunk_base[j] <= q
+        int lo = 0, hi = k;  // chunk_base[k] = total > q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_base[mid] <= q) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const int64_t m_begin = offsets[j] + int64_t(q - chunk_base[j]) * REDUCE_CHUNK;
+        const int64_t m_end = min(m_begin + REDUCE_CHUNK, offsets[j + 1]);
+        double acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c] = 0.0;
+        if (small) {
+            for (int64_t m = m_begin; m < m_end; m += G) {
+                const int64_t mm = m + g;
+                if (active && mm < m_end) {
+                    const int64_t row = perm[mm];
+                    acc[0] = acc[0] + double(X[row * d + f0]);
+                }
+            }
+            // partial g -> group 0, added in g order
+            double tot = acc[0];
+            for (int gg = 1; gg < G; ++gg) {
+                const double o = __shfl(acc[0], gg * d + f0);
+                tot = tot + o;
+            }
+            if (g == 0 && f0 < d) part[int64_t(q) * d + f0] = tot;
+        } else {
+            int64_t m = m_begin;
+            for (; m + 4 <= m_end; m += 4) {
+                const int64_t r0 = perm[m], r1 = perm[m + 1], r2 = perm[m + 2], r3 = perm[m + 3];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int f = f0 + 64 * c;
+                    if (f < d) {
+                        const double v0 = double(X[r0 * d + f]);
+                        const double v1 = double(X[r1 * d + f]);
+                        const double v2 = double(X[r2 * d + f]);
+                        const double v3 = double(X[r3 * d + f]);
+                        acc[c] = acc[c] + v0;
+                        acc[c] = acc[c] + v1;
+                        acc[c] = acc[c] + v2;
+                        acc[c] = acc[c] + v3;
+                    }
+                }
+            }
+            for (; m < m_end; ++m) {
+                const int64_t r0 = perm[m];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int f = f0 + 64 * c;
+                    if (f < d) acc[c] = acc[c] + double(X[r0 * d + f]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int f = f0 + 64 * c;
+                if (f < d) part[int64_t(q) * d + f] = acc[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// merge chunk sums per cluster (chunk order) and divide: C_j = (F)(S1_j / n_j).
+// Empty clusters are left for fill_empty_kernel.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void finalize_centroids_kernel(
+    const double* __restrict__ part, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ chunk_base, int k, int d, F* __restrict__ Cnext) {
+    const int64_t total = int64_t(k) * d;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int j = int(e / d), f = int(e % d);
+        const int32_t cnt = counts[j];
+        if (cnt == 0) continue;
+        double s = 0.0;
+        for (int q = chunk_base[j]; q < chunk_base[j + 1]; ++q) s = s + part[int64_t(q) * d + f];
+        Cnext[e] = F(s / double(cnt));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// objective: d64 per row against its label's centroid, summed per OBJ_CHUNK rows
+// in row order; optional per-row d64 output.  One workgroup per chunk.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void objective_chunks_kernel(
+    const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C,
+    const int32_t* __restrict__ labels, double* __restrict__ d64_out,
+    double* __restrict__ chunk_sums) {
+    __shared__ double vals[OBJ_CHUNK];
+    const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+    for (int64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
+        const int64_t r0 = q * OBJ_CHUNK;
+        const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
+        for (int t = threadIdx.x; t < rows; t += blockDim.x) {
+            const int64_t row = r0 + t;
+            const double v = dist64<F>(X + row * d, C + int64_t(labels[row]) * d, d);
+            vals[t] = v;
+            if (d64_out) d64_out[row] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int t = 0; t < rows; ++t) s = s + vals[t];
+            chunk_sums[q] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void sum_sequential_kernel(const double* __restrict__ v, int64_t count,
+                                      double* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int64_t i = 0; i < count; ++i) s = s + v[i];
+        *out = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// empty clusters: radix select of the m largest d64 (keys are the raw bits of
+// non-negative doubles, which order like the values).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void digit_hist_kernel(const double* __restrict__ d64, int64_t n,
+                                                         unsigned long long prefix,
+                                                         unsigned long long mask, int shift,
+                                                         unsigned int* __restrict__ hist) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void collect_candidates_kernel(
+    const double* __restrict__ d64, int64_t n, unsigned long long thr_key,
+    int32_t* __restrict__ gt_list, int32_t* __restrict__ eq_list, int32_t* __restrict__ counters) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+        if (key > thr_key) gt_list[atomicAdd(&counters[0], 1)] = int32_t(i);
+        else if (key == thr_key) eq_list[atomicAdd(&counters[1], 1)] = int32_t(i);
+    }
+}
+
+__global__ void gather_f64_kernel(const double* __restrict__ src, const int32_t* __restrict__ idx,
+                                  int64_t count, double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void fill_empty_kernel(const F* __restrict__ X, int d,
+                                                         const int32_t* __restrict__ empty_list,
+                                                         const int32_t* __restrict__ cand, int m,
+                                                         F* __restrict__ Cnext) {
+    const int64_t total = int64_t(m) * d;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int q = int(e / d), f = int(e % d);
+        Cnext[int64_t(empty_list[q]) * d + f] = X[int64_t(cand[q]) * d + f];
+    }
+}
+
+}  // namespace km

@@ -1,0 +1,250 @@
+"""ctypes binding of libkmeans_hip.so (include/onedal_amd/kmeans_hip.h).
+
+The Python layer is plumbing for tests and benchmarks: every computation runs
+in the HIP library.  Importing works without a GPU; creating an Engine needs
+the built library and a visible device and raises otherwise.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+KM_OK = 0
+KM_DOMAIN_ERROR = 1
+KM_INVALID_ARGUMENT = 2
+KM_DEVICE_BAD_ALLOC = 3
+KM_HOST_BAD_ALLOC = 4
+KM_UNIMPLEMENTED = 5
+KM_INTERNAL_ERROR = 6
+
+KM_FLAG_EXACT_OBJECTIVE = 0x1
+KM_FLAG_BF16_REFINE = 0x2
+KM_FLAG_DEVICE_POINTERS = 0x4
+
+KM_OPT_ASSIGN_PATH = 1
+KM_OPT_TIMING = 2
+
+PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
+
+EXPORTED_SYMBOLS = [
+    "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
+    "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
+    "km_lloyd_infer_f64",
+]
+
+
+class KmStats(ctypes.Structure):
+    _fields_ = [
+        ("assign_launches", ctypes.c_int64),
+        ("assign_kernel_ms", ctypes.c_double),
+        ("iterations_ms", ctypes.c_double),
+        ("refined_candidate_rows", ctypes.c_int64),
+        ("refined_full_rows", ctypes.c_int64),
+        ("empty_cluster_events", ctypes.c_int64),
+        ("assign_path", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("km status %d: %s" % (status, message))
+        self.status = status
+        self.message = message
+
+
+class DomainError(EngineError):
+    pass
+
+
+class InvalidArgument(EngineError):
+    pass
+
+
+class Unimplemented(EngineError):
+    pass
+
+
+_ERRORS = {KM_DOMAIN_ERROR: DomainError, KM_INVALID_ARGUMENT: InvalidArgument,
+           KM_UNIMPLEMENTED: Unimplemented}
+
+_lib = None
+
+
+def load_library():
+    """Load libkmeans_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libkmeans_hip.so is not built (%s); run python -m onedal_amd.build"
+                           % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P, I64, D, U32, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32, ctypes.c_int
+    L.km_context_create.argtypes = [I, ctypes.POINTER(P)]
+    L.km_context_create.restype = I
+    L.km_context_destroy.argtypes = [P]
+    L.km_context_destroy.restype = I
+    L.km_last_error.argtypes = [P]
+    L.km_last_error.restype = ctypes.c_char_p
+    L.km_context_set_option.argtypes = [P, I, I64]
+    L.km_context_set_option.restype = I
+    L.km_get_stats.argtypes = [P, ctypes.POINTER(KmStats)]
+    L.km_get_stats.restype = I
+    for sfx in ("f32", "f64"):
+        f = getattr(L, "km_lloyd_train_" + sfx)
+        f.argtypes = [P, P, I64, I64, P, I64, I64, D, U32, P, P, P, P]
+        f.restype = I
+        f = getattr(L, "km_lloyd_infer_" + sfx)
+        f.argtypes = [P, P, I64, I64, P, I64, U32, P, P]
+        f.restype = I
+    _lib = L
+    return L
+
+
+def _sfx(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return "f32"
+    if dtype == np.float64:
+        return "f64"
+    raise TypeError("k-means supports float32 and float64 data, got %s" % dtype)
+
+
+class _Dev:
+    """Device-pointer view of a torch CUDA tensor (duck-typed, no torch import)."""
+
+    def __init__(self, t):
+        self.ptr = t.data_ptr()
+        self.shape = tuple(t.shape)
+        self.dtype = np.dtype(str(t.dtype).replace("torch.", ""))
+
+
+def _is_torch_cuda(a):
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+class Engine:
+    """One HIP device context (km_ctx) and its stream."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        ctx = ctypes.c_void_p()
+        st = self.lib.km_context_create(int(device), ctypes.byref(ctx))
+        if st != KM_OK:
+            raise EngineError(st, "km_context_create(%d) failed (no usable HIP device?)" % device)
+        self.ctx = ctx
+
+    def close(self):
+        if self.ctx:
+            self.lib.km_context_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != KM_OK:
+            msg = self.lib.km_last_error(self.ctx).decode()
+            raise _ERRORS.get(st, EngineError)(st, msg)
+
+    def set_option(self, option, value):
+        self._check(self.lib.km_context_set_option(self.ctx, int(option), int(value)))
+
+    def stats(self):
+        s = KmStats()
+        self._check(self.lib.km_get_stats(self.ctx, ctypes.byref(s)))
+        return s.as_dict()
+
+    # -- host (numpy) entry points -------------------------------------------------
+    def train(self, x, c0, max_iter, accuracy_threshold=0.0, bf16=False):
+        """Host arrays in, host arrays out: (centroids, labels, objective, iterations)."""
+        if _is_torch_cuda(x):
+            return self.train_device(x, c0, max_iter, accuracy_threshold, bf16)
+        x = np.ascontiguousarray(x)
+        sfx = _sfx(x.dtype)
+        n, d = x.shape
+        c0 = np.ascontiguousarray(c0, dtype=x.dtype)
+        k = c0.shape[0]
+        c = np.empty((k, d), x.dtype)
+        labels = np.empty(n, np.int32)
+        obj = ctypes.c_double()
+        iters = ctypes.c_int64()
+        flags = KM_FLAG_BF16_REFINE if bf16 else 0
+        st = getattr(self.lib, "km_lloyd_train_" + sfx)(
+            self.ctx, x.ctypes.data, n, d, c0.ctypes.data, k, int(max_iter),
+            float(accuracy_threshold), flags, c.ctypes.data, labels.ctypes.data,
+            ctypes.byref(obj), ctypes.byref(iters))
+        self._check(st)
+        return c, labels, obj.value, iters.value
+
+    def infer(self, x, c, objective=False, bf16=False):
+        if _is_torch_cuda(x):
+            return self.infer_device(x, c, objective, bf16)
+        x = np.ascontiguousarray(x)
+        sfx = _sfx(x.dtype)
+        n, d = x.shape
+        c = np.ascontiguousarray(c, dtype=x.dtype)
+        labels = np.empty(n, np.int32)
+        obj = ctypes.c_double()
+        flags = (KM_FLAG_EXACT_OBJECTIVE if objective else 0) | (KM_FLAG_BF16_REFINE if bf16 else 0)
+        st = getattr(self.lib, "km_lloyd_infer_" + sfx)(
+            self.ctx, x.ctypes.data, n, d, c.ctypes.data, c.shape[0], flags, labels.ctypes.data,
+            ctypes.byref(obj))
+        self._check(st)
+        return labels, (obj.value if objective else None)
+
+    # -- device (torch CUDA tensor) entry points -----------------------------------
+    def train_device(self, x, c0, max_iter, accuracy_threshold=0.0, bf16=False, out=None):
+        """x, c0: contiguous CUDA tensors on this context's device.  Returns device
+        tensors (centroids, labels) plus objective and iteration count."""
+        import torch
+
+        assert x.is_contiguous() and c0.is_contiguous()
+        sfx = _sfx(str(x.dtype).replace("torch.", ""))
+        n, d = x.shape
+        k = c0.shape[0]
+        if out is None:
+            c = torch.empty((k, d), dtype=x.dtype, device=x.device)
+            labels = torch.empty(n, dtype=torch.int32, device=x.device)
+        else:
+            c, labels = out
+        obj = ctypes.c_double()
+        iters = ctypes.c_int64()
+        flags = KM_FLAG_DEVICE_POINTERS | (KM_FLAG_BF16_REFINE if bf16 else 0)
+        st = getattr(self.lib, "km_lloyd_train_" + sfx)(
+            self.ctx, x.data_ptr(), n, d, c0.data_ptr(), k, int(max_iter),
+            float(accuracy_threshold), flags, c.data_ptr(), labels.data_ptr(),
+            ctypes.byref(obj), ctypes.byref(iters))
+        self._check(st)
+        return c, labels, obj.value, iters.value
+
+    def infer_device(self, x, c, objective=False, bf16=False):
+        import torch
+
+        sfx = _sfx(str(x.dtype).replace("torch.", ""))
+        n, d = x.shape
+        labels = torch.empty(n, dtype=torch.int32, device=x.device)
+        obj = ctypes.c_double()
+        flags = KM_FLAG_DEVICE_POINTERS | (KM_FLAG_EXACT_OBJECTIVE if objective else 0) | (
+            KM_FLAG_BF16_REFINE if bf16 else 0)
+        st = getattr(self.lib, "km_lloyd_infer_" + sfx)(
+            self.ctx, x.data_ptr(), n, d, c.data_ptr(), c.shape[0], flags, labels.data_ptr(),
+            ctypes.byref(obj))
+        self._check(st)
+        return labels, (obj.value if objective else None)

@@ -1,0 +1,167 @@
+"""ctypes wrapper over the CPU oracle (oracle/kmeans_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product path
+(onedal_amd/) never imports this module.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libkmeans_oracle.so")
+
+# Order constants of the engine's FP64 reductions (must equal the ones in
+# onedal_amd/csrc/kmeans_config.hpp; the tests check that they do).
+REDUCE_CHUNK = 256
+OBJ_CHUNK = 4096
+
+
+def reduce_groups(d):
+    """Interleaved partial sums per chunk used by the engine for a given d."""
+    return max(1, 64 // d) if d <= 32 else 1
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    D = ctypes.c_double
+    I = ctypes.c_int
+    for sfx in ("f32", "f64"):
+        f = getattr(L, "oracle_daal_lloyd_" + sfx)
+        f.argtypes = [P, I64, I64, P, I64, I64, D, I, I64, I, P, P, P, P]
+        f.restype = I
+        f = getattr(L, "oracle_canonical_lloyd_" + sfx)
+        f.argtypes = [P, I64, I64, P, I64, I64, D, I64, I64, I64, I, P, P, P, P, P]
+        f.restype = I
+        f = getattr(L, "oracle_canonical_assign_" + sfx)
+        f.argtypes = [P, I64, I64, P, I64, P, P, I]
+        f.restype = None
+        f = getattr(L, "oracle_canonical_objective_" + sfx)
+        f.argtypes = [P, I64, I64, P, P, I64, I]
+        f.restype = D
+        f = getattr(L, "oracle_canonical_sums_" + sfx)
+        f.argtypes = [P, I64, I64, I64, P, I64, I64, P, P]
+        f.restype = None
+        f = getattr(L, "oracle_fp64_assign_" + sfx)
+        f.argtypes = [P, I64, I64, P, I64, P, P, P, I]
+        f.restype = None
+
+
+def _sfx(x):
+    if x.dtype == np.float32:
+        return "f32"
+    if x.dtype == np.float64:
+        return "f64"
+    raise TypeError("oracle supports float32/float64, got %s" % x.dtype)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _threads(nthreads):
+    return int(nthreads if nthreads else min(8, os.cpu_count() or 1))
+
+
+def daal_lloyd(x, c0, max_iter, thr=0.0, stop_rule=0, block_size=512, nthreads=None):
+    """Reference (DAAL) CPU Lloyd restatement. stop_rule: 0 DAAL shift, 1 GPU objective, 2 fixed."""
+    x = np.ascontiguousarray(x)
+    c0 = np.ascontiguousarray(c0, dtype=x.dtype)
+    n, d = x.shape
+    k = c0.shape[0]
+    c = np.empty_like(c0)
+    labels = np.empty(n, np.int32)
+    obj = ctypes.c_double()
+    iters = ctypes.c_int64()
+    st = getattr(lib(), "oracle_daal_lloyd_" + _sfx(x))(
+        _ptr(x), n, d, _ptr(c0), k, int(max_iter), float(thr), int(stop_rule), int(block_size),
+        _threads(nthreads), _ptr(c), _ptr(labels), ctypes.byref(obj), ctypes.byref(iters))
+    if st != 0:
+        raise RuntimeError("oracle: number of clusters is too large for the candidates")
+    return c, labels, obj.value, iters.value
+
+
+def canonical_lloyd(x, c0, max_iter, thr=0.0, nthreads=None):
+    """The engine's arithmetic contract restated on the CPU (bit-exact target)."""
+    x = np.ascontiguousarray(x)
+    c0 = np.ascontiguousarray(c0, dtype=x.dtype)
+    n, d = x.shape
+    k = c0.shape[0]
+    c = np.empty_like(c0)
+    labels = np.empty(n, np.int32)
+    obj = ctypes.c_double()
+    iters = ctypes.c_int64()
+    iter_obj = np.zeros(max(1, int(max_iter)), np.float64)
+    st = getattr(lib(), "oracle_canonical_lloyd_" + _sfx(x))(
+        _ptr(x), n, d, _ptr(c0), k, int(max_iter), float(thr), REDUCE_CHUNK, reduce_groups(d),
+        OBJ_CHUNK, _threads(nthreads), _ptr(c), _ptr(labels), ctypes.byref(obj),
+        ctypes.byref(iters), _ptr(iter_obj))
+    if st != 0:
+        raise RuntimeError("oracle: more empty clusters than points")
+    return c, labels, obj.value, iters.value, iter_obj[: iters.value]
+
+
+def canonical_assign(x, c, nthreads=None):
+    x = np.ascontiguousarray(x)
+    c = np.ascontiguousarray(c, dtype=x.dtype)
+    n, d = x.shape
+    labels = np.empty(n, np.int32)
+    best = np.empty(n, x.dtype)
+    getattr(lib(), "oracle_canonical_assign_" + _sfx(x))(
+        _ptr(x), n, d, _ptr(c), c.shape[0], _ptr(labels), _ptr(best), _threads(nthreads))
+    return labels, best
+
+
+def canonical_objective(x, c, labels, nthreads=None):
+    x = np.ascontiguousarray(x)
+    c = np.ascontiguousarray(c, dtype=x.dtype)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    n, d = x.shape
+    return getattr(lib(), "oracle_canonical_objective_" + _sfx(x))(
+        _ptr(x), n, d, _ptr(c), _ptr(labels), OBJ_CHUNK, _threads(nthreads))
+
+
+def canonical_sums(x, labels, k):
+    x = np.ascontiguousarray(x)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    n, d = x.shape
+    s1 = np.empty((k, d), np.float64)
+    counts = np.empty(k, np.int64)
+    getattr(lib(), "oracle_canonical_sums_" + _sfx(x))(
+        _ptr(x), n, d, k, _ptr(labels), REDUCE_CHUNK, reduce_groups(d), _ptr(s1), _ptr(counts))
+    return s1, counts
+
+
+def fp64_assign(x, c, nthreads=None):
+    """Labels from exact-in-double distances plus the best and second-best distance."""
+    x = np.ascontiguousarray(x)
+    c = np.ascontiguousarray(c, dtype=x.dtype)
+    n, d = x.shape
+    labels = np.empty(n, np.int32)
+    best = np.empty(n, np.float64)
+    second = np.empty(n, np.float64)
+    getattr(lib(), "oracle_fp64_assign_" + _sfx(x))(
+        _ptr(x), n, d, _ptr(c), c.shape[0], _ptr(labels), _ptr(best), _ptr(second),
+        _threads(nthreads))
+    return labels, best, second
