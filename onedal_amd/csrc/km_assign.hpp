@@ -45,11 +45,12 @@ struct top2 {
     }
 };
 
-// Settle one row from its slots (see the file comment).
+// Settle one row from its slots (see the file comment).  X = ||x|| rounded up,
+// C = max_j ||c_j|| rounded up.
 template <typename F>
 __device__ __forceinline__ void decide_row(int64_t row, const top2<F>* slots, int nslots,
-                                           double band, int32_t* __restrict__ labels,
-                                           const refine_lists& L) {
+                                           const band_consts& bc, double X, double C,
+                                           int32_t* __restrict__ labels, const refine_lists& L) {
     F best = fp_traits<F>::inf;
     int bi = -1;
     for (int s = 0; s < nslots; ++s) {
@@ -58,8 +59,7 @@ __device__ __forceinline__ void decide_row(int64_t row, const top2<F>* slots, in
             bi = slots[s].i1;
         }
     }
-    // 4.44e-16*|best| covers the rounding of this sum in double
-    const double T = double(best) + band + fabs(double(best)) * 4.44e-16;
+    const double T = bi >= 0 ? band_threshold(bc, X, C, double(best)) : 0.0;
     bool cert = (bi >= 0) && (T < double(fp_traits<F>::inf));
     int cnt = 0;
     int cand[MAXC];
@@ -163,8 +163,7 @@ __global__ __launch_bounds__(256) void assign_generic_kernel(
         if (tid < TN) {
             const int64_t row = n0 + tid;
             if (row < n) {
-                const double band = band_of(bc, double(xnorm_up[row]), cmax);
-                decide_row<F>(row, slots[tid], 16, band, labels, L);
+                decide_row<F>(row, slots[tid], 16, bc, double(xnorm_up[row]), cmax, labels, L);
             }
         }
         __syncthreads();
@@ -173,31 +172,54 @@ __global__ __launch_bounds__(256) void assign_generic_kernel(
 
 // ---------------------------------------------------------------------------
 // FP32 matrix-core kernel: v_mfma_f32_32x32x2_f32, d <= DP (DP in {32,64,128,256}).
+// Rows are read as float4 with row stride ldx (a multiple of 4 floats, 16-byte
+// aligned base; columns d..ldx-1 must be zero).
 //
 // Workgroup = 4 waves; each wave owns NB blocks of 32 rows, held in VGPRs for
-// the whole pass (B operand), and streams every centroid through LDS in blocks
-// of 32 (A operand).  Lane l = 32*h + r: h selects the feature half
-// [h*DP/2, (h+1)*DP/2) so one MFMA step s multiplies feature s of half 0 and
-// feature DP/2+s of half 1; the 32x32 f32 accumulator of block (M, N) puts
-// D[m][n] with n = l % 32, m = 8*(i/4) + 4*h + (i%4) in register i.  A lane
-// therefore sees 16 of the 32 centroids of each block for a single row: each
-// row has two slots (h = 0, 1).
+// the whole pass (B operand; the next tile's rows are prefetched into a second
+// register set), and reads centroids from LDS in blocks of 32 (A operand).
+// Lane l = 32*h + r: h selects the feature half [h*DP/2, (h+1)*DP/2), so MFMA
+// step s multiplies feature s of half 0 and feature DP/2+s of half 1.  The
+// 32x32 accumulator puts D[m][n] (m = centroid, n = row) at lane n + 32*h',
+// register i, with m = 8*(i/4) + 4*h' + (i%4): a lane sees 16 of the 32
+// centroids of a block for one row.  Each lane keeps two slots (even / odd
+// centroid blocks), so a row has four slots.
+//
+// RES: all centroids stay in LDS for the whole launch (loaded once per
+// workgroup).  Otherwise blocks of 32 centroids are double-buffered through
+// LDS, the next block prefetched into registers while the current one runs.
 // ---------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int DP, int NB, bool VEC>
-__global__ __launch_bounds__(256) void assign_mfma_f32_kernel(
-    const float* __restrict__ X, int64_t n, int d, const float* __restrict__ Cpad, int k,
+template <int DP>
+struct mfma_f32_layout {
+    static constexpr int HALF = DP / 2;
+    static constexpr int CST = DP + 4;  // LDS row stride in floats
+};
+
+// LDS bytes needed by assign_mfma_f32_kernel
+__host__ inline size_t mfma_f32_lds_bytes(int dp, int k, bool res) {
+    const int cst = dp + 4;
+    if (res) {
+        const int kp = (k + 31) / 32 * 32;
+        return size_t(kp) * cst * 4 + size_t(kp) * 4;
+    }
+    return size_t(2) * 32 * cst * 4 + 2 * 32 * 4;
+}
+
+template <int DP, int NB, bool RES>
+__global__ __launch_bounds__(256, DP >= 256 ? 1 : 2) void assign_mfma_f32_kernel(
+    const float* __restrict__ X, int64_t ldx, int64_t n, int d, const float* __restrict__ Cpad, int k,
     const float* __restrict__ cn, const float* __restrict__ xnorm_up,
     const double* __restrict__ cmax_p, band_consts bc, int32_t* __restrict__ labels,
     refine_lists L) {
     constexpr int HALF = DP / 2;
-    constexpr int CST = DP + 4;            // LDS row stride (floats)
+    constexpr int CST = DP + 4;
     constexpr int ROWS_PER_WAVE = 32 * NB;
     constexpr int ROWS_PER_WG = 4 * ROWS_PER_WAVE;
-    __shared__ __align__(16) float Cs[2][32 * CST];
-    __shared__ __align__(16) float Cn[2][32];
+    constexpr int F4_PER_ROW = DP / 4;
+    extern __shared__ __align__(16) float smem[];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -206,69 +228,106 @@ __global__ __launch_bounds__(256) void assign_mfma_f32_kernel(
     const int h = lane >> 5;
     const double cmax = *cmax_p;
     const int mblocks = (k + 31) / 32;
+    const int kp = mblocks * 32;
     const int64_t ntiles = (n + ROWS_PER_WG - 1) / ROWS_PER_WG;
 
-    // stage centroid block mb (32 x DP floats, zero padded rows past k) into buffer b
-    auto stage = [&](int mb, int b) {
-        constexpr int F4_PER_ROW = DP / 4;
-        constexpr int TOTAL = 32 * F4_PER_ROW;
-        for (int e = tid; e < TOTAL; e += 256) {
+    float* Cs = smem;                                   // RES: kp rows; else 2 x 32 rows
+    float* Cn = smem + (RES ? kp : 64) * CST;           // RES: kp; else 2 x 32
+
+    if (RES) {
+        for (int e = tid; e < kp * F4_PER_ROW; e += 256) {
+            const int row = e / F4_PER_ROW, c4 = e % F4_PER_ROW;
+            floatx4 v = {0.f, 0.f, 0.f, 0.f};
+            if (row < k) v = *reinterpret_cast<const floatx4*>(Cpad + int64_t(row) * DP + c4 * 4);
+            *reinterpret_cast<floatx4*>(&Cs[row * CST + c4 * 4]) = v;
+        }
+        for (int j = tid; j < kp; j += 256) Cn[j] = j < k ? cn[j] : fp_traits<float>::inf;
+        __syncthreads();
+    }
+
+    // streamed centroid block: each thread moves STG float4 of a 32 x DP block
+    constexpr int STG = (32 * F4_PER_ROW + 255) / 256;
+    floatx4 pre[STG];
+    float pre_cn = 0.f;
+    auto prefetch = [&](int mb) {
+#pragma unroll
+        for (int q = 0; q < STG; ++q) {
+            const int e = tid + q * 256;
             const int row = e / F4_PER_ROW, c4 = e % F4_PER_ROW;
             const int cj = mb * 32 + row;
             floatx4 v = {0.f, 0.f, 0.f, 0.f};
-            if (cj < k) v = *reinterpret_cast<const floatx4*>(Cpad + int64_t(cj) * DP + c4 * 4);
-            *reinterpret_cast<floatx4*>(&Cs[b][row * CST + c4 * 4]) = v;
+            if (e < 32 * F4_PER_ROW && cj < k)
+                v = *reinterpret_cast<const floatx4*>(Cpad + int64_t(cj) * DP + c4 * 4);
+            pre[q] = v;
         }
-        if (tid < 32) {
-            const int cj = mb * 32 + tid;
-            Cn[b][tid] = cj < k ? cn[cj] : fp_traits<float>::inf;
+        if (tid < 32) pre_cn = (mb * 32 + tid < k) ? cn[mb * 32 + tid] : fp_traits<float>::inf;
+    };
+    auto commit = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < STG; ++q) {
+            const int e = tid + q * 256;
+            if (e < 32 * F4_PER_ROW) {
+                const int row = e / F4_PER_ROW, c4 = e % F4_PER_ROW;
+                *reinterpret_cast<floatx4*>(&Cs[(b * 32 + row) * CST + c4 * 4]) = pre[q];
+            }
         }
+        if (tid < 32) Cn[b * 32 + tid] = pre_cn;
     };
 
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // rows of a tile into registers
+    auto load_rows = [&](int64_t tile, float (&dst)[NB][HALF]) {
         const int64_t wrow0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
-        // B operand: this lane's half-row of each of its NB rows
-        float xr[NB][HALF];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t row = wrow0 + nb * 32 + r32;
-            const bool ok = row < n;
-            const float* src = X + (ok ? row : 0) * int64_t(d);
-            if (VEC) {
+            const bool ok = tile < ntiles && row < n;
+            const float* src = X + (ok ? row : 0) * ldx;
 #pragma unroll
-                for (int s = 0; s < HALF; s += 4) {
-                    const int f = h * HALF + s;
-                    floatx4 v = {0.f, 0.f, 0.f, 0.f};
-                    if (ok && f < d) v = *reinterpret_cast<const floatx4*>(src + f);
-                    xr[nb][s] = v[0];
-                    xr[nb][s + 1] = v[1];
-                    xr[nb][s + 2] = v[2];
-                    xr[nb][s + 3] = v[3];
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < HALF; ++s) {
-                    const int f = h * HALF + s;
-                    xr[nb][s] = (ok && f < d) ? src[f] : 0.f;
-                }
+            for (int s = 0; s < HALF; s += 4) {
+                const int f = h * HALF + s;
+                floatx4 v = {0.f, 0.f, 0.f, 0.f};
+                if (ok && f < d) v = *reinterpret_cast<const floatx4*>(src + f);
+                dst[nb][s] = v[0];
+                dst[nb][s + 1] = v[1];
+                dst[nb][s + 2] = v[2];
+                dst[nb][s + 3] = v[3];
             }
         }
-        top2<float> st[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) st[nb].init();
+    };
 
-        __syncthreads();
-        stage(0, 0);
-        __syncthreads();
-        for (int mb = 0; mb < mblocks; ++mb) {
-            const int b = mb & 1;
-            if (mb + 1 < mblocks) stage(mb + 1, b ^ 1);
+    // prefetch the next tile's rows only when both register sets fit
+    constexpr bool PREF = DP <= 64 && NB * HALF <= 64;
+    float xr[NB][HALF];
+    float xn[NB][HALF];
+    int64_t tile = blockIdx.x;
+    if (PREF) load_rows(tile, xn);
+
+    for (; tile < ntiles; tile += gridDim.x) {
+        if (PREF) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int s = 0; s < HALF; ++s) xr[nb][s] = xn[nb][s];
+            load_rows(tile + gridDim.x, xn);  // in flight during this tile
+        } else {
+            load_rows(tile, xr);
+        }
+
+        top2<float> st[NB][2];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            st[nb][0].init();
+            st[nb][1].init();
+        }
+
+        // one block of 32 centroids from LDS rows [lrow0, lrow0+32), global ids mb*32.., slot sl
+        auto block = [&](int lrow0, const float* cnb, int mb, int sl) {
             floatx16 acc[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                 for (int i = 0; i < 16; ++i) acc[nb][i] = 0.f;
-            const float* arow = &Cs[b][r32 * CST + h * HALF];
+            const float* arow = &Cs[(lrow0 + r32) * CST + h * HALF];
 #pragma unroll
             for (int s = 0; s < HALF; s += 4) {
                 const floatx4 a4 = *reinterpret_cast<const floatx4*>(arow + s);
@@ -279,34 +338,65 @@ __global__ __launch_bounds__(256) void assign_mfma_f32_kernel(
                         acc[nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q], xr[nb][s + q],
                                                                        acc[nb], 0, 0, 0);
             }
-            // epilogue: rows m = 8*(i/4) + 4*h + (i%4) of this centroid block
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const floatx4 c4 = *reinterpret_cast<const floatx4*>(&Cn[b][8 * q + 4 * h]);
+                const floatx4 c4 = *reinterpret_cast<const floatx4*>(cnb + 8 * q + 4 * h);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int cj = mb * 32 + 8 * q + 4 * h + t;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
-                        st[nb].push(__builtin_fmaf(-2.f, acc[nb][4 * q + t], c4[t]), cj);
+                        st[nb][sl].push(__builtin_fmaf(-2.f, acc[nb][4 * q + t], c4[t]), cj);
                 }
             }
+        };
+
+        if (RES) {
+            int mb = 0;
+            for (; mb + 1 < mblocks; mb += 2) {
+                block(mb * 32, Cn + mb * 32, mb, 0);
+                block(mb * 32 + 32, Cn + mb * 32 + 32, mb + 1, 1);
+            }
+            if (mb < mblocks) block(mb * 32, Cn + mb * 32, mb, 0);
+        } else {
             __syncthreads();
+            prefetch(0);
+            commit(0);
+            __syncthreads();
+            int mb = 0;
+            for (; mb + 1 < mblocks; mb += 2) {
+                prefetch(mb + 1);
+                block(0, Cn, mb, 0);
+                commit(1);
+                __syncthreads();
+                if (mb + 2 < mblocks) prefetch(mb + 2);
+                block(32, Cn + 32, mb + 1, 1);
+                if (mb + 2 < mblocks) commit(0);
+                __syncthreads();
+            }
+            if (mb < mblocks) {
+                block(0, Cn, mb, 0);
+                __syncthreads();
+            }
         }
-        // settle rows: lane r32 (h = 0) merges its slot with lane r32 + 32
+
+        // settle rows: lane r32 (h = 0) merges its two slots with lane r32 + 32's two
+        const int64_t wrow0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            top2<float> sl[2];
-            sl[0] = st[nb];
-            sl[1].m1 = __shfl_xor(st[nb].m1, 32);
-            sl[1].m2 = __shfl_xor(st[nb].m2, 32);
-            sl[1].i1 = __shfl_xor(st[nb].i1, 32);
-            sl[1].i2 = __shfl_xor(st[nb].i2, 32);
-            const int64_t row = wrow0 + nb * 32 + r32;
-            if (h == 0 && row < n) {
-                const double band = band_of(bc, double(xnorm_up[row]), cmax);
-                decide_row<float>(row, sl, 2, band, labels, L);
+            top2<float> sl[4];
+            sl[0] = st[nb][0];
+            sl[1] = st[nb][1];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                sl[2 + q].m1 = __shfl_xor(st[nb][q].m1, 32);
+                sl[2 + q].m2 = __shfl_xor(st[nb][q].m2, 32);
+                sl[2 + q].i1 = __shfl_xor(st[nb][q].i1, 32);
+                sl[2 + q].i2 = __shfl_xor(st[nb][q].i2, 32);
             }
+            const int64_t row = wrow0 + nb * 32 + r32;
+            if (h == 0 && row < n)
+                decide_row<float>(row, sl, 4, bc, double(xnorm_up[row]), cmax, labels, L);
         }
     }
 }

@@ -11,8 +11,8 @@
 //     inside the bound.
 //   * centroid sums in double, in a fixed order (REDUCE_CHUNK, reduce_groups);
 //     C_j = (F)(S1_j / count_j).
-//   * objective = sum_i d64_i in row order (OBJ_CHUNK), d64 = sum_f
-//     ((double)x_f - (double)c_f)^2.
+//   * objective = sum_i d64_i in a fixed order (objective_chunks_kernel), d64 =
+//     sum_f ((double)x_f - (double)c_f)^2.
 // All code here is compiled with -ffp-contract=off: fused multiply-adds appear
 // only where fma() is written out.
 #pragma once
@@ -69,13 +69,15 @@ struct centroid_stats {
 };
 
 // Error-bound constants for one assignment launch (host computed).
-//   |v_j - (||c_j||^2 - 2 x.c_j)| <= a_xc * ||x|| * cmax + a_cc * cmax^2
-//   |D_j - ||x - c_j||^2|         <= g_canon * (||x|| + cmax)^2
-// band(x) = 2 * (a_xc*X*C + a_cc*C^2 + g_canon*(X+C)^2) * (1 + 2^-20)
+//   |v_j - V_j| <= da = a_xc * ||x|| * cmax + a_cc * cmax^2,  V_j = ||c_j||^2 - 2 x.c_j
+//   |D_j - dist_j| <= g * dist_j,  g = gamma_{d+2}  (D: exact-form distance in F)
+// If D_j is minimal then dist_j <= dist_i (1+g)/(1-g) for the approximate argmin
+// i, so v_j <= v_i + 2 da + 2 g' dist_i with g' = g/(1-g) and
+// dist_i <= v_i + da + ||x||^2.  band = 2 da + 2 g' (v_i + da + ||x||^2).
 struct band_consts {
     double a_xc;
     double a_cc;
-    double g_canon;
+    double g_canon;   // g' = g / (1 - g)
 };
 
 // gamma_m = m*u / (1 - m*u)
@@ -105,10 +107,16 @@ __device__ __forceinline__ double dist64(const F* __restrict__ x, const F* __res
     return acc;
 }
 
-__device__ __forceinline__ double band_of(const band_consts& b, double X, double C) {
-    const double s = X + C;
-    const double v = b.a_xc * X * C + b.a_cc * C * C + b.g_canon * s * s;
-    return 2.0 * v * (1.0 + 9.5367431640625e-07);
+// Threshold T for a row: every centroid that can be the exact-form argmin has
+// approximate value <= T.  X = ||x|| (rounded up), C = max ||c_j|| (rounded up),
+// m1 = smallest approximate value of the row.  The relative and absolute slack
+// terms cover the rounding of this computation in double.
+__device__ __forceinline__ double band_threshold(const band_consts& b, double X, double C,
+                                                 double m1) {
+    const double da = b.a_xc * X * C + b.a_cc * C * C;
+    const double dist_up = fmax(0.0, m1 + da + X * X);
+    const double band = (2.0 * da + 2.0 * b.g_canon * dist_up) * (1.0 + 9.5367431640625e-07);
+    return m1 + band + fabs(m1) * 4.44e-16 + (X * X + C * C) * 4.44e-16;
 }
 
 }  // namespace km

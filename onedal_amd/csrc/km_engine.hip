@@ -94,7 +94,7 @@ struct km_ctx {
     bool timing = false;
 
     // workspace
-    km::dbuf x_dev, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
+    km::dbuf x_dev, x_pad, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
         c_out_dev;
@@ -134,6 +134,8 @@ struct work {
     int d = 0, k = 0, dpad = 0;
     // device views
     const F* X = nullptr;
+    const F* Xv = nullptr;    // rows with stride ldx (multiple of 4, 16-byte aligned) for float4 reads
+    int64_t ldx = 0;
     float* xnorm = nullptr;
     int32_t* labels = nullptr;
     refine_lists L{};
@@ -164,7 +166,8 @@ static band_consts make_band(int d) {
     const double ga = gamma_m(double(d), ua);
     b.a_xc = 2.0 * ga + 2.01 * u;
     b.a_cc = 2.01 * u + gamma_m(double(d), std::is_same<F, double>::value ? ua : 2.220446049250313e-16);
-    b.g_canon = gamma_m(double(d + 2), u);
+    const double g = gamma_m(double(d + 2), u);
+    b.g_canon = g / (1.0 - g);
     return b;
 }
 
@@ -178,21 +181,27 @@ static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, bool need_pad) {
     KM_LAUNCH_CHECK();
 }
 
+template <int DP, int NB, bool RES>
+static void launch_mfma_f32_mode(km_ctx* ctx, work<float>& w, const band_consts& bc, int grid,
+                                 size_t lds) {
+    auto kern = assign_mfma_f32_kernel<DP, NB, RES>;
+    KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, w.Xv, w.ldx, w.n, w.d, w.cpad,
+                       w.k, w.cn, w.xnorm, (const double*)w.cmax_bits, bc, w.labels, w.L);
+    KM_LAUNCH_CHECK();
+}
+
 template <int DP, int NB>
 static void launch_mfma_f32(km_ctx* ctx, work<float>& w, const band_consts& bc) {
     constexpr int ROWS_PER_WG = 4 * 32 * NB;
     const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
     const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
-    const bool vec = (w.d % 4) == 0 && (reinterpret_cast<uintptr_t>(w.X) % 16) == 0;
-    if (vec)
-        hipLaunchKernelGGL((assign_mfma_f32_kernel<DP, NB, true>), dim3(grid), dim3(256), 0,
-                           ctx->stream, w.X, w.n, w.d, w.cpad, w.k, w.cn, w.xnorm,
-                           (const double*)w.cmax_bits, bc, w.labels, w.L);
+    // centroids resident in LDS when two workgroups still fit a CU (160 KB)
+    const size_t res_lds = mfma_f32_lds_bytes(DP, w.k, true);
+    if (res_lds <= 78 * 1024)
+        launch_mfma_f32_mode<DP, NB, true>(ctx, w, bc, grid, res_lds);
     else
-        hipLaunchKernelGGL((assign_mfma_f32_kernel<DP, NB, false>), dim3(grid), dim3(256), 0,
-                           ctx->stream, w.X, w.n, w.d, w.cpad, w.k, w.cn, w.xnorm,
-                           (const double*)w.cmax_bits, bc, w.labels, w.L);
-    KM_LAUNCH_CHECK();
+        launch_mfma_f32_mode<DP, NB, false>(ctx, w, bc, grid, mfma_f32_lds_bytes(DP, w.k, false));
 }
 
 // assignment of every row against w.X/C (centroid_prep already done for C)
@@ -387,7 +396,8 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
 }
 
 template <typename F>
-static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, int k) {
+static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, int k,
+                       bool vec_view) {
     w.n = n;
     w.d = d;
     w.k = k;
@@ -403,6 +413,18 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
     w.cn = ctx->cn.get<F>(k);
     w.cpad = ctx->cpad.get<F>(int64_t(k) * w.dpad);
     w.cmax_bits = ctx->cmax.get<unsigned long long>(1);
+    // float4-readable view of X for the matrix-core kernels
+    w.Xv = Xd;
+    w.ldx = d;
+    if (vec_view && ((d % 4) != 0 || (reinterpret_cast<uintptr_t>(Xd) % 16) != 0)) {
+        const int64_t ldx = (d + 3) / 4 * 4;
+        F* xp = ctx->x_pad.get<F>(n * ldx);
+        const int g = grid_for(ctx, n * ldx, 256, 8);
+        hipLaunchKernelGGL(pad_rows_kernel<F>, dim3(g), dim3(256), 0, ctx->stream, Xd, n, d, ldx, xp);
+        KM_LAUNCH_CHECK();
+        w.Xv = xp;
+        w.ldx = ldx;
+    }
     // every label stays a valid cluster id from here on
     fill<int32_t>(ctx, w.labels, n, 0);
     const int grid = grid_for(ctx, n * 64, 256, 8);
@@ -429,8 +451,8 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
         Xd = xb;
     }
     work<F> w;
-    setup_work<F>(ctx, w, Xd, n, d, k);
     const int path = choose_path<F>(ctx, d);
+    setup_work<F>(ctx, w, Xd, n, d, k, path == 2);
     ctx->stats.assign_path = path;
     F* Ccur = ctx->c_cur.get<F>(int64_t(k) * d);
     F* Cnext = ctx->c_next.get<F>(int64_t(k) * d);
@@ -544,8 +566,8 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
         Xd = xb;
     }
     work<F> w;
-    setup_work<F>(ctx, w, Xd, n, d, k);
     const int path = choose_path<F>(ctx, d);
+    setup_work<F>(ctx, w, Xd, n, d, k, path == 2);
     ctx->stats.assign_path = path;
     F* Cd = ctx->c_cur.get<F>(int64_t(k) * d);
     KM_HIP(hipMemcpyAsync(Cd, C, sizeof(F) * k * d,
@@ -665,7 +687,7 @@ km_status km_context_destroy(km_ctx* ctx) {
     if (!ctx) return KM_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    km::dbuf* bufs[] = {&ctx->x_dev, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
+    km::dbuf* bufs[] = {&ctx->x_dev, &ctx->x_pad, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
                         &ctx->xnorm, &ctx->labels, &ctx->lists_rows, &ctx->lists_cnt,
                         &ctx->lists_ids, &ctx->lists_full, &ctx->lists_counters, &ctx->counts,
                         &ctx->offsets, &ctx->chunk_base, &ctx->empty_list, &ctx->meta, &ctx->perm,

@@ -26,6 +26,19 @@ __global__ void fill_kernel(T* __restrict__ p, int64_t count, T value) {
         p[i] = value;
 }
 
+// copy rows into a zero-padded matrix with row stride ldx (for float4 row reads)
+template <typename F>
+__global__ void pad_rows_kernel(const F* __restrict__ X, int64_t n, int d, int64_t ldx,
+                                F* __restrict__ out) {
+    const int64_t total = n * ldx;
+    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t r = e / ldx;
+        const int c = int(e - r * ldx);
+        out[e] = c < d ? X[r * d + c] : F(0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // row norms: ||x_i|| rounded up to float (used only by the error bound)
 // ---------------------------------------------------------------------------
@@ -223,20 +236,19 @@ __global__ __launch_bounds__(256) void reduce_chunks_kernel(
             if (g == 0 && f0 < d) part[int64_t(q) * d + f0] = tot;
         } else {
             int64_t m = m_begin;
-            for (; m + 4 <= m_end; m += 4) {
-                const int64_t r0 = perm[m], r1 = perm[m + 1], r2 = perm[m + 2], r3 = perm[m + 3];
+            for (; m + 8 <= m_end; m += 8) {
+                int64_t r[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) r[u] = perm[m + u];
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const int f = f0 + 64 * c;
                     if (f < d) {
-                        const double v0 = double(X[r0 * d + f]);
-                        const double v1 = double(X[r1 * d + f]);
-                        const double v2 = double(X[r2 * d + f]);
-                        const double v3 = double(X[r3 * d + f]);
-                        acc[c] = acc[c] + v0;
-                        acc[c] = acc[c] + v1;
-                        acc[c] = acc[c] + v2;
-                        acc[c] = acc[c] + v3;
+                        double v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) v[u] = double(X[r[u] * d + f]);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc[c] = acc[c] + v[u];
                     }
                 }
             }
@@ -272,37 +284,76 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
         const int32_t cnt = counts[j];
         if (cnt == 0) continue;
         double s = 0.0;
-        for (int q = chunk_base[j]; q < chunk_base[j + 1]; ++q) s = s + part[int64_t(q) * d + f];
+        int q = chunk_base[j];
+        const int q1 = chunk_base[j + 1];
+        // loads issued ahead, adds kept in chunk order
+        for (; q + 8 <= q1; q += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[int64_t(q + u) * d + f];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s = s + v[u];
+        }
+        for (; q < q1; ++q) s = s + part[int64_t(q) * d + f];
         Cnext[e] = F(s / double(cnt));
     }
 }
 
 // ---------------------------------------------------------------------------
-// objective: d64 per row against its label's centroid, summed per OBJ_CHUNK rows
-// in row order; optional per-row d64 output.  One workgroup per chunk.
+// objective: d64 per row against its label's centroid.  Order (fixed): rows in
+// chunks of OBJ_CHUNK; thread t of a chunk sums rows t, t+256, t+512, ... in
+// order; the 256 partials are combined by the tree p[t] += p[t+s], s = 128..1;
+// chunk sums are added in chunk order by sum_chunks_kernel.  Rows are staged
+// through LDS (coalesced) before each thread walks its row's features in order.
 // ---------------------------------------------------------------------------
 template <typename F>
 __global__ __launch_bounds__(256) void objective_chunks_kernel(
     const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C,
     const int32_t* __restrict__ labels, double* __restrict__ d64_out,
     double* __restrict__ chunk_sums) {
-    __shared__ double vals[OBJ_CHUNK];
+    constexpr int STAGE_BYTES = 32768;
+    __shared__ __align__(16) unsigned char stage_raw[STAGE_BYTES];
+    __shared__ double part[256];
+    F* xs = reinterpret_cast<F*>(stage_raw);
+    const int ld = d + 1;
+    const int rows_sub = max(1, min(256, int(STAGE_BYTES / sizeof(F)) / ld));
+    const int t = threadIdx.x;
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     for (int64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
         const int64_t r0 = q * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
-        for (int t = threadIdx.x; t < rows; t += blockDim.x) {
-            const int64_t row = r0 + t;
-            const double v = dist64<F>(X + row * d, C + int64_t(labels[row]) * d, d);
-            vals[t] = v;
-            if (d64_out) d64_out[row] = v;
+        double p = 0.0;
+        // rows t + 256*i are visited in increasing i; a sub-tile holds rows_sub rows
+        for (int s0 = 0; s0 < rows; s0 += rows_sub) {
+            const int sr = min(rows_sub, rows - s0);
+            __syncthreads();
+            for (int e = t; e < sr * d; e += 256) {
+                const int r = e / d, f = e - (e / d) * d;
+                xs[r * ld + f] = X[(r0 + s0 + r) * d + f];
+            }
+            __syncthreads();
+            // thread t owns chunk rows congruent to t mod 256
+            // the one row of this sub-tile with (s0 + r) % 256 == t, if any
+            for (int r = (t - (s0 & 255) + 256) & 255; r < sr; r += 256) {
+                const int64_t row = r0 + s0 + r;
+                const F* c = C + int64_t(labels[row]) * d;
+                double acc = 0.0;
+                for (int f = 0; f < d; ++f) {
+                    const double diff = double(xs[r * ld + f]) - double(c[f]);
+                    const double sq = diff * diff;
+                    acc = acc + sq;
+                }
+                if (d64_out) d64_out[row] = acc;
+                p = p + acc;
+            }
         }
+        part[t] = p;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int t = 0; t < rows; ++t) s = s + vals[t];
-            chunk_sums[q] = s;
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) part[t] = part[t] + part[t + s];
+            __syncthreads();
         }
+        if (t == 0) chunk_sums[q] = part[0];
         __syncthreads();
     }
 }
@@ -311,7 +362,15 @@ __global__ void sum_sequential_kernel(const double* __restrict__ v, int64_t coun
                                       double* __restrict__ out) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double s = 0.0;
-        for (int64_t i = 0; i < count; ++i) s = s + v[i];
+        int64_t i = 0;
+        for (; i + 16 <= count; i += 16) {
+            double w[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) w[u] = v[i + u];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s = s + w[u];
+        }
+        for (; i < count; ++i) s = s + v[i];
         *out = s;
     }
 }

@@ -47,8 +47,9 @@
 //        d64 (ties: lowest row index) in that order; the iteration objective drops by
 //        their d64.  Same policy as the reference
 //        (empty_cluster_handling_dpc.cpp:26-75, 402-455; kmeans_lloyd_batch_impl.i:169-186).
-//      - objective: sum of d64_i in row order, cut into chunks of OBJ_CH rows; chunk
-//        sums are added in chunk order.
+//      - objective: rows cut into chunks of OBJ_CH; in a chunk, partial t sums rows
+//        t, t+256, ... in order, the 256 partials are combined by the tree
+//        p[t] += p[t+s] (s = 128..1); chunk sums are added in chunk order.
 //      - loop and stop rule of the oneDAL GPU driver
 //        (train_kernel_lloyd_dense_dpc.cpp:147-196): per-iteration objective is the
 //        pre-update objective plus the empty-cluster correction; a final assignment
@@ -391,7 +392,9 @@ void canonical_assign(const F* x, int64_t n, int64_t d, const F* c, int64_t k, i
     });
 }
 
-// objective in the engine's order: d64 summed per OBJ_CH-row chunk, chunks in order
+// objective in the engine's order: rows in chunks of obj_ch; partial t of a chunk
+// sums rows t, t+256, ... in order; the 256 partials are combined by the tree
+// p[t] += p[t+s] for s = 128, 64, ..., 1; chunk sums are added in chunk order.
 template <typename F>
 double canonical_objective(const F* x, int64_t n, int64_t d, const F* c, const int32_t* labels,
                            int64_t obj_ch, std::vector<double>* d64_out, int nthreads) {
@@ -400,11 +403,17 @@ double canonical_objective(const F* x, int64_t n, int64_t d, const F* c, const i
         for (int64_t i = b; i < e; ++i) d64[i] = dist64(x + i * d, c + int64_t(labels[i]) * d, d);
     });
     double total = 0.0;
+    double part[256];
     for (int64_t r0 = 0; r0 < n; r0 += obj_ch) {
-        double part = 0.0;
         const int64_t r1 = std::min(n, r0 + obj_ch);
-        for (int64_t i = r0; i < r1; ++i) part = part + d64[i];
-        total = total + part;
+        for (int t = 0; t < 256; ++t) {
+            double p = 0.0;
+            for (int64_t i = r0 + t; i < r1; i += 256) p = p + d64[i];
+            part[t] = p;
+        }
+        for (int s = 128; s > 0; s >>= 1)
+            for (int t = 0; t < s; ++t) part[t] = part[t] + part[t + s];
+        total = total + part[0];
     }
     if (d64_out) *d64_out = std::move(d64);
     return total;
