@@ -20,6 +20,11 @@ METRIC = ("k-means point-iters/s + %roofline, bf16+refine vs fp32 path; p50/p99 
 PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 
 WORKLOADS = {
+    # Precision form: n=2e7, d=128, k=1024, 1024 anisotropic Gaussians (per-dimension
+    # scale log-uniform in [0.1,10], centers U[-50,50]^d), random-row init, 20 iterations,
+    # BF16 matrix-core distances with exact FP32 re-score, FP64 centroid/objective sums
+    "precision_c3": dict(n=20_000_000, d=128, k=1024, iters=20, center_range=50.0, sigma=None,
+                         anisotropic=True, bf16=True),
     # Plain FP32 Lloyd (drop-in descriptor path): n=1e7, d=64, k=256, 256 Gaussians,
     # centers U[-20,20]^d, per-cluster sigma U[0.5,2], random-row init, 20 iterations
     "fp32_c2": dict(n=10_000_000, d=64, k=256, iters=20, center_range=20.0, sigma=(0.5, 2.0),
@@ -84,7 +89,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fp32_c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="precision_c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP32-path and FP64-path comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -120,8 +127,9 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         stats.append(eng.stats())
-        log("step %d: %.3f s, iterations %d, objective %.6e, refined rows %d/%d" % (
-            s, times[-1], it, obj, stats[-1]["refined_candidate_rows"], stats[-1]["refined_full_rows"]))
+        log("step %d: %.3f s, iterations %d, objective %.9e, re-scored rows cand/collect/full %d/%d/%d" % (
+            s, times[-1], it, obj, stats[-1]["refined_candidate_rows"],
+            stats[-1]["refined_collect_rows"], stats[-1]["refined_full_rows"]))
     elapsed = time.perf_counter() - t_all
     pit = n * iters * args.steps / elapsed
 
@@ -135,6 +143,40 @@ def main():
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": None,
             "kernel": "assign (fused distance GEMM + argmin)", "avg_launch_ms": round(avg_ms, 4)}
+
+    compare = None
+    if wl["bf16"] and not args.no_compare:
+        # same data and init on the plain FP32 path: throughput ratio and label identity
+        log("fp32 path on the same workload ...")
+        c32 = torch.empty_like(c_out)
+        l32 = torch.empty_like(l_out)
+        t0 = time.perf_counter()
+        _, _, obj32, _ = eng.train_device(x, c0, iters, 0.0, bf16=False, out=(c32, l32))
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t0
+        st32 = eng.stats()
+        same = bool(torch.equal(l32, l_out)) and bool(torch.equal(c32, c_out))
+        compare = {"fp32_path_value": n * iters / t32, "fp32_path_ms_per_step": 1e3 * t32,
+                   "speedup_vs_fp32": (n * iters / t32 and pit / (n * iters / t32)),
+                   "labels_and_centroids_identical_to_fp32": same,
+                   "fp32_assign_avg_launch_ms": st32["assign_kernel_ms"] / max(1, st32["assign_launches"]),
+                   "objective_bf16": obj, "objective_fp32": obj32}
+        del c32, l32
+        # FP64 Lloyd (double data, same initial rows) for the inertia error of the precision form
+        log("fp64 path on the same workload ...")
+        xd = x.double()
+        cd = c0.double()
+        cdo = torch.empty_like(cd)
+        ldo = torch.empty_like(l_out)
+        t0 = time.perf_counter()
+        _, _, obj64, _ = eng.train_device(xd, cd, iters, 0.0, out=(cdo, ldo))
+        torch.cuda.synchronize()
+        compare["fp64_path_ms_per_step"] = 1e3 * (time.perf_counter() - t0)
+        compare["objective_fp64"] = obj64
+        compare["inertia_rel_err_vs_fp64"] = abs(obj - obj64) / obj64
+        compare["label_agreement_vs_fp64"] = float((ldo == l_out).float().mean().item())
+        del xd, cd, cdo, ldo
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -153,7 +195,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16+f32" if wl["bf16"] else "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "d": d, "k": k, "iterations": iters,
                    "path": "bf16+refine" if wl["bf16"] else "fp32",
@@ -161,7 +203,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "loop_ms_per_step": sum(s["iterations_ms"] for s in stats) / len(stats),
-        "refined_rows_per_step": stats[-1]["refined_candidate_rows"] + stats[-1]["refined_full_rows"],
+        "refined_rows_per_step": {k_: stats[-1][k_] for k_ in ("refined_candidate_rows",
+                                                                 "refined_collect_rows",
+                                                                 "refined_full_rows")},
+        "comparison": compare,
     }
     print(json.dumps(out), flush=True)
 

@@ -48,8 +48,10 @@ typedef enum {
 /* call flags */
 #define KM_FLAG_EXACT_OBJECTIVE 0x1u /* infer: also return the objective
                                         (result_options::compute_exact_objective_function) */
-#define KM_FLAG_BF16_REFINE 0x2u     /* distances on BF16 matrix cores, near-ties
-                                        re-scored in the input precision */
+#define KM_FLAG_BF16_REFINE 0x2u     /* float data, d <= 256: distances on BF16 matrix
+                                        cores, near-ties re-scored in float; labels are
+                                        identical to the FP32 path.  Ignored for double
+                                        data and d > 256. */
 #define KM_FLAG_DEVICE_POINTERS 0x4u /* X, C0/C and every output are device pointers
                                         on the context's device */
 
@@ -66,6 +68,7 @@ typedef struct {
     int64_t empty_cluster_events;   /* iterations that refilled empty clusters */
     int32_t assign_path;            /* path used: 1 generic, 2 FP32 MFMA, 3 BF16 MFMA */
     int32_t reserved;
+    int64_t refined_collect_rows;   /* BF16 path: rows whose candidates were re-collected */
 } km_stats;
 
 /* Create a context on HIP device `device` with its own stream. */

@@ -54,13 +54,21 @@ struct fp_traits<double> {
     static constexpr double inf = __builtin_inf();
 };
 
+// Candidate ids gathered per row by the BF16 collect pass.
+constexpr int MAXC2 = 16;
+
 // Device-side work lists filled by the assignment kernels.
 struct refine_lists {
     int32_t* cand_rows;    // rows with 2..MAXC certified candidates
     int32_t* cand_count;   // number of candidates per entry
     int32_t* cand_ids;     // MAXC ids per entry
     int32_t* full_rows;    // rows re-scored against every centroid
-    int32_t* counters;     // [0] = #cand entries, [1] = #full entries
+    int32_t* counters;     // [0] #cand entries, [1] #full entries, [2] #collect entries
+    // BF16 path: rows whose slots could not certify the candidate set
+    int32_t* coll_rows;    // row ids
+    double* coll_T;        // their thresholds (shifted value domain)
+    int32_t* coll_count;   // ids gathered per entry (may exceed MAXC2: overflow)
+    int32_t* coll_ids;     // MAXC2 ids per entry
 };
 
 // Per-iteration centroid statistics used by the error bound.

@@ -31,6 +31,7 @@
 
 #include "../../include/onedal_amd/kmeans_hip.h"
 #include "km_assign.hpp"
+#include "km_assign_bf16.hpp"
 #include "km_common.hpp"
 #include "km_refine.hpp"
 #include "km_update.hpp"
@@ -94,6 +95,7 @@ struct km_ctx {
     bool timing = false;
 
     // workspace
+    km::dbuf x_bf16, enorm, xmax, c_bf16, params, coll_rows, coll_T, coll_count, coll_ids;
     km::dbuf x_dev, x_pad, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
@@ -141,7 +143,13 @@ struct work {
     refine_lists L{};
     F* cn = nullptr;
     F* cpad = nullptr;
-    unsigned long long* cmax_bits = nullptr;
+    unsigned long long* cmax_bits = nullptr;  // [0] max ||c|| ([1], [2]: BF16 path)
+    // BF16 path
+    uint16_t* Xb = nullptr;       // n x dpad bf16 rows
+    float* enorm = nullptr;       // ||x - bf16(x)|| (up)
+    unsigned long long* xmax_bits = nullptr;
+    uint16_t* Cb = nullptr;       // k x dpad bf16(-2 c)
+    bf16_params* params = nullptr;
 };
 
 static int pad_dim(int d) {
@@ -150,12 +158,15 @@ static int pad_dim(int d) {
     return p;
 }
 
+// 1 generic tile kernel, 2 FP32 matrix cores, 3 BF16 matrix cores + exact re-score.
+// BF16 applies to float data with d <= 256; double data always takes path 1.
 template <typename F>
-static int choose_path(const km_ctx* ctx, int d) {
+static int choose_path(const km_ctx* ctx, int d, uint32_t flags) {
     if (std::is_same<F, double>::value) return 1;
+    if (d > 256) return 1;
+    if (flags & KM_FLAG_BF16_REFINE) return 3;
     if (ctx->assign_path_opt == 1) return 1;
-    if (d <= 256) return 2;
-    return 1;
+    return 2;
 }
 
 template <typename F>
@@ -171,9 +182,36 @@ static band_consts make_band(int d) {
     return b;
 }
 
+static bf16_consts make_bf16_consts(int d, int k) {
+    bf16_consts c;
+    int mbits = 0;
+    while ((1 << mbits) < (k + 31) / 32) mbits++;
+    c.ib = 4 + mbits;
+    c.u = fp_traits<float>::u;
+    c.g_acc = gamma_m(double(d + 1), fp_traits<float>::u_acc);
+    c.trunc = std::ldexp(1.0, c.ib - 23);
+    const double g = gamma_m(double(d + 2), fp_traits<float>::u);
+    c.g_canon = g / (1.0 - g);
+    return c;
+}
+
 // per-centroid prep: ||c||^2, padded copy, max ||c||
 template <typename F>
-static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, bool need_pad) {
+static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, int path) {
+    if constexpr (std::is_same<F, float>::value) {
+        if (path == 3) {
+            fill<unsigned long long>(ctx, w.cmax_bits, 3, 0ull);
+            const int grid = grid_for(ctx, int64_t(w.k) * 64, 256, 4);
+            hipLaunchKernelGGL(centroid_prep_bf16_kernel, dim3(grid), dim3(256), 0, ctx->stream, C,
+                               w.k, w.d, w.dpad, w.Cb, w.cn, w.cmax_bits);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(bf16_params_kernel, dim3(1), dim3(64), 0, ctx->stream, w.cmax_bits,
+                               w.xmax_bits, make_bf16_consts(w.d, w.k), w.params);
+            KM_LAUNCH_CHECK();
+            return;
+        }
+    }
+    const bool need_pad = path == 2;
     fill<unsigned long long>(ctx, w.cmax_bits, 1, 0ull);
     const int grid = grid_for(ctx, int64_t(w.k) * 64, 256, 4);
     hipLaunchKernelGGL(centroid_prep_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, C, w.k, w.d,
@@ -204,10 +242,31 @@ static void launch_mfma_f32(km_ctx* ctx, work<float>& w, const band_consts& bc) 
         launch_mfma_f32_mode<DP, NB, false>(ctx, w, bc, grid, mfma_f32_lds_bytes(DP, w.k, false));
 }
 
+template <int DPB>
+static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
+    constexpr int NB = DPB >= 256 ? 1 : 2;  // row registers of two blocks fit below 256 VGPRs
+    constexpr int ROWS_PER_WG = 4 * 32 * NB;
+    const size_t lds = bf16_lds_bytes(DPB);
+    const bf16_consts kc = make_bf16_consts(w.d, w.k);
+    if (!collect) {
+        const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
+        const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
+        hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, false>), dim3(grid), dim3(256), lds,
+                           ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cn, w.xnorm, w.enorm, w.params, kc,
+                           w.labels, w.L);
+    } else {
+        // row count is on the device; a fixed grid strides over it
+        hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, true>), dim3(ctx->cu_count * 2), dim3(256),
+                           lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cn, w.xnorm, w.enorm, w.params,
+                           kc, w.labels, w.L);
+    }
+    KM_LAUNCH_CHECK();
+}
+
 // assignment of every row against w.X/C (centroid_prep already done for C)
 template <typename F>
 static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
-    fill<int32_t>(ctx, w.L.counters, 2, 0);
+    fill<int32_t>(ctx, w.L.counters, 3, 0);
     const band_consts bc = make_band<F>(w.d);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -222,6 +281,14 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
                 case 64: launch_mfma_f32<64, 2>(ctx, w, bc); break;
                 case 128: launch_mfma_f32<128, 1>(ctx, w, bc); break;
                 default: launch_mfma_f32<256, 1>(ctx, w, bc); break;
+            }
+        }
+        if (path == 3) {
+            switch (w.dpad) {
+                case 32: launch_bf16<32>(ctx, w, false); break;
+                case 64: launch_bf16<64>(ctx, w, false); break;
+                case 128: launch_bf16<128>(ctx, w, false); break;
+                default: launch_bf16<256>(ctx, w, false); break;
             }
         }
     }
@@ -240,6 +307,19 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
     hipLaunchKernelGGL(refine_candidates_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d,
                        C, w.L, w.labels);
     KM_LAUNCH_CHECK();
+    if constexpr (std::is_same<F, float>::value) {
+        if (path == 3) {
+            switch (w.dpad) {
+                case 32: launch_bf16<32>(ctx, w, true); break;
+                case 64: launch_bf16<64>(ctx, w, true); break;
+                case 128: launch_bf16<128>(ctx, w, true); break;
+                default: launch_bf16<256>(ctx, w, true); break;
+            }
+            hipLaunchKernelGGL(refine_collected_kernel, dim3(rgrid), dim3(256), 0, ctx->stream, w.X,
+                               w.d, C, w.L, w.labels);
+            KM_LAUNCH_CHECK();
+        }
+    }
     hipLaunchKernelGGL(refine_full_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d, C,
                        w.k, w.L, w.labels);
     KM_LAUNCH_CHECK();
@@ -397,7 +477,8 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
 
 template <typename F>
 static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, int k,
-                       bool vec_view) {
+                       int path) {
+    const bool vec_view = path == 2;
     w.n = n;
     w.d = d;
     w.k = k;
@@ -409,10 +490,10 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
     w.L.cand_count = ctx->lists_cnt.get<int32_t>(n);
     w.L.cand_ids = ctx->lists_ids.get<int32_t>(n * int64_t(MAXC));
     w.L.full_rows = ctx->lists_full.get<int32_t>(n);
-    w.L.counters = ctx->lists_counters.get<int32_t>(2);
+    w.L.counters = ctx->lists_counters.get<int32_t>(3);
     w.cn = ctx->cn.get<F>(k);
     w.cpad = ctx->cpad.get<F>(int64_t(k) * w.dpad);
-    w.cmax_bits = ctx->cmax.get<unsigned long long>(1);
+    w.cmax_bits = ctx->cmax.get<unsigned long long>(4);
     // float4-readable view of X for the matrix-core kernels
     w.Xv = Xd;
     w.ldx = d;
@@ -428,6 +509,24 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
     // every label stays a valid cluster id from here on
     fill<int32_t>(ctx, w.labels, n, 0);
     const int grid = grid_for(ctx, n * 64, 256, 8);
+    if constexpr (std::is_same<F, float>::value) {
+        if (path == 3) {
+            w.Xb = ctx->x_bf16.get<uint16_t>(n * int64_t(w.dpad));
+            w.enorm = ctx->enorm.get<float>(n);
+            w.xmax_bits = ctx->xmax.get<unsigned long long>(2);
+            w.Cb = ctx->c_bf16.get<uint16_t>(int64_t(k) * w.dpad);
+            w.params = ctx->params.get<bf16_params>(1);
+            w.L.coll_rows = ctx->coll_rows.get<int32_t>(n);
+            w.L.coll_T = ctx->coll_T.get<double>(n);
+            w.L.coll_count = ctx->coll_count.get<int32_t>(n);
+            w.L.coll_ids = ctx->coll_ids.get<int32_t>(n * int64_t(MAXC2));
+            fill<unsigned long long>(ctx, w.xmax_bits, 2, 0ull);
+            hipLaunchKernelGGL(convert_rows_bf16_kernel, dim3(grid), dim3(256), 0, ctx->stream, Xd, n,
+                               d, w.dpad, w.Xb, w.xnorm, w.enorm, w.xmax_bits);
+            KM_LAUNCH_CHECK();
+            return;
+        }
+    }
     hipLaunchKernelGGL(row_norm_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, Xd, n, d, w.xnorm);
     KM_LAUNCH_CHECK();
 }
@@ -435,6 +534,7 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
 static void collect_refine_stats(km_ctx* ctx, const int32_t* host_counters) {
     ctx->stats.refined_candidate_rows += host_counters[0];
     ctx->stats.refined_full_rows += host_counters[1];
+    ctx->stats.refined_collect_rows += host_counters[2];
 }
 
 template <typename F>
@@ -451,8 +551,8 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
         Xd = xb;
     }
     work<F> w;
-    const int path = choose_path<F>(ctx, d);
-    setup_work<F>(ctx, w, Xd, n, d, k, path == 2);
+    const int path = choose_path<F>(ctx, d, flags);
+    setup_work<F>(ctx, w, Xd, n, d, k, path);
     ctx->stats.assign_path = path;
     F* Ccur = ctx->c_cur.get<F>(int64_t(k) * d);
     F* Cnext = ctx->c_next.get<F>(int64_t(k) * d);
@@ -478,7 +578,7 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
     double prev = double(std::numeric_limits<F>::max());
     int64_t iter = 0;
     for (iter = 0; iter < max_iter; ++iter) {
-        centroid_prep<F>(ctx, w, Ccur, path == 2);
+        centroid_prep<F>(ctx, w, Ccur, path);
         assign<F>(ctx, w, Ccur, path);
         fill<int32_t>(ctx, counts, k, 0);
         {
@@ -502,15 +602,15 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
         double* objp = nullptr;
         if (need_obj) objp = objective<F>(ctx, w, Ccur, nullptr);
         KM_HIP(hipMemcpyAsync(pin, meta, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+        KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
         if (need_obj)
-            KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KM_HIP(hipStreamSynchronize(ctx->stream));
         collect_refine_stats(ctx, pin + 2);
         const int m = pin[0];
         double obj = 0.0;
-        if (need_obj) std::memcpy(&obj, pin + 4, sizeof(double));
+        if (need_obj) std::memcpy(&obj, pin + 6, sizeof(double));
         if (m > 0) handle_empty<F>(ctx, w, Ccur, Cnext, m, empty_list, &obj, need_obj);
         std::swap(Ccur, Cnext);
         if (need_obj) {
@@ -522,20 +622,20 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
         }
     }
     // final assignment with the final centroids and the objective
-    centroid_prep<F>(ctx, w, Ccur, path == 2);
+    centroid_prep<F>(ctx, w, Ccur, path);
     assign<F>(ctx, w, Ccur, path);
     double* objp = objective<F>(ctx, w, Ccur, nullptr);
     if (ctx->timing) KM_HIP(hipEventRecord(loop1, ctx->stream));
-    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
-    KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     const hipMemcpyKind out_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     KM_HIP(hipMemcpyAsync(C_out, Ccur, sizeof(F) * k * d, out_kind, ctx->stream));
     KM_HIP(hipMemcpyAsync(labels_out, w.labels, sizeof(int32_t) * n, out_kind, ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));
     collect_refine_stats(ctx, pin + 2);
     double obj;
-    std::memcpy(&obj, pin + 4, sizeof(double));
+    std::memcpy(&obj, pin + 6, sizeof(double));
     if (obj_out) *obj_out = obj;
     if (iters_out) *iters_out = iter;
     if (ctx->timing) {
@@ -566,27 +666,27 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
         Xd = xb;
     }
     work<F> w;
-    const int path = choose_path<F>(ctx, d);
-    setup_work<F>(ctx, w, Xd, n, d, k, path == 2);
+    const int path = choose_path<F>(ctx, d, flags);
+    setup_work<F>(ctx, w, Xd, n, d, k, path);
     ctx->stats.assign_path = path;
     F* Cd = ctx->c_cur.get<F>(int64_t(k) * d);
     KM_HIP(hipMemcpyAsync(Cd, C, sizeof(F) * k * d,
                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
-    centroid_prep<F>(ctx, w, Cd, path == 2);
+    centroid_prep<F>(ctx, w, Cd, path);
     assign<F>(ctx, w, Cd, path);
     int32_t* pin = ctx->pinned;
     const bool want_obj = flags & KM_FLAG_EXACT_OBJECTIVE;
     if (want_obj) {
         double* objp = objective<F>(ctx, w, Cd, nullptr);
-        KM_HIP(hipMemcpyAsync(pin + 4, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     }
-    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+    KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     KM_HIP(hipMemcpyAsync(labels_out, w.labels, sizeof(int32_t) * n,
                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));
     collect_refine_stats(ctx, pin + 2);
-    if (want_obj && obj_out) std::memcpy(obj_out, pin + 4, sizeof(double));
+    if (want_obj && obj_out) std::memcpy(obj_out, pin + 6, sizeof(double));
 }
 
 // --- precondition checks (reference order: train_ops.hpp:40-61, common.cpp:82-115) ---
@@ -687,7 +787,8 @@ km_status km_context_destroy(km_ctx* ctx) {
     if (!ctx) return KM_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    km::dbuf* bufs[] = {&ctx->x_dev, &ctx->x_pad, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
+    km::dbuf* bufs[] = {&ctx->x_bf16, &ctx->enorm, &ctx->xmax, &ctx->c_bf16, &ctx->params,
+                        &ctx->coll_rows, &ctx->coll_T, &ctx->coll_count, &ctx->coll_ids, &ctx->x_dev, &ctx->x_pad, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
                         &ctx->xnorm, &ctx->labels, &ctx->lists_rows, &ctx->lists_cnt,
                         &ctx->lists_ids, &ctx->lists_full, &ctx->lists_counters, &ctx->counts,
                         &ctx->offsets, &ctx->chunk_base, &ctx->empty_list, &ctx->meta, &ctx->perm,

@@ -46,6 +46,7 @@ class KmStats(ctypes.Structure):
         ("empty_cluster_events", ctypes.c_int64),
         ("assign_path", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("refined_collect_rows", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -13,13 +13,15 @@ from test_oracle_golden import load_gold, match_map, rel_close
 
 pytestmark = pytest.mark.gpu
 
-PATHS = {"generic": 1, "mfma_f32": 2}
+PATHS = {"generic": 1, "mfma_f32": 2, "bf16": 0}
 
 
 def set_path(engine, path):
+    """Select the assignment path; returns the bf16 flag to pass to train/infer."""
     from onedal_amd.engine import KM_OPT_ASSIGN_PATH
 
     engine.set_option(KM_OPT_ASSIGN_PATH, PATHS[path])
+    return path == "bf16"
 
 
 def assert_bit_exact(got, want):
@@ -33,11 +35,11 @@ def assert_bit_exact(got, want):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("path", ["generic", "mfma_f32"])
+@pytest.mark.parametrize("path", ["generic", "mfma_f32", "bf16"])
 def test_gold_dataset(engine, oracle, dtype, path):
-    set_path(engine, path)
+    bf16 = set_path(engine, path)
     g, x, c0, ec, er = load_gold(dtype)
-    res = engine.train(x, c0, g["max_iteration_count"], 0.0)
+    res = engine.train(x, c0, g["max_iteration_count"], 0.0, bf16=bf16)
     c, lab, obj, it = res
     # reference tolerances (test/fixture.hpp:156-203)
     mm = match_map(ec, c)
@@ -47,18 +49,18 @@ def test_gold_dataset(engine, oracle, dtype, path):
     # bit-exact against the canonical restatement
     assert_bit_exact(res, oracle.canonical_lloyd(x, c0, g["max_iteration_count"], 0.0))
     # inference with the trained model (test/fixture.hpp:185-193)
-    ilab, iobj = engine.infer(x, c, objective=True)
+    ilab, iobj = engine.infer(x, c, objective=True, bf16=bf16)
     assert np.array_equal(er, mm[ilab])
     assert iobj == obj
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("path", ["generic", "mfma_f32"])
+@pytest.mark.parametrize("path", ["generic", "mfma_f32", "bf16"])
 def test_empty_clusters_fixture(engine, oracle, dtype, path):
-    set_path(engine, path)
+    bf16 = set_path(engine, path)
     x = np.array([-10, -9.5, -9, -8.5, -8, -1, 1, 9, 9.5, 10], dtype).reshape(10, 1)
     c0 = np.array([-10, -10, -10], dtype).reshape(3, 1)
-    c, lab, obj, it = engine.train(x, c0, 1, 0.0)
+    c, lab, obj, it = engine.train(x, c0, 1, 0.0, bf16=bf16)
     assert rel_close(c.ravel(), np.array([-1.65, 10, 9.5], dtype), 5 * np.finfo(dtype).eps * 100)
     assert lab.tolist() == [0, 0, 0, 0, 0, 0, 0, 2, 2, 1]
     assert it == 1
@@ -83,15 +85,15 @@ SHAPES = [
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("path", ["generic", "mfma_f32"])
+@pytest.mark.parametrize("path", ["generic", "mfma_f32", "bf16"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "n%d_d%d_k%d" % s[:3])
 def test_train_bit_exact_vs_canonical(engine, oracle, dtype, path, shape):
     n, d, k, iters = shape
-    set_path(engine, path)
+    bf16 = set_path(engine, path)
     x = gaussian_mixture(n, d, k, seed=n + d + k, center_range=10.0, sigma=(0.5, 2.0), dtype=dtype)
     rng = np.random.default_rng(k)
     c0 = x[rng.choice(n, size=k, replace=False)].copy()
-    got = engine.train(x, c0, iters, 0.0)
+    got = engine.train(x, c0, iters, 0.0, bf16=bf16)
     assert_bit_exact(got, oracle.canonical_lloyd(x, c0, iters, 0.0))
 
 
@@ -112,12 +114,12 @@ def test_train_vs_daal_restatement(engine, oracle, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("path", ["generic", "mfma_f32"])
+@pytest.mark.parametrize("path", ["generic", "mfma_f32", "bf16"])
 def test_infer_bit_exact(engine, oracle, dtype, path):
-    set_path(engine, path)
+    bf16 = set_path(engine, path)
     x = gaussian_mixture(20000, 48, 33, seed=11, dtype=dtype)
     c = gaussian_mixture(33, 48, 33, seed=12, dtype=dtype)
-    lab, obj = engine.infer(x, c, objective=True)
+    lab, obj = engine.infer(x, c, objective=True, bf16=bf16)
     wl, _ = oracle.canonical_assign(x, c)
     assert np.array_equal(lab, wl)
     assert obj == oracle.canonical_objective(x, c, wl)
@@ -197,3 +199,28 @@ def test_badargs(engine):
     # still usable after errors
     c, lab, obj, it = engine.train(np.arange(20, dtype=np.float32).reshape(10, 2), c0 + [[0, 0], [5, 5], [15, 15]], 2, 0.0)
     assert it == 2
+
+
+@pytest.mark.parametrize("shape", [(40000, 128, 1024, 3), (30000, 64, 256, 4), (20000, 32, 100, 4)],
+                         ids=lambda s: "n%d_d%d_k%d" % s[:3])
+def test_bf16_labels_identical_to_fp32(engine, oracle, shape):
+    """The precision form's contract: BF16 distances + exact re-score give the same
+    labels, centroids and objective as the FP32 path, bit for bit, on anisotropic
+    Gaussian mixtures like the precision workload (centers U[-50,50]^d, per-dimension
+    scale log-uniform in [0.1, 10])."""
+    n, d, k, iters = shape
+    x = gaussian_mixture(n, d, k, seed=d, center_range=50.0, anisotropic=True)
+    rng = np.random.default_rng(1)
+    c0 = x[rng.choice(n, size=k, replace=False)].copy()
+    set_path(engine, "mfma_f32")
+    a = engine.train(x, c0, iters, 0.0)
+    set_path(engine, "bf16")
+    b = engine.train(x, c0, iters, 0.0, bf16=True)
+    st = engine.stats()
+    assert st["assign_path"] == 3
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0], b[0])
+    assert a[2] == b[2] and a[3] == b[3]
+    # near-ties were actually re-scored on this data
+    assert st["refined_candidate_rows"] > 0
+    assert_bit_exact(b, oracle.canonical_lloyd(x, c0, iters, 0.0))
