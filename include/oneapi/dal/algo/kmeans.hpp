@@ -1,0 +1,511 @@
+// oneapi/dal/algo/kmeans.hpp -- drop-in C++ surface of oneDAL's k-means
+// (dense Lloyd) over the MI355X engine (include/onedal_amd/kmeans_hip.h).
+//
+// Mirrors the reference interface:
+//   descriptor<Float, Method, Task>   cpp/oneapi/dal/algo/kmeans/common.hpp:155-220
+//   model<Task>                       common.hpp:224-258
+//   train_input / train_result        train_types.hpp:41-161
+//   infer_input / infer_result        infer_types.hpp:42-138
+//   result_options                    common.hpp:61-84
+//   dal::train / dal::infer           cpp/oneapi/dal/train.hpp:26-59, infer.hpp
+// with the same setter validation (common.cpp:82-115), preconditions
+// (detail/train_ops.hpp:40-61, detail/infer_ops.hpp:40-56), result checks
+// (train_types.cpp:141-154, infer_types.cpp:91-110) and messages.
+//
+// Where the reference takes a sycl::queue, this surface takes a
+// oneapi::dal::hip::queue (one HIP device + stream); the overloads without a
+// queue use device 0.  Tables are host tables (homogen_table::wrap); device
+// pointers go through the C ABI directly (KM_FLAG_DEVICE_POINTERS).
+// Extension (no reference counterpart): hip::queue::set_bf16_refine(true)
+// selects BF16 matrix-core distances with exact float re-score for float
+// descriptors; results are identical to the default path.
+#pragma once
+
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <string>
+#include <type_traits>
+
+#include "../../../onedal_amd/kmeans_hip.h"
+#include "../exceptions.hpp"
+#include "../table.hpp"
+
+namespace oneapi::dal {
+
+// ---------------------------------------------------------------------------
+// device queue
+// ---------------------------------------------------------------------------
+namespace hip {
+
+class queue {
+public:
+    explicit queue(int device = 0) : impl_(std::make_shared<impl>(device)) {}
+    int get_device() const { return impl_->device; }
+    km_ctx* get_context() const { return impl_->ctx; }
+    // extension: BF16 matrix-core distances + exact re-score (float data only)
+    queue& set_bf16_refine(bool on) {
+        impl_->bf16 = on;
+        return *this;
+    }
+    bool get_bf16_refine() const { return impl_->bf16; }
+
+private:
+    struct impl {
+        int device;
+        km_ctx* ctx = nullptr;
+        bool bf16 = false;
+        explicit impl(int dev) : device(dev) {
+            if (km_context_create(dev, &ctx) != KM_OK)
+                throw unsupported_device("No usable HIP device " + std::to_string(dev));
+        }
+        ~impl() {
+            if (ctx) km_context_destroy(ctx);
+        }
+        impl(const impl&) = delete;
+        impl& operator=(const impl&) = delete;
+    };
+    std::shared_ptr<impl> impl_;
+};
+
+inline queue& default_queue() {
+    static queue q(0);
+    return q;
+}
+
+// Throw the oneapi::dal exception matching a km_status.
+inline void check(km_status st, km_ctx* ctx) {
+    if (st == KM_OK) return;
+    const std::string msg = ctx ? km_last_error(ctx) : "invalid context";
+    switch (st) {
+        case KM_DOMAIN_ERROR: throw domain_error(msg);
+        case KM_INVALID_ARGUMENT: throw invalid_argument(msg);
+        case KM_DEVICE_BAD_ALLOC: throw device_bad_alloc();
+        case KM_HOST_BAD_ALLOC: throw host_bad_alloc();
+        case KM_UNIMPLEMENTED: throw unimplemented(msg);
+        default: throw internal_error(msg);
+    }
+}
+
+}  // namespace hip
+
+// ---------------------------------------------------------------------------
+// result option ids (util/result_option_id.hpp)
+// ---------------------------------------------------------------------------
+class result_option_id_base {
+public:
+    using bitset_t = std::uint64_t;
+    constexpr result_option_id_base() : mask_(0) {}
+    constexpr explicit result_option_id_base(bitset_t m) : mask_(m) {}
+    constexpr explicit operator bool() const { return mask_ > 0; }
+    constexpr bitset_t get_mask() const { return mask_; }
+    constexpr bool test(const result_option_id_base& f) const { return (mask_ & f.mask_) != 0; }
+    static constexpr result_option_id_base make_by_index(std::int64_t i) {
+        return result_option_id_base(bitset_t(1) << i);
+    }
+
+private:
+    bitset_t mask_;
+};
+
+namespace kmeans {
+
+namespace task {
+struct clustering {};
+using by_default = clustering;
+}  // namespace task
+
+namespace method {
+struct lloyd_dense {};
+struct lloyd_csr {};
+using by_default = lloyd_dense;
+}  // namespace method
+
+class result_option_id : public result_option_id_base {
+public:
+    constexpr result_option_id() = default;
+    constexpr explicit result_option_id(const result_option_id_base& b) : result_option_id_base(b) {}
+};
+
+inline result_option_id operator|(const result_option_id& a, const result_option_id& b) {
+    return result_option_id(result_option_id_base(a.get_mask() | b.get_mask()));
+}
+
+namespace result_options {
+// common.cpp:24-26: bit 0
+const inline result_option_id compute_exact_objective_function =
+    result_option_id(result_option_id_base::make_by_index(0));
+}  // namespace result_options
+
+namespace detail {
+
+template <typename Task = task::by_default>
+class descriptor_base {
+    static_assert(std::is_same_v<Task, task::clustering>);
+
+public:
+    using float_t = float;
+    using method_t = method::by_default;
+    using task_t = Task;
+
+    descriptor_base() : impl_(std::make_shared<impl>()) {}
+    std::int64_t get_cluster_count() const { return impl_->cluster_count; }
+    std::int64_t get_max_iteration_count() const { return impl_->max_iteration_count; }
+    double get_accuracy_threshold() const { return impl_->accuracy_threshold; }
+    result_option_id get_result_options() const { return impl_->result_options; }
+
+protected:
+    void set_cluster_count_impl(std::int64_t v) {
+        using namespace oneapi::dal::detail::error_messages;
+        if (v <= 0) throw domain_error(cluster_count_leq_zero());
+        if (v > std::numeric_limits<std::int32_t>::max()) throw domain_error(cluster_count_gt_max_int32());
+        impl_->cluster_count = v;
+    }
+    void set_max_iteration_count_impl(std::int64_t v) {
+        if (v < 0) throw domain_error(oneapi::dal::detail::error_messages::max_iteration_count_lt_zero());
+        impl_->max_iteration_count = v;
+    }
+    void set_accuracy_threshold_impl(double v) {
+        if (v < 0.0) throw domain_error(oneapi::dal::detail::error_messages::accuracy_threshold_lt_zero());
+        impl_->accuracy_threshold = v;
+    }
+    void set_result_options_impl(const result_option_id& v) {
+        if (!bool(v)) throw domain_error(oneapi::dal::detail::error_messages::empty_set_of_result_options());
+        impl_->result_options = v;
+    }
+
+private:
+    // defaults of common.cpp:36-42
+    struct impl {
+        std::int64_t cluster_count = 2;
+        std::int64_t max_iteration_count = 100;
+        double accuracy_threshold = 0.0;
+        result_option_id result_options{};
+    };
+    std::shared_ptr<impl> impl_;
+};
+
+}  // namespace detail
+
+template <typename Float = float, typename Method = method::by_default,
+          typename Task = task::by_default>
+class descriptor : public detail::descriptor_base<Task> {
+    static_assert(std::is_same_v<Float, float> || std::is_same_v<Float, double>);
+    static_assert(std::is_same_v<Method, method::lloyd_dense> ||
+                  std::is_same_v<Method, method::lloyd_csr>);
+    using base_t = detail::descriptor_base<Task>;
+
+public:
+    using float_t = Float;
+    using method_t = Method;
+    using task_t = Task;
+
+    explicit descriptor(std::int64_t cluster_count = 2) { set_cluster_count(cluster_count); }
+    std::int64_t get_cluster_count() const { return base_t::get_cluster_count(); }
+    auto& set_cluster_count(std::int64_t v) {
+        base_t::set_cluster_count_impl(v);
+        return *this;
+    }
+    std::int64_t get_max_iteration_count() const { return base_t::get_max_iteration_count(); }
+    auto& set_max_iteration_count(std::int64_t v) {
+        base_t::set_max_iteration_count_impl(v);
+        return *this;
+    }
+    double get_accuracy_threshold() const { return base_t::get_accuracy_threshold(); }
+    auto& set_accuracy_threshold(double v) {
+        base_t::set_accuracy_threshold_impl(v);
+        return *this;
+    }
+    result_option_id get_result_options() const { return base_t::get_result_options(); }
+    auto& set_result_options(const result_option_id& v) {
+        base_t::set_result_options_impl(v);
+        return *this;
+    }
+};
+
+template <typename Task = task::by_default>
+class model {
+public:
+    using task_t = Task;
+    model() : impl_(std::make_shared<impl>()) {}
+    const table& get_centroids() const { return impl_->centroids; }
+    auto& set_centroids(const table& v) {
+        impl_->centroids = v;
+        return *this;
+    }
+    std::int64_t get_cluster_count() const { return impl_->centroids.get_row_count(); }
+
+private:
+    struct impl {
+        table centroids;
+    };
+    std::shared_ptr<impl> impl_;
+};
+
+template <typename Task = task::by_default>
+class train_input {
+public:
+    using task_t = Task;
+    train_input(const table& data) : data_(data) {}
+    train_input(const table& data, const table& initial_centroids)
+            : data_(data), initial_centroids_(initial_centroids) {}
+    const table& get_data() const { return data_; }
+    auto& set_data(const table& v) {
+        data_ = v;
+        return *this;
+    }
+    const table& get_initial_centroids() const { return initial_centroids_; }
+    auto& set_initial_centroids(const table& v) {
+        initial_centroids_ = v;
+        return *this;
+    }
+
+private:
+    table data_, initial_centroids_;
+};
+
+template <typename Task = task::by_default>
+class train_result {
+public:
+    using task_t = Task;
+    const model<Task>& get_model() const { return model_; }
+    auto& set_model(const model<Task>& v) {
+        model_ = v;
+        return *this;
+    }
+    [[deprecated]] const table& get_labels() const { return get_responses(); }
+    const table& get_responses() const { return responses_; }
+    auto& set_responses(const table& v) {
+        responses_ = v;
+        return *this;
+    }
+    std::int64_t get_iteration_count() const { return iteration_count_; }
+    auto& set_iteration_count(std::int64_t v) {
+        if (v < 0) throw domain_error(oneapi::dal::detail::error_messages::iteration_count_lt_zero());
+        iteration_count_ = v;
+        return *this;
+    }
+    double get_objective_function_value() const { return objective_; }
+    auto& set_objective_function_value(double v) {
+        if (v < 0.0)
+            throw domain_error(oneapi::dal::detail::error_messages::objective_function_value_lt_zero());
+        objective_ = v;
+        return *this;
+    }
+
+private:
+    model<Task> model_;
+    table responses_;
+    std::int64_t iteration_count_ = 0;
+    double objective_ = 0.0;
+};
+
+template <typename Task = task::by_default>
+class infer_input {
+public:
+    using task_t = Task;
+    infer_input(const model<Task>& m, const table& data) : model_(m), data_(data) {}
+    const model<Task>& get_model() const { return model_; }
+    auto& set_model(const model<Task>& v) {
+        model_ = v;
+        return *this;
+    }
+    const table& get_data() const { return data_; }
+    auto& set_data(const table& v) {
+        data_ = v;
+        return *this;
+    }
+
+private:
+    model<Task> model_;
+    table data_;
+};
+
+template <typename Task = task::by_default>
+class infer_result {
+public:
+    using task_t = Task;
+    [[deprecated]] const table& get_labels() const { return get_responses(); }
+    const table& get_responses() const { return responses_; }
+    auto& set_responses(const table& v) {
+        responses_ = v;
+        return *this;
+    }
+    double get_objective_function_value() const {
+        if (!options_.test(result_options::compute_exact_objective_function))
+            throw domain_error(
+                oneapi::dal::detail::error_messages::this_result_is_not_enabled_via_result_options());
+        return objective_;
+    }
+    auto& set_objective_function_value(double v) {
+        using namespace oneapi::dal::detail::error_messages;
+        if (!options_.test(result_options::compute_exact_objective_function))
+            throw domain_error(this_result_is_not_enabled_via_result_options());
+        if (v < 0.0) throw domain_error(objective_function_value_lt_zero());
+        objective_ = v;
+        return *this;
+    }
+    const result_option_id& get_result_options() const { return options_; }
+    auto& set_result_options(const result_option_id& v) {
+        options_ = v;
+        return *this;
+    }
+
+private:
+    table responses_;
+    double objective_ = 0.0;
+    result_option_id options_{};
+};
+
+namespace detail {
+
+// train_ops::check_preconditions (detail/train_ops.hpp:40-61)
+template <typename Desc>
+void check_train_preconditions(const Desc& desc, const train_input<>& in) {
+    using namespace oneapi::dal::detail::error_messages;
+    const table& x = in.get_data();
+    if (!x.has_data()) throw domain_error(input_data_is_empty());
+    if (x.get_row_count() > std::numeric_limits<std::int32_t>::max())
+        throw domain_error(row_count_gt_max_int32());
+    if (desc.get_cluster_count() > x.get_row_count())
+        throw invalid_argument(cluster_count_exceeds_data_row_count());
+    const table& c0 = in.get_initial_centroids();
+    if (c0.has_data()) {
+        if (c0.get_row_count() != desc.get_cluster_count())
+            throw invalid_argument(input_initial_centroids_rc_neq_desc_cluster_count());
+        if (c0.get_column_count() != x.get_column_count())
+            throw invalid_argument(input_initial_centroids_cc_neq_input_data_cc());
+    }
+}
+
+// infer_ops::check_preconditions (detail/infer_ops.hpp:40-56)
+template <typename Desc>
+void check_infer_preconditions(const Desc& desc, const infer_input<>& in) {
+    using namespace oneapi::dal::detail::error_messages;
+    if (!in.get_data().has_data()) throw domain_error(input_data_is_empty());
+    const table& c = in.get_model().get_centroids();
+    if (!c.has_data()) throw domain_error(input_model_centroids_are_empty());
+    if (c.get_row_count() != desc.get_cluster_count())
+        throw invalid_argument(input_model_centroids_rc_neq_desc_cluster_count());
+    if (c.get_column_count() != in.get_data().get_column_count())
+        throw invalid_argument(input_model_centroids_cc_neq_input_data_cc());
+}
+
+template <typename Float, typename Method>
+train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                        const train_input<>& in) {
+    static_assert(std::is_same_v<Method, method::lloyd_dense>,
+                  "only the dense Lloyd method runs on this engine");
+    check_train_preconditions(desc, in);
+    const table& xt = in.get_data();
+    const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
+    const std::int64_t k = desc.get_cluster_count();
+    if (!in.get_initial_centroids().has_data())
+        throw unimplemented(
+            "Initial centroids are required: the default k-means++ initialisation is not part "
+            "of this engine");
+    const auto x = row_accessor<const Float>(xt).pull();
+    const auto c0 = row_accessor<const Float>(in.get_initial_centroids()).pull();
+    array<Float> c(k * d);
+    array<std::int32_t> labels(n);
+    double obj = 0.0;
+    std::int64_t iters = 0;
+    const std::uint32_t flags = q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u;
+    km_status st;
+    if constexpr (std::is_same_v<Float, float>)
+        st = km_lloyd_train_f32(q.get_context(), x.get_data(), n, d, c0.get_data(), k,
+                                desc.get_max_iteration_count(), desc.get_accuracy_threshold(),
+                                flags, c.get_mutable_data(), labels.get_mutable_data(), &obj, &iters);
+    else
+        st = km_lloyd_train_f64(q.get_context(), x.get_data(), n, d, c0.get_data(), k,
+                                desc.get_max_iteration_count(), desc.get_accuracy_threshold(),
+                                flags, c.get_mutable_data(), labels.get_mutable_data(), &obj, &iters);
+    hip::check(st, q.get_context());
+    train_result<> r;
+    r.set_model(model<>().set_centroids(homogen_table::wrap(c, k, d)));
+    r.set_responses(homogen_table::wrap(labels, n, 1));
+    r.set_iteration_count(iters);
+    r.set_objective_function_value(obj);
+    return r;
+}
+
+template <typename Float, typename Method>
+infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                        const infer_input<>& in) {
+    static_assert(std::is_same_v<Method, method::lloyd_dense>,
+                  "only the dense Lloyd method runs on this engine");
+    check_infer_preconditions(desc, in);
+    const table& xt = in.get_data();
+    const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
+    const std::int64_t k = desc.get_cluster_count();
+    const auto x = row_accessor<const Float>(xt).pull();
+    const auto c = row_accessor<const Float>(in.get_model().get_centroids()).pull();
+    const bool want_obj = desc.get_result_options().test(result_options::compute_exact_objective_function);
+    array<std::int32_t> labels(n);
+    double obj = 0.0;
+    const std::uint32_t flags = (want_obj ? KM_FLAG_EXACT_OBJECTIVE : 0u) |
+                                (q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u);
+    km_status st;
+    if constexpr (std::is_same_v<Float, float>)
+        st = km_lloyd_infer_f32(q.get_context(), x.get_data(), n, d, c.get_data(), k, flags,
+                                labels.get_mutable_data(), &obj);
+    else
+        st = km_lloyd_infer_f64(q.get_context(), x.get_data(), n, d, c.get_data(), k, flags,
+                                labels.get_mutable_data(), &obj);
+    hip::check(st, q.get_context());
+    infer_result<> r;
+    r.set_result_options(desc.get_result_options());
+    r.set_responses(homogen_table::wrap(labels, n, 1));
+    if (want_obj) r.set_objective_function_value(obj);
+    return r;
+}
+
+}  // namespace detail
+}  // namespace kmeans
+
+// ---------------------------------------------------------------------------
+// dal::train / dal::infer (train.hpp:26-59, infer.hpp)
+// ---------------------------------------------------------------------------
+template <typename Float, typename Method>
+kmeans::train_result<> train(hip::queue& q, const kmeans::descriptor<Float, Method>& desc,
+                             const kmeans::train_input<>& input) {
+    return kmeans::detail::train_on(q, desc, input);
+}
+template <typename Float, typename Method>
+kmeans::train_result<> train(hip::queue& q, const kmeans::descriptor<Float, Method>& desc,
+                             const table& data, const table& initial_centroids = table{}) {
+    return kmeans::detail::train_on(q, desc, kmeans::train_input<>(data, initial_centroids));
+}
+// Without a queue: the default device (0).  Preconditions are checked before
+// the device is touched, as the reference checks them before dispatch.
+template <typename Float, typename Method>
+kmeans::train_result<> train(const kmeans::descriptor<Float, Method>& desc,
+                             const kmeans::train_input<>& input) {
+    kmeans::detail::check_train_preconditions(desc, input);
+    return kmeans::detail::train_on(hip::default_queue(), desc, input);
+}
+template <typename Float, typename Method>
+kmeans::train_result<> train(const kmeans::descriptor<Float, Method>& desc, const table& data,
+                             const table& initial_centroids = table{}) {
+    return train(desc, kmeans::train_input<>(data, initial_centroids));
+}
+
+template <typename Float, typename Method>
+kmeans::infer_result<> infer(hip::queue& q, const kmeans::descriptor<Float, Method>& desc,
+                             const kmeans::infer_input<>& input) {
+    return kmeans::detail::infer_on(q, desc, input);
+}
+template <typename Float, typename Method>
+kmeans::infer_result<> infer(hip::queue& q, const kmeans::descriptor<Float, Method>& desc,
+                             const kmeans::model<>& m, const table& data) {
+    return kmeans::detail::infer_on(q, desc, kmeans::infer_input<>(m, data));
+}
+template <typename Float, typename Method>
+kmeans::infer_result<> infer(const kmeans::descriptor<Float, Method>& desc,
+                             const kmeans::model<>& m, const table& data) {
+    const kmeans::infer_input<> input(m, data);
+    kmeans::detail::check_infer_preconditions(desc, input);
+    return kmeans::detail::infer_on(hip::default_queue(), desc, input);
+}
+
+}  // namespace oneapi::dal

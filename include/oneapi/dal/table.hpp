@@ -1,0 +1,137 @@
+// oneapi/dal/table.hpp -- minimal homogeneous table of the drop-in C++ surface.
+//
+// Covers what the k-means path exchanges (reference cpp/oneapi/dal/table/
+// homogen.hpp:48-114, row_accessor.hpp:70-135): row-major float / double /
+// int32 data held on the host, shared on copy, and row_accessor<const T>::pull
+// with element-type conversion.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "exceptions.hpp"
+
+namespace oneapi::dal {
+
+enum class data_type { int32, float32, float64 };
+
+namespace detail {
+template <typename T>
+constexpr data_type data_type_of() {
+    if constexpr (std::is_same_v<T, float>) return data_type::float32;
+    else if constexpr (std::is_same_v<T, double>) return data_type::float64;
+    else {
+        static_assert(std::is_same_v<T, std::int32_t>, "tables hold float, double or int32");
+        return data_type::int32;
+    }
+}
+inline std::size_t size_of(data_type t) { return t == data_type::float64 ? 8 : 4; }
+}  // namespace detail
+
+// Host array with shared ownership (the reference's dal::array, reduced).
+template <typename T>
+class array {
+public:
+    array() = default;
+    explicit array(std::int64_t count) : data_(std::make_shared<std::vector<T>>(count)) {}
+    static array zeros(std::int64_t count) { return array(count); }
+    const T* get_data() const { return data_ ? data_->data() : nullptr; }
+    T* get_mutable_data() { return data_ ? data_->data() : nullptr; }
+    std::int64_t get_count() const { return data_ ? std::int64_t(data_->size()) : 0; }
+    const T& operator[](std::int64_t i) const { return (*data_)[i]; }
+
+private:
+    std::shared_ptr<std::vector<T>> data_;
+};
+
+class table {
+public:
+    table() = default;
+    bool has_data() const { return rows_ > 0 && cols_ > 0 && data_ != nullptr; }
+    std::int64_t get_row_count() const { return rows_; }
+    std::int64_t get_column_count() const { return cols_; }
+    data_type get_data_type() const { return dtype_; }
+    const void* get_raw_data() const { return data_.get(); }
+
+protected:
+    std::shared_ptr<const void> data_;
+    std::int64_t rows_ = 0, cols_ = 0;
+    data_type dtype_ = data_type::float32;
+};
+
+class homogen_table : public table {
+public:
+    homogen_table() = default;
+
+    // Non-owning view of row-major host data (the caller keeps it alive).
+    template <typename T>
+    static homogen_table wrap(const T* data, std::int64_t rows, std::int64_t cols) {
+        homogen_table t;
+        t.data_ = std::shared_ptr<const void>(data, [](const void*) {});
+        t.rows_ = rows;
+        t.cols_ = cols;
+        t.dtype_ = detail::data_type_of<T>();
+        return t;
+    }
+
+    // Shares ownership of a dal::array.
+    template <typename T>
+    static homogen_table wrap(const array<T>& a, std::int64_t rows, std::int64_t cols) {
+        homogen_table t;
+        auto holder = std::make_shared<array<T>>(a);
+        t.data_ = std::shared_ptr<const void>(holder, holder->get_data());
+        t.rows_ = rows;
+        t.cols_ = cols;
+        t.dtype_ = detail::data_type_of<T>();
+        return t;
+    }
+};
+
+struct range {
+    std::int64_t start_idx = 0;
+    std::int64_t end_idx = -1;  // -1: up to the last row
+};
+
+template <typename T>
+class row_accessor {
+    static_assert(std::is_const_v<T>, "row_accessor gives read-only access");
+    using V = std::remove_const_t<T>;
+
+public:
+    explicit row_accessor(const table& t) : t_(t) {}
+
+    // Rows [r.start_idx, r.end_idx) as a host array of V, converted if needed.
+    array<V> pull(const range& r = {}) const {
+        const std::int64_t end = r.end_idx < 0 ? t_.get_row_count() : r.end_idx;
+        if (r.start_idx < 0 || end > t_.get_row_count() || r.start_idx > end)
+            throw out_of_range("Row range is out of table bounds");
+        const std::int64_t cols = t_.get_column_count();
+        const std::int64_t count = (end - r.start_idx) * cols;
+        array<V> out(count);
+        const char* base = static_cast<const char*>(t_.get_raw_data()) +
+                           r.start_idx * cols * detail::size_of(t_.get_data_type());
+        switch (t_.get_data_type()) {
+            case data_type::float32: convert(reinterpret_cast<const float*>(base), out, count); break;
+            case data_type::float64: convert(reinterpret_cast<const double*>(base), out, count); break;
+            case data_type::int32: convert(reinterpret_cast<const std::int32_t*>(base), out, count); break;
+        }
+        return out;
+    }
+
+private:
+    template <typename S>
+    static void convert(const S* src, array<V>& out, std::int64_t count) {
+        V* dst = out.get_mutable_data();
+        if constexpr (std::is_same_v<S, V>) {
+            if (count) std::memcpy(dst, src, sizeof(V) * count);
+        } else {
+            for (std::int64_t i = 0; i < count; ++i) dst[i] = V(src[i]);
+        }
+    }
+    table t_;
+};
+
+}  // namespace oneapi::dal

@@ -298,6 +298,12 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
 
         auto block = [&](int b, int mb, int sl) {
             const float* cnb = Cn + b * 32;
+            const uint16_t* arow = &Cs[(b * 32 + r32) * CST + h * HALF];
+            // all A fragments and start values of the block first, then the MFMA chain
+            bf16x8 a[STEPS];
+#pragma unroll
+            for (int s = 0; s < STEPS; ++s)
+                a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uintx4*>(arow + 8 * s));
             floatx16 init;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -306,14 +312,12 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 for (int t = 0; t < 4; ++t) init[4 * q + t] = c4[t];
             }
             floatx16 acc[NB];
-            const uint16_t* arow = &Cs[(b * 32 + r32) * CST + h * HALF];
 #pragma unroll
             for (int s = 0; s < STEPS; ++s) {
-                const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uintx4*>(arow + 8 * s));
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
-                    acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xr[nb][s], s == 0 ? init : acc[nb],
-                                                                      0, 0, 0);
+                    acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], xr[nb][s],
+                                                                      s == 0 ? init : acc[nb], 0, 0, 0);
             }
             if (COLLECT) {
 #pragma unroll
@@ -332,12 +336,20 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                     }
             } else {
                 const uint32_t tag = uint32_t(mb) << 4;
+                const uint32_t lowmask = ~kmask;
+                // the mask lives in a VGPR so (bits & mask) | tag is one v_and_or_b32
+                // (a VALU op may read only one scalar operand on this target)
+                uint32_t kmv;
+                asm volatile("v_mov_b32 %0, %1" : "=v"(kmv) : "s"(kmask));
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                     for (int i = 0; i < 16; i += 2) {
-                        const uint32_t ka = (__float_as_uint(acc[nb][i]) & kmask) | tag | uint32_t(i);
-                        const uint32_t kb = (__float_as_uint(acc[nb][i + 1]) & kmask) | tag | uint32_t(i + 1);
+                        // bitfield insert: value bits where kmask is set, (block, register) below
+                        const uint32_t ta = (tag | uint32_t(i)) & lowmask;
+                        const uint32_t tb = (tag | uint32_t(i + 1)) & lowmask;
+                        const uint32_t ka = (__float_as_uint(acc[nb][i]) & kmv) | ta;
+                        const uint32_t kb = (__float_as_uint(acc[nb][i + 1]) & kmv) | tb;
                         // second smallest of {m1, ka, kb} (m1 <= m2 always)
                         const uint32_t med = max(min(m1[nb][sl], ka), min(max(m1[nb][sl], ka), kb));
                         m2[nb][sl] = min(m2[nb][sl], med);
@@ -346,25 +358,29 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             }
         };
 
+        // block mb runs from LDS buffer mb % 2 into slot mb % 4; the next block is
+        // prefetched into registers meanwhile and committed to the other buffer
+        auto step = [&](int mb, int b, int sl) {
+            const bool more = mb + 1 < mblocks;
+            if (more) prefetch(mb + 1);
+            block(b, mb, sl);
+            if (more) commit(b ^ 1);
+            __syncthreads();
+        };
         __syncthreads();
         prefetch(0);
         commit(0);
         __syncthreads();
         int mb = 0;
-        for (; mb + 1 < mblocks; mb += 2) {
-            prefetch(mb + 1);
-            block(0, mb, mb & 3);
-            commit(1);
-            __syncthreads();
-            if (mb + 2 < mblocks) prefetch(mb + 2);
-            block(1, mb + 1, (mb + 1) & 3);
-            if (mb + 2 < mblocks) commit(0);
-            __syncthreads();
+        for (; mb + 3 < mblocks; mb += 4) {
+            step(mb, 0, 0);
+            step(mb + 1, 1, 1);
+            step(mb + 2, 0, 2);
+            step(mb + 3, 1, 3);
         }
-        if (mb < mblocks) {
-            block(0, mb, mb & 3);
-            __syncthreads();
-        }
+        if (mb < mblocks) step(mb, 0, 0);
+        if (mb + 1 < mblocks) step(mb + 1, 1, 1);
+        if (mb + 2 < mblocks) step(mb + 2, 0, 2);
         if (COLLECT) continue;
 
         // settle rows: lane r32 (h = 0) merges its 4 slots with lane r32 + 32's 4
@@ -423,35 +439,6 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 L.coll_count[e] = 0;
             }
         }
-    }
-}
-
-// Collected ids -> exact re-score; overflowing entries go to the full list.
-__global__ __launch_bounds__(256) void refine_collected_kernel(const float* __restrict__ X, int d,
-                                                               const float* __restrict__ C,
-                                                               refine_lists L,
-                                                               int32_t* __restrict__ labels) {
-    const int total = L.counters[2];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-        const int64_t row = L.coll_rows[e];
-        const int cnt = L.coll_count[e];
-        if (cnt > MAXC2 || cnt == 0) {
-            const int f = atomicAdd(&L.counters[1], 1);
-            L.full_rows[f] = int32_t(row);
-            continue;
-        }
-        const float* x = X + row * d;
-        float best = fp_traits<float>::inf;
-        int bi = 0x7fffffff;
-        for (int c = 0; c < cnt; ++c) {
-            const int j = L.coll_ids[int64_t(e) * MAXC2 + c];
-            const float v = canon_dist<float>(x, C + int64_t(j) * d, d);
-            if (v < best || (v == best && j < bi)) {
-                best = v;
-                bi = j;
-            }
-        }
-        labels[row] = bi;
     }
 }
 

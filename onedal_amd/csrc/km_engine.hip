@@ -263,6 +263,20 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
     KM_LAUNCH_CHECK();
 }
 
+// exact re-score of a list of (row, candidate ids) entries; overflow -> full list
+template <typename F>
+static void refine_pairs(km_ctx* ctx, work<F>& w, const F* C, const int32_t* rows,
+                         const int32_t* counts, const int32_t* ids, int stride,
+                         const int32_t* count_p) {
+    const int rb = int(std::max<int64_t>(1, std::min<int64_t>(16, 16384 / (int64_t(w.d) * sizeof(F)))));
+    const size_t lds = size_t(4) * rb * w.d * sizeof(F);
+    auto kern = refine_pairs_kernel<F>;
+    KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(kern, dim3(ctx->cu_count * 4), dim3(256), lds, ctx->stream, w.X, w.d, C, rows,
+                       counts, ids, stride, count_p, rb, w.L.full_rows, w.L.counters + 1, w.labels);
+    KM_LAUNCH_CHECK();
+}
+
 // assignment of every row against w.X/C (centroid_prep already done for C)
 template <typename F>
 static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
@@ -304,9 +318,7 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
     }
     ctx->stats.assign_launches++;
     const int rgrid = ctx->cu_count * 4;
-    hipLaunchKernelGGL(refine_candidates_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d,
-                       C, w.L, w.labels);
-    KM_LAUNCH_CHECK();
+    refine_pairs<F>(ctx, w, C, w.L.cand_rows, w.L.cand_count, w.L.cand_ids, MAXC, w.L.counters + 0);
     if constexpr (std::is_same<F, float>::value) {
         if (path == 3) {
             switch (w.dpad) {
@@ -315,9 +327,8 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
                 case 128: launch_bf16<128>(ctx, w, true); break;
                 default: launch_bf16<256>(ctx, w, true); break;
             }
-            hipLaunchKernelGGL(refine_collected_kernel, dim3(rgrid), dim3(256), 0, ctx->stream, w.X,
-                               w.d, C, w.L, w.labels);
-            KM_LAUNCH_CHECK();
+            refine_pairs<F>(ctx, w, C, w.L.coll_rows, w.L.coll_count, w.L.coll_ids, MAXC2,
+                            w.L.counters + 2);
         }
     }
     hipLaunchKernelGGL(refine_full_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d, C,

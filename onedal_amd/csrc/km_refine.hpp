@@ -11,27 +11,114 @@
 
 namespace km {
 
-// One thread per listed row; 2..MAXC candidate ids.
+// Exact re-score of listed rows against their candidate ids (certified
+// candidates from the assignment kernels, or ids gathered by the BF16 collect
+// pass).  Entry e: row rows[e], counts[e] ids at ids[e * stride].  An entry with
+// no ids or more than `stride` (a collect overflow) goes to the full list.
+// Each wave takes RB entries: their rows are staged in LDS with coalesced loads,
+// the (row, id) pairs are spread over the 64 lanes, each lane forms the exact
+// distance of its pair (features in order; centroid read as 4-wide vectors when
+// rows are 16-byte aligned), and lane r keeps the minimum of entry r
+// (ties to the lowest id).
 template <typename F>
-__global__ __launch_bounds__(256) void refine_candidates_kernel(
-    const F* __restrict__ X, int d, const F* __restrict__ C, const refine_lists L,
-    int32_t* __restrict__ labels) {
-    const int total = L.counters[0];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-        const int64_t row = L.cand_rows[e];
-        const int cnt = L.cand_count[e];
-        const F* x = X + row * d;
-        F best = fp_traits<F>::inf;
-        int bi = 0x7fffffff;
-        for (int c = 0; c < cnt; ++c) {
-            const int j = L.cand_ids[int64_t(e) * MAXC + c];
-            const F v = canon_dist<F>(x, C + int64_t(j) * d, d);
-            if (v < best || (v == best && j < bi)) {
-                best = v;
-                bi = j;
+__global__ __launch_bounds__(256) void refine_pairs_kernel(
+    const F* __restrict__ X, int d, const F* __restrict__ C, const int32_t* __restrict__ rows,
+    const int32_t* __restrict__ counts, const int32_t* __restrict__ ids, int stride,
+    const int32_t* __restrict__ count_p, int rb, int32_t* __restrict__ full_rows,
+    int32_t* __restrict__ full_counter, int32_t* __restrict__ labels) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ F pd[4][64];
+    __shared__ int pj[4][64];
+    __shared__ int pstart[4][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    F* xs = reinterpret_cast<F*>(smem_raw) + size_t(wave) * rb * d;
+    const int total = *count_p;
+    const int nbatch = (total + rb - 1) / rb;
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0;
+    for (int b = gw; b < nbatch; b += nw) {
+        const int e0 = b * rb;
+        const int nr = min(rb, total - e0);
+        int my_row = 0, my_cnt = 0;
+        if (lane < nr) {
+            my_row = rows[e0 + lane];
+            my_cnt = counts[e0 + lane];
+            if (my_cnt > stride || my_cnt <= 0) {
+                full_rows[atomicAdd(full_counter, 1)] = my_row;
+                my_cnt = 0;
             }
         }
-        labels[row] = bi;
+        // stage the rows (lanes over features)
+        for (int r = 0; r < nr; ++r) {
+            const int64_t row = __shfl(my_row, r);
+            for (int f = lane; f < d; f += 64) xs[r * d + f] = X[row * d + f];
+        }
+        // exclusive scan of the pair counts over lanes 0..nr-1
+        int incl = my_cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int my_start = incl - my_cnt;
+        const int npairs = __shfl(incl, 63);
+        pstart[wave][lane] = my_start;
+        __builtin_amdgcn_wave_barrier();
+        F best = fp_traits<F>::inf;
+        int bj = 0x7fffffff;
+        for (int p0 = 0; p0 < npairs; p0 += 64) {
+            const int p = p0 + lane;
+            F dist = fp_traits<F>::inf;
+            int j = 0x7fffffff;
+            if (p < npairs) {
+                // entry owning pair p: last r with start[r] <= p
+                int r = 0;
+                for (int q = 1; q < nr; ++q)
+                    if (pstart[wave][q] <= p) r = q;
+                j = ids[int64_t(e0 + r) * stride + (p - pstart[wave][r])];
+                const F* x = xs + r * d;
+                const F* c = C + int64_t(j) * d;
+                F acc = F(0);
+                if (vec) {
+                    for (int f = 0; f < d; f += 4) {
+                        typedef F F4 __attribute__((ext_vector_type(4)));
+                        const F4 c4 = *reinterpret_cast<const F4*>(c + f);
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const F diff = x[f + t] - c4[t];
+                            const F sq = diff * diff;
+                            acc = acc + sq;
+                        }
+                    }
+                } else {
+                    for (int f = 0; f < d; ++f) {
+                        const F diff = x[f] - c[f];
+                        const F sq = diff * diff;
+                        acc = acc + sq;
+                    }
+                }
+                dist = acc;
+            }
+            pd[wave][lane] = dist;
+            pj[wave][lane] = j;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nr && my_cnt > 0) {
+                const int q0 = max(my_start, p0), q1 = min(my_start + my_cnt, p0 + 64);
+                for (int q = q0; q < q1; ++q) {
+                    const F v = pd[wave][q - p0];
+                    const int jj = pj[wave][q - p0];
+                    if (v < best || (v == best && jj < bj)) {
+                        best = v;
+                        bj = jj;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane < nr && my_cnt > 0) labels[my_row] = bj;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 

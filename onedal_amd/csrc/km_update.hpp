@@ -303,49 +303,49 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
 // objective: d64 per row against its label's centroid.  Order (fixed): rows in
 // chunks of OBJ_CHUNK; thread t of a chunk sums rows t, t+256, t+512, ... in
 // order; the 256 partials are combined by the tree p[t] += p[t+s], s = 128..1;
-// chunk sums are added in chunk order by sum_chunks_kernel.  Rows are staged
-// through LDS (coalesced) before each thread walks its row's features in order.
+// chunk sums are added in chunk order by sum_sequential_kernel.
 // ---------------------------------------------------------------------------
 template <typename F>
 __global__ __launch_bounds__(256) void objective_chunks_kernel(
     const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C,
     const int32_t* __restrict__ labels, double* __restrict__ d64_out,
     double* __restrict__ chunk_sums) {
-    constexpr int STAGE_BYTES = 32768;
-    __shared__ __align__(16) unsigned char stage_raw[STAGE_BYTES];
     __shared__ double part[256];
-    F* xs = reinterpret_cast<F*>(stage_raw);
-    const int ld = d + 1;
-    const int rows_sub = max(1, min(256, int(STAGE_BYTES / sizeof(F)) / ld));
     const int t = threadIdx.x;
+    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(F))) == 0 &&
+                     (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0;
+    typedef F F4 __attribute__((ext_vector_type(4)));
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     for (int64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
         const int64_t r0 = q * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
         double p = 0.0;
-        // rows t + 256*i are visited in increasing i; a sub-tile holds rows_sub rows
-        for (int s0 = 0; s0 < rows; s0 += rows_sub) {
-            const int sr = min(rows_sub, rows - s0);
-            __syncthreads();
-            for (int e = t; e < sr * d; e += 256) {
-                const int r = e / d, f = e - (e / d) * d;
-                xs[r * ld + f] = X[(r0 + s0 + r) * d + f];
-            }
-            __syncthreads();
-            // thread t owns chunk rows congruent to t mod 256
-            // the one row of this sub-tile with (s0 + r) % 256 == t, if any
-            for (int r = (t - (s0 & 255) + 256) & 255; r < sr; r += 256) {
-                const int64_t row = r0 + s0 + r;
-                const F* c = C + int64_t(labels[row]) * d;
-                double acc = 0.0;
+        // thread t owns chunk rows t, t+256, ... in increasing order
+        for (int r = t; r < rows; r += 256) {
+            const int64_t row = r0 + r;
+            const F* x = X + row * d;
+            const F* c = C + int64_t(labels[row]) * d;
+            double acc = 0.0;
+            if (vec) {
+                for (int f = 0; f < d; f += 4) {
+                    const F4 x4 = *reinterpret_cast<const F4*>(x + f);
+                    const F4 c4 = *reinterpret_cast<const F4*>(c + f);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double diff = double(x4[u]) - double(c4[u]);
+                        const double sq = diff * diff;
+                        acc = acc + sq;
+                    }
+                }
+            } else {
                 for (int f = 0; f < d; ++f) {
-                    const double diff = double(xs[r * ld + f]) - double(c[f]);
+                    const double diff = double(x[f]) - double(c[f]);
                     const double sq = diff * diff;
                     acc = acc + sq;
                 }
-                if (d64_out) d64_out[row] = acc;
-                p = p + acc;
             }
+            if (d64_out) d64_out[row] = acc;
+            p = p + acc;
         }
         part[t] = p;
         __syncthreads();

@@ -1,0 +1,121 @@
+// Device checks of the oneapi::dal::kmeans drop-in surface: the reference's gold
+// dataset (test/fixture.hpp:156-203), its empty-cluster fixture
+// (fixture.hpp:205-219) and its smoke-data inference objective
+// (fixture.hpp:221-252), through dal::train / dal::infer on a HIP queue.
+// argv[1]: text file "n d k" then data, initial centroids, expected centroids,
+// expected responses, expected objective (whitespace separated).
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <vector>
+
+#include "oneapi/dal/algo/kmeans.hpp"
+
+namespace dal = oneapi::dal;
+namespace km = oneapi::dal::kmeans;
+
+static int failures = 0;
+#define CHECK(cond)                                                              \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);          \
+            failures++;                                                          \
+        }                                                                        \
+    } while (0)
+
+template <typename F>
+static void gold(const char* path, dal::hip::queue& q) {
+    std::ifstream in(path);
+    int64_t n, d, k;
+    in >> n >> d >> k;
+    std::vector<F> x(n * d), c0(k * d), ec(k * d);
+    std::vector<int> er(n);
+    double eobj;
+    for (auto& v : x) in >> v;
+    for (auto& v : c0) in >> v;
+    for (auto& v : ec) in >> v;
+    for (auto& v : er) in >> v;
+    in >> eobj;
+    const auto desc = km::descriptor<F>().set_cluster_count(k).set_max_iteration_count(100).set_accuracy_threshold(0.0);
+    const auto res = dal::train(q, desc, dal::homogen_table::wrap(x.data(), n, d),
+                                dal::homogen_table::wrap(c0.data(), k, d));
+    const auto c = dal::row_accessor<const F>(res.get_model().get_centroids()).pull();
+    const auto r = dal::row_accessor<const int>(res.get_responses()).pull();
+    // match map (fixture.hpp:677-706) and tolerances (fixture.hpp:182-202)
+    std::vector<int> mm(k);
+    for (int64_t i = 0; i < k; ++i) {
+        double best = 1e300;
+        for (int64_t j = 0; j < k; ++j) {
+            double s = 0;
+            for (int64_t f = 0; f < d; ++f) s += (ec[j * d + f] - c[i * d + f]) * (ec[j * d + f] - c[i * d + f]);
+            if (s < best) { best = s; mm[i] = int(j); }
+        }
+    }
+    for (int64_t i = 0; i < k; ++i)
+        for (int64_t f = 0; f < d; ++f) {
+            const double l = ec[mm[i] * d + f], v = c[i * d + f];
+            CHECK(std::fabs(l - v) / (std::fabs(l) + std::fabs(v) + 1e-300) < 1e-7);
+        }
+    for (int64_t i = 0; i < n; ++i) CHECK(er[i] == mm[r[i]]);
+    CHECK(std::fabs(res.get_objective_function_value() - eobj) / eobj < 1e-7);
+    CHECK(res.get_iteration_count() == 100);
+    // inference with the objective
+    const auto idesc = km::descriptor<F>(k).set_result_options(km::result_options::compute_exact_objective_function);
+    const auto ir = dal::infer(q, idesc, res.get_model(), dal::homogen_table::wrap(x.data(), n, d));
+    const auto ri = dal::row_accessor<const int>(ir.get_responses()).pull();
+    for (int64_t i = 0; i < n; ++i) CHECK(ri[i] == r[i]);
+    CHECK(ir.get_objective_function_value() == res.get_objective_function_value());
+}
+
+template <typename F>
+static void empty_clusters(dal::hip::queue& q) {
+    F data[] = {-10, -9.5, -9, -8.5, -8, -1, 1, 9, 9.5, 10};
+    F init[] = {-10, -10, -10};
+    const auto res = dal::train(q, km::descriptor<F>(3).set_max_iteration_count(1),
+                                dal::homogen_table::wrap(data, 10, 1), dal::homogen_table::wrap(init, 3, 1));
+    const auto c = dal::row_accessor<const F>(res.get_model().get_centroids()).pull();
+    const double want[] = {-1.65, 10, 9.5};
+    for (int j = 0; j < 3; ++j) CHECK(std::fabs(c[j] - want[j]) <= 1e-6 * std::fabs(want[j]));
+    const auto r = dal::row_accessor<const int>(res.get_responses()).pull();
+    const int wr[] = {0, 0, 0, 0, 0, 0, 0, 2, 2, 1};
+    for (int i = 0; i < 10; ++i) CHECK(r[i] == wr[i]);
+}
+
+static void smoke_infer(dal::hip::queue& q) {
+    // fixture.hpp:221-252: the model {(-1.5,-1.5),(1.5,1.5)} on the smoke data has objective 4
+    float x[] = {1, 1, 2, 2, 1, 2, 2, 1, -1, -1, -1, -2, -2, -1, -2, -2};
+    float c[] = {-1.5f, -1.5f, 1.5f, 1.5f};
+    const auto m = km::model<>().set_centroids(dal::homogen_table::wrap(c, 2, 2));
+    const auto desc = km::descriptor<float>(2).set_result_options(km::result_options::compute_exact_objective_function);
+    const auto ir = dal::infer(q, desc, m, dal::homogen_table::wrap(x, 8, 2));
+    CHECK(std::fabs(ir.get_objective_function_value() - 4.0) < 1e-5 * 4.0);
+    const auto r = dal::row_accessor<const int>(ir.get_responses()).pull();
+    const int wr[] = {1, 1, 1, 1, 0, 0, 0, 0};
+    for (int i = 0; i < 8; ++i) CHECK(r[i] == wr[i]);
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    dal::hip::queue q(0);
+    gold<float>(argv[1], q);
+    gold<double>(argv[1], q);
+    empty_clusters<float>(q);
+    empty_clusters<double>(q);
+    smoke_infer(q);
+    dal::hip::queue qb(0);
+    qb.set_bf16_refine(true);
+    gold<float>(argv[1], qb);
+    empty_clusters<float>(qb);
+    // no queue: default device
+    float data[] = {0, 0, 0.1f, 0, 5, 5, 5.1f, 5};
+    float init[] = {0, 0, 5, 5};
+    const auto res = dal::train(km::descriptor<float>(2).set_max_iteration_count(3),
+                                dal::homogen_table::wrap(data, 4, 2), dal::homogen_table::wrap(init, 2, 2));
+    CHECK(res.get_iteration_count() == 3);
+    if (failures) {
+        std::printf("%d failure(s)\n", failures);
+        return 1;
+    }
+    std::printf("api device checks passed\n");
+    return 0;
+}

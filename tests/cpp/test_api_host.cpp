@@ -1,0 +1,123 @@
+// Host-side checks of the oneapi::dal::kmeans drop-in surface (no device call):
+// descriptor setters, preconditions, result-option behaviour and tables, with the
+// reference's exception types and messages (test/badarg.cpp:114-218,
+// common.cpp:82-115, train_types.cpp:141-154, infer_types.cpp:91-110).
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+
+#include "oneapi/dal/algo/kmeans.hpp"
+
+namespace dal = oneapi::dal;
+namespace km = oneapi::dal::kmeans;
+
+static int failures = 0;
+#define CHECK(cond)                                                              \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);          \
+            failures++;                                                          \
+        }                                                                        \
+    } while (0)
+
+template <typename E>
+static void expect_throw(const std::function<void()>& fn, const char* msg, int line) {
+    try {
+        fn();
+        std::printf("FAIL line %d: no exception\n", line);
+        failures++;
+    } catch (const E& e) {
+        if (msg && std::string(e.what()) != msg) {
+            std::printf("FAIL line %d: message '%s' != '%s'\n", line, e.what(), msg);
+            failures++;
+        }
+    } catch (const std::exception& e) {
+        std::printf("FAIL line %d: wrong exception type: %s\n", line, e.what());
+        failures++;
+    }
+}
+#define EXPECT_THROW(E, expr, msg) expect_throw<E>([&] { expr; }, msg, __LINE__)
+
+int main() {
+    using namespace dal::detail::error_messages;
+    // descriptor defaults and setters
+    km::descriptor<float> d;
+    CHECK(d.get_cluster_count() == 2);
+    CHECK(d.get_max_iteration_count() == 100);
+    CHECK(d.get_accuracy_threshold() == 0.0);
+    CHECK(!bool(d.get_result_options()));
+    EXPECT_THROW(dal::domain_error, d.set_cluster_count(0), cluster_count_leq_zero());
+    EXPECT_THROW(dal::domain_error, d.set_cluster_count(-1), cluster_count_leq_zero());
+    EXPECT_THROW(dal::domain_error, d.set_cluster_count(int64_t(1) << 40), cluster_count_gt_max_int32());
+    EXPECT_THROW(dal::domain_error, d.set_max_iteration_count(-1), max_iteration_count_lt_zero());
+    EXPECT_THROW(dal::domain_error, d.set_accuracy_threshold(-0.5), accuracy_threshold_lt_zero());
+    EXPECT_THROW(dal::domain_error, d.set_result_options(km::result_option_id{}), empty_set_of_result_options());
+    // std:: base types are catchable too
+    EXPECT_THROW(std::domain_error, km::descriptor<double>(0), nullptr);
+    d.set_cluster_count(3).set_max_iteration_count(5).set_accuracy_threshold(0.01);
+    CHECK(d.get_cluster_count() == 3 && d.get_max_iteration_count() == 5);
+
+    // train preconditions, in the reference's order (detail/train_ops.hpp:40-61)
+    float x[20];
+    for (int i = 0; i < 20; ++i) x[i] = float(i);
+    float c3[6] = {0, 0, 1, 1, 2, 2};
+    float c2[4] = {0, 0, 1, 1};
+    float c3x3[9] = {0};
+    const auto X = dal::homogen_table::wrap(x, 10, 2);
+    EXPECT_THROW(dal::domain_error, dal::train(d, dal::table{}, dal::homogen_table::wrap(c3, 3, 2)),
+                 input_data_is_empty());
+    EXPECT_THROW(dal::invalid_argument,
+                 dal::train(km::descriptor<float>(11), X, dal::homogen_table::wrap(c3, 3, 2)),
+                 cluster_count_exceeds_data_row_count());
+    EXPECT_THROW(dal::invalid_argument, dal::train(d, X, dal::homogen_table::wrap(c2, 2, 2)),
+                 input_initial_centroids_rc_neq_desc_cluster_count());
+    EXPECT_THROW(dal::invalid_argument, dal::train(d, X, dal::homogen_table::wrap(c3x3, 3, 3)),
+                 input_initial_centroids_cc_neq_input_data_cc());
+
+    // infer preconditions (detail/infer_ops.hpp:40-56)
+    km::model<> empty_model;
+    EXPECT_THROW(dal::domain_error, dal::infer(d, empty_model, dal::table{}), input_data_is_empty());
+    EXPECT_THROW(dal::domain_error, dal::infer(d, empty_model, X), input_model_centroids_are_empty());
+    const auto m2 = km::model<>().set_centroids(dal::homogen_table::wrap(c2, 2, 2));
+    EXPECT_THROW(dal::invalid_argument, dal::infer(d, m2, X),
+                 input_model_centroids_rc_neq_desc_cluster_count());
+    const auto m33 = km::model<>().set_centroids(dal::homogen_table::wrap(c3x3, 3, 3));
+    EXPECT_THROW(dal::invalid_argument, dal::infer(d, m33, X),
+                 input_model_centroids_cc_neq_input_data_cc());
+
+    // results
+    km::train_result<> tr;
+    EXPECT_THROW(dal::domain_error, tr.set_iteration_count(-1), iteration_count_lt_zero());
+    EXPECT_THROW(dal::domain_error, tr.set_objective_function_value(-1.0), objective_function_value_lt_zero());
+    km::infer_result<> ir;
+    EXPECT_THROW(dal::domain_error, ir.get_objective_function_value(),
+                 this_result_is_not_enabled_via_result_options());
+    EXPECT_THROW(dal::domain_error, ir.set_objective_function_value(1.0),
+                 this_result_is_not_enabled_via_result_options());
+    ir.set_result_options(km::result_options::compute_exact_objective_function);
+    ir.set_objective_function_value(4.0);
+    CHECK(ir.get_objective_function_value() == 4.0);
+    EXPECT_THROW(dal::domain_error, ir.set_objective_function_value(-1.0), objective_function_value_lt_zero());
+
+    // tables and accessors, with conversion
+    CHECK(X.has_data() && X.get_row_count() == 10 && X.get_column_count() == 2);
+    const auto rows = dal::row_accessor<const double>(X).pull({2, 4});
+    CHECK(rows.get_count() == 4 && rows[0] == 4.0 && rows[3] == 7.0);
+    int32_t lab[3] = {2, 0, 1};
+    const auto L = dal::homogen_table::wrap(lab, 3, 1);
+    const auto lf = dal::row_accessor<const float>(L).pull();
+    CHECK(lf[0] == 2.0f && lf[2] == 1.0f);
+    EXPECT_THROW(dal::out_of_range, dal::row_accessor<const float>(L).pull({1, 5}), nullptr);
+    // model shares its table on copy (pimpl semantics)
+    auto m = km::model<>().set_centroids(dal::homogen_table::wrap(c3, 3, 2));
+    const auto m_copy = m;
+    CHECK(m_copy.get_cluster_count() == 3);
+
+    if (failures) {
+        std::printf("%d failure(s)\n", failures);
+        return 1;
+    }
+    std::printf("api host checks passed\n");
+    return 0;
+}
