@@ -189,16 +189,46 @@ __global__ void bf16_params_kernel(const unsigned long long* __restrict__ cmax_b
     *out = p;
 }
 
-// LDS bytes of assign_bf16_kernel
+// cnB_j = ||c_j||^2 + B rounded up (v_j keeps its lower bound); padded ids get
+// a huge start value so they are never chosen.  kp = 32 * ceil(k / 32) entries.
+__global__ void cnb_kernel(const float* __restrict__ cn, int k, int kp,
+                           const bf16_params* __restrict__ params_p, float* __restrict__ cnb) {
+    const float Bf = float(params_p->B);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < kp; j += gridDim.x * blockDim.x)
+        cnb[j] = j < k ? add_round_up(cn[j], Bf) : __uint_as_float(0x7F000000u);
+}
+
+// Centroid blocks live in a ring of BF16_RING LDS buffers, filled BF16_RING-1
+// blocks ahead with direct-to-LDS loads.
+constexpr int BF16_RING = 4;
+
+// 16 bytes per lane from global memory straight into LDS: lane l writes
+// lds_wave_base + 16 l (wave-uniform base, through M0).  Issued as inline asm so
+// the compiler does not insert its own conservative vmcnt(0) before every LDS
+// read; ordering is by the explicit vmcnt waits and barriers of the caller.
+__device__ __forceinline__ void lds_dma16(const void* gsrc, void* lds_wave_base) {
+    // the base is wave-uniform; readfirstlane tells the compiler so (SGPR operand)
+    const uint32_t m0 = __builtin_amdgcn_readfirstlane(uint32_t(reinterpret_cast<uintptr_t>(lds_wave_base)));
+    asm volatile(
+        "s_mov_b32 m0, %0\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %1, off"
+        :
+        : "s"(m0), "v"(gsrc)
+        : "memory", "m0");
+}
+
+// LDS bytes of assign_bf16_kernel: ring of 32 x dpb bf16 blocks + their cnB
 __host__ inline size_t bf16_lds_bytes(int dpb) {
-    const int cst = dpb + 8;  // bf16 elements per LDS row
-    return size_t(2) * 32 * cst * 2 + 2 * 32 * 4;
+    return size_t(BF16_RING) * (size_t(32) * dpb * 2 + 32 * 4);
 }
 
 // ---------------------------------------------------------------------------
 // Main / collect kernel.  Workgroup = 4 waves; each wave owns NB blocks of 32
-// rows (B operand in VGPRs, 8 bf16 per step); centroid blocks of 32 rows are
-// double-buffered through LDS (A operand) with a register prefetch.
+// rows (B operand in VGPRs, 8 bf16 per step); centroid blocks of 32 rows (A
+// operand) stream through an LDS ring by global_load_lds_dwordx4, three blocks
+// ahead of use.  A block is stored unpadded, 16-byte chunk c of row r at chunk
+// c ^ (r & SWZ), so the 32 lanes reading one chunk column hit distinct banks.
 // Lane l = 32*h + r: half h covers features [h*DPB/2, (h+1)*DPB/2), 8 per step.
 // Slots: lane keeps 4 (centroid block index mod 4); a row has 8.
 // COLLECT: rows come from L.coll_rows; every j with v_j <= T is recorded.
@@ -206,19 +236,23 @@ __host__ inline size_t bf16_lds_bytes(int dpb) {
 template <int DPB, int NB, bool COLLECT>
 __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     const uint16_t* __restrict__ Xb, int64_t n, const uint16_t* __restrict__ Cb, int k,
-    const float* __restrict__ cn, const float* __restrict__ xnorm_up,
+    const float* __restrict__ cnb_g, const float* __restrict__ xnorm_up,
     const float* __restrict__ enorm_up, const bf16_params* __restrict__ params_p,
     bf16_consts kc, int32_t* __restrict__ labels, refine_lists L) {
     constexpr int HALF = DPB / 2;
     constexpr int STEPS = HALF / 8;
-    constexpr int CST = DPB + 8;  // LDS row stride (bf16)
     constexpr int ROWS_PER_WAVE = 32 * NB;
     constexpr int ROWS_PER_WG = 4 * ROWS_PER_WAVE;
-    constexpr int U4_PER_ROW = DPB / 8;  // 16-byte pieces per centroid row
+    constexpr int CPR = DPB / 8;                  // 16-byte chunks per centroid row
+    constexpr int SWZ = (CPR < 16 ? CPR : 16) - 1;  // chunk swizzle mask
+    constexpr int ROWB = DPB * 2;                 // bytes per centroid row
+    constexpr int BLKB = 32 * ROWB;               // bytes per block
+    constexpr int NCOPY = BLKB / 1024 + 1;        // 1 KB copies per block (+1: cnB)
+    constexpr int CPW = (NCOPY + 3) / 4;          // copies per wave per block (uniform)
     constexpr int NSL = 4;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    uint16_t* Cs = reinterpret_cast<uint16_t*>(smem_raw);                     // [2][32][CST]
-    float* Cn = reinterpret_cast<float*>(smem_raw + size_t(2) * 32 * CST * 2);  // [2][32]
+    unsigned char* ring = smem_raw;                                 // [RING][BLKB]
+    unsigned char* ring_cn = smem_raw + size_t(BF16_RING) * BLKB;   // [RING][32] floats
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -226,43 +260,51 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     const int r32 = lane & 31;
     const int h = lane >> 5;
     const bf16_params P = *params_p;
-    const float Bf = float(P.B);
     const int mblocks = (k + 31) / 32;
     const uint32_t kmask = 0x7FFFFFFFu & ~((1u << kc.ib) - 1u);
     const int64_t nrows = COLLECT ? int64_t(L.counters[2]) : n;
     const int64_t ntiles = (nrows + ROWS_PER_WG - 1) / ROWS_PER_WG;
 
-    constexpr int STG = (32 * U4_PER_ROW + 255) / 256;
-    uintx4 pre[STG];
-    float pre_cn = 0.f;
-    auto prefetch = [&](int mb) {
+    // Direct-to-LDS copies of block mb into ring slot `slot`.  Copy q (q < NCOPY-1)
+    // fills LDS bytes [q KB, (q+1) KB) of the block; lane l writes 16-byte chunk
+    // p = 64 q + l, i.e. physical row p / CPR, physical chunk p % CPR, which holds
+    // logical chunk (p % CPR) ^ (row & SWZ).  Copy NCOPY-1 moves the 32 cnB values.
+    // Every wave issues exactly CPW copies per block (repeating copy 0 when it has
+    // fewer; the repeat writes the same bytes), so vmcnt waits are uniform.
+    auto issue_block = [&](int mb, int slot) {
 #pragma unroll
-        for (int q = 0; q < STG; ++q) {
-            const int e = tid + q * 256;
-            const int row = e / U4_PER_ROW, c8 = e % U4_PER_ROW;
-            const int cj = mb * 32 + row;
-            uintx4 v = {0u, 0u, 0u, 0u};
-            if (e < 32 * U4_PER_ROW && cj < k)
-                v = *reinterpret_cast<const uintx4*>(Cb + int64_t(cj) * DPB + c8 * 8);
-            pre[q] = v;
-        }
-        if (tid < 32) {
-            const int cj = mb * 32 + tid;
-            // cnB rounded up so that v_j keeps its lower bound; padded rows are never chosen
-            pre_cn = cj < k ? add_round_up(cn[cj], Bf) : __uint_as_float(0x7F000000u);
-        }
-    };
-    auto commit = [&](int b) {
-#pragma unroll
-        for (int q = 0; q < STG; ++q) {
-            const int e = tid + q * 256;
-            if (e < 32 * U4_PER_ROW) {
-                const int row = e / U4_PER_ROW, c8 = e % U4_PER_ROW;
-                *reinterpret_cast<uintx4*>(&Cs[(b * 32 + row) * CST + c8 * 8]) = pre[q];
+        for (int c = 0; c < CPW; ++c) {
+            int q = wave + 4 * c;
+            if (q >= NCOPY) q = 0;
+            if (q < NCOPY - 1) {
+                const int p = q * 64 + lane;
+                const int prow = p / CPR, pc = p % CPR;
+                const int lc = pc ^ (prow & SWZ);
+                const int cj = min(mb * 32 + prow, k - 1);
+                const uint16_t* src = Cb + int64_t(cj) * DPB + lc * 8;
+                lds_dma16(src, ring + slot * BLKB + q * 1024);
+            } else if (lane < 8) {
+                lds_dma16(cnb_g + mb * 32 + lane * 4, ring_cn + slot * 128);
             }
         }
-        if (tid < 32) Cn[b * 32 + tid] = pre_cn;
     };
+    // wait until at most `pending` blocks' copies of this wave are in flight
+    auto wait_blocks = [&](int pending) {
+        if (COLLECT || pending == 0) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        } else if (pending == 1) {
+            if constexpr (CPW == 1) asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+            else if constexpr (CPW == 2) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+            else if constexpr (CPW == 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
+        } else {
+            if constexpr (CPW == 1) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+            else if constexpr (CPW == 2) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+            else if constexpr (CPW == 3) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(10)" ::: "memory");
+        }
+    };
+    static_assert(CPW <= 5, "copy count per wave");
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t w0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
@@ -296,14 +338,16 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 m2[nb][q] = 0xFFFFFFFFu;
             }
 
-        auto block = [&](int b, int mb, int sl) {
-            const float* cnb = Cn + b * 32;
-            const uint16_t* arow = &Cs[(b * 32 + r32) * CST + h * HALF];
+        auto block = [&](int slot, int mb, int sl) {
+            const float* cnb = reinterpret_cast<const float*>(ring_cn + slot * 128);
+            const unsigned char* arow = ring + slot * BLKB + r32 * ROWB;
             // all A fragments and start values of the block first, then the MFMA chain
             bf16x8 a[STEPS];
 #pragma unroll
-            for (int s = 0; s < STEPS; ++s)
-                a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uintx4*>(arow + 8 * s));
+            for (int s = 0; s < STEPS; ++s) {
+                const int pc = (h * (HALF / 8) + s) ^ (r32 & SWZ);
+                a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uintx4*>(arow + pc * 16));
+            }
             floatx16 init;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -358,29 +402,29 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             }
         };
 
-        // block mb runs from LDS buffer mb % 2 into slot mb % 4; the next block is
-        // prefetched into registers meanwhile and committed to the other buffer
-        auto step = [&](int mb, int b, int sl) {
-            const bool more = mb + 1 < mblocks;
-            if (more) prefetch(mb + 1);
-            block(b, mb, sl);
-            if (more) commit(b ^ 1);
+        // step mb: wait for block mb's copies (blocks mb+1, mb+2 may stay in flight),
+        // barrier (all waves' copies visible; everyone is done with block mb-1),
+        // refill the slot of block mb-1 with block mb+3, then run block mb
+        auto step = [&](int mb, int sl) {
+            wait_blocks(min(2, mblocks - 1 - mb));
             __syncthreads();
+            if (mb + 3 < mblocks) issue_block(mb + 3, (mb + 3) & (BF16_RING - 1));
+            block(mb & (BF16_RING - 1), mb, sl);
         };
+        // row loads complete here (a waitcnt the compiler sees, so it does not re-wait
+        // for them inside the block loop, where vmcnt also counts the ring copies);
+        // the previous tile's last readers are done before the prologue refills the ring
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
-        prefetch(0);
-        commit(0);
-        __syncthreads();
-        int mb = 0;
-        for (; mb + 3 < mblocks; mb += 4) {
-            step(mb, 0, 0);
-            step(mb + 1, 1, 1);
-            step(mb + 2, 0, 2);
-            step(mb + 3, 1, 3);
+        issue_block(0, 0);
+        if (1 < mblocks) issue_block(1, 1);
+        if (2 < mblocks) issue_block(2, 2);
+        for (int mb = 0; mb < mblocks; mb += 4) {
+            step(mb, 0);
+            if (mb + 1 < mblocks) step(mb + 1, 1);
+            if (mb + 2 < mblocks) step(mb + 2, 2);
+            if (mb + 3 < mblocks) step(mb + 3, 3);
         }
-        if (mb < mblocks) step(mb, 0, 0);
-        if (mb + 1 < mblocks) step(mb + 1, 1, 1);
-        if (mb + 2 < mblocks) step(mb + 2, 0, 2);
         if (COLLECT) continue;
 
         // settle rows: lane r32 (h = 0) merges its 4 slots with lane r32 + 32's 4

@@ -95,7 +95,7 @@ struct km_ctx {
     bool timing = false;
 
     // workspace
-    km::dbuf x_bf16, enorm, xmax, c_bf16, params, coll_rows, coll_T, coll_count, coll_ids;
+    km::dbuf x_bf16, enorm, xmax, c_bf16, cnb, params, coll_rows, coll_T, coll_count, coll_ids;
     km::dbuf x_dev, x_pad, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
@@ -149,6 +149,7 @@ struct work {
     float* enorm = nullptr;       // ||x - bf16(x)|| (up)
     unsigned long long* xmax_bits = nullptr;
     uint16_t* Cb = nullptr;       // k x dpad bf16(-2 c)
+    float* cnb = nullptr;         // ||c||^2 + B (rounded up), 32 * ceil(k / 32) entries
     bf16_params* params = nullptr;
 };
 
@@ -161,10 +162,11 @@ static int pad_dim(int d) {
 // 1 generic tile kernel, 2 FP32 matrix cores, 3 BF16 matrix cores + exact re-score.
 // BF16 applies to float data with d <= 256; double data always takes path 1.
 template <typename F>
-static int choose_path(const km_ctx* ctx, int d, uint32_t flags) {
+static int choose_path(const km_ctx* ctx, int d, int k, uint32_t flags) {
     if (std::is_same<F, double>::value) return 1;
     if (d > 256) return 1;
-    if (flags & KM_FLAG_BF16_REFINE) return 3;
+    // keys carry (block, register) in at most 15 low bits: k <= 65536
+    if ((flags & KM_FLAG_BF16_REFINE) && k <= 65536) return 3;
     if (ctx->assign_path_opt == 1) return 1;
     return 2;
 }
@@ -207,6 +209,10 @@ static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, int path) {
             KM_LAUNCH_CHECK();
             hipLaunchKernelGGL(bf16_params_kernel, dim3(1), dim3(64), 0, ctx->stream, w.cmax_bits,
                                w.xmax_bits, make_bf16_consts(w.d, w.k), w.params);
+            KM_LAUNCH_CHECK();
+            const int kp = (w.k + 31) / 32 * 32;
+            hipLaunchKernelGGL(cnb_kernel, dim3(grid_for(ctx, kp, 256, 1)), dim3(256), 0, ctx->stream,
+                               w.cn, w.k, kp, w.params, w.cnb);
             KM_LAUNCH_CHECK();
             return;
         }
@@ -252,12 +258,12 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
         const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
         const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
         hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, false>), dim3(grid), dim3(256), lds,
-                           ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cn, w.xnorm, w.enorm, w.params, kc,
+                           ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params, kc,
                            w.labels, w.L);
     } else {
         // row count is on the device; a fixed grid strides over it
         hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, true>), dim3(ctx->cu_count * 2), dim3(256),
-                           lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cn, w.xnorm, w.enorm, w.params,
+                           lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params,
                            kc, w.labels, w.L);
     }
     KM_LAUNCH_CHECK();
@@ -526,6 +532,7 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
             w.enorm = ctx->enorm.get<float>(n);
             w.xmax_bits = ctx->xmax.get<unsigned long long>(2);
             w.Cb = ctx->c_bf16.get<uint16_t>(int64_t(k) * w.dpad);
+            w.cnb = ctx->cnb.get<float>((k + 31) / 32 * 32);
             w.params = ctx->params.get<bf16_params>(1);
             w.L.coll_rows = ctx->coll_rows.get<int32_t>(n);
             w.L.coll_T = ctx->coll_T.get<double>(n);
@@ -562,7 +569,7 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
         Xd = xb;
     }
     work<F> w;
-    const int path = choose_path<F>(ctx, d, flags);
+    const int path = choose_path<F>(ctx, d, k, flags);
     setup_work<F>(ctx, w, Xd, n, d, k, path);
     ctx->stats.assign_path = path;
     F* Ccur = ctx->c_cur.get<F>(int64_t(k) * d);
@@ -677,7 +684,7 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
         Xd = xb;
     }
     work<F> w;
-    const int path = choose_path<F>(ctx, d, flags);
+    const int path = choose_path<F>(ctx, d, k, flags);
     setup_work<F>(ctx, w, Xd, n, d, k, path);
     ctx->stats.assign_path = path;
     F* Cd = ctx->c_cur.get<F>(int64_t(k) * d);
@@ -798,7 +805,7 @@ km_status km_context_destroy(km_ctx* ctx) {
     if (!ctx) return KM_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    km::dbuf* bufs[] = {&ctx->x_bf16, &ctx->enorm, &ctx->xmax, &ctx->c_bf16, &ctx->params,
+    km::dbuf* bufs[] = {&ctx->x_bf16, &ctx->enorm, &ctx->xmax, &ctx->c_bf16, &ctx->cnb, &ctx->params,
                         &ctx->coll_rows, &ctx->coll_T, &ctx->coll_count, &ctx->coll_ids, &ctx->x_dev, &ctx->x_pad, &ctx->c_cur, &ctx->c_next, &ctx->cn, &ctx->cpad, &ctx->cmax,
                         &ctx->xnorm, &ctx->labels, &ctx->lists_rows, &ctx->lists_cnt,
                         &ctx->lists_ids, &ctx->lists_full, &ctx->lists_counters, &ctx->counts,
