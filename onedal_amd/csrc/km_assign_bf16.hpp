@@ -82,6 +82,12 @@ __device__ __forceinline__ double bf16_threshold(const bf16_consts& k, const bf1
     return m1 + band + (fabs(m1) + X * X + p.C * p.C) * 4.44e-16;
 }
 
+// Centroid count padded to whole groups of four 32-centroid blocks: the block
+// loop of the assignment kernel then runs without tail conditions.  Padding
+// centroids are zero rows with a huge start value (never chosen, never inside a
+// threshold).
+__host__ __device__ inline int bf16_padded_k(int k) { return (k + 127) / 128 * 128; }
+
 // ---------------------------------------------------------------------------
 // X -> padded BF16 rows (stride DPB), ||x|| and ||x - bf16(x)|| (rounded up),
 // and their maxima.  One wave per row.
@@ -130,15 +136,19 @@ __global__ __launch_bounds__(256) void convert_rows_bf16_kernel(
 
 // ---------------------------------------------------------------------------
 // centroids -> bf16(-2 c) padded rows, ||c||^2 (F), maxima of ||c||,
-// ||bf16(c)||, ||c - bf16(c)||.  One wave per centroid.
+// ||bf16(c)||, ||c - bf16(c)||.  One wave per centroid.  Rows k .. kp-1
+// (kp = bf16_padded_k(k)) are zero, so the last centroid blocks read defined
+// values without clamping.
 // max_bits: [0] C, [1] Ch, [2] Ec
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void centroid_prep_bf16_kernel(
-    const float* __restrict__ C, int k, int d, int dpb, uint16_t* __restrict__ Cb,
+    const float* __restrict__ C, int k, int kp, int d, int dpb, uint16_t* __restrict__ Cb,
     float* __restrict__ cn, unsigned long long* __restrict__ max_bits) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int j = k + wave; j < kp; j += nwaves)
+        for (int f = lane; f < dpb; f += 64) Cb[int64_t(j) * dpb + f] = 0;
     for (int j = wave; j < k; j += nwaves) {
         double s = 0.0, sh = 0.0, se = 0.0;
         for (int f = lane; f < dpb; f += 64) {
@@ -190,7 +200,7 @@ __global__ void bf16_params_kernel(const unsigned long long* __restrict__ cmax_b
 }
 
 // cnB_j = ||c_j||^2 + B rounded up (v_j keeps its lower bound); padded ids get
-// a huge start value so they are never chosen.  kp = 32 * ceil(k / 32) entries.
+// a huge start value so they are never chosen.  kp = bf16_padded_k(k) entries.
 __global__ void cnb_kernel(const float* __restrict__ cn, int k, int kp,
                            const bf16_params* __restrict__ params_p, float* __restrict__ cnb) {
     const float Bf = float(params_p->B);
@@ -202,25 +212,73 @@ __global__ void cnb_kernel(const float* __restrict__ cn, int k, int kp,
 // blocks ahead with direct-to-LDS loads.
 constexpr int BF16_RING = 4;
 
-// 16 bytes per lane from global memory straight into LDS: lane l writes
-// lds_wave_base + 16 l (wave-uniform base, through M0).  Issued as inline asm so
-// the compiler does not insert its own conservative vmcnt(0) before every LDS
-// read; ordering is by the explicit vmcnt waits and barriers of the caller.
-__device__ __forceinline__ void lds_dma16(const void* gsrc, void* lds_wave_base) {
-    // the base is wave-uniform; readfirstlane tells the compiler so (SGPR operand)
-    const uint32_t m0 = __builtin_amdgcn_readfirstlane(uint32_t(reinterpret_cast<uintptr_t>(lds_wave_base)));
+// a wave-uniform pointer moved to scalar registers (an SGPR asm operand)
+__device__ __forceinline__ const void* uniform_ptr(const void* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<const void*>((uint64_t(hi) << 32) | lo);
+}
+
+// LDS byte address of a shared-memory pointer, as a wave-uniform value
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return __builtin_amdgcn_readfirstlane(uint32_t(reinterpret_cast<uintptr_t>(p)));
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane l reads
+// gbase + voff (scalar 64-bit base, 32-bit lane offset) and writes
+// lds_base + 16 l (wave-uniform, through M0).  Issued as inline asm so the
+// compiler does not insert its own conservative vmcnt(0) before every LDS read;
+// ordering is by the explicit vmcnt waits and barriers of the caller.
+__device__ __forceinline__ void lds_dma16(const void* gbase, uint32_t voff, uint32_t lds_base) {
     asm volatile(
         "s_mov_b32 m0, %0\n\t"
         "s_nop 0\n\t"
-        "global_load_lds_dwordx4 %1, off"
+        "global_load_lds_dwordx4 %1, %2"
         :
-        : "s"(m0), "v"(gsrc)
+        : "s"(lds_base), "v"(voff), "s"(gbase)
         : "memory", "m0");
 }
 
-// LDS bytes of assign_bf16_kernel: ring of 32 x dpb bf16 blocks + their cnB
+// wait until at most N vector-memory operations of this wave are outstanding
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// 4 bytes per lane, as lds_dma16: lane l writes lds_base + 4 l
+__device__ __forceinline__ void lds_dma4(const void* gbase, uint32_t voff, uint32_t lds_base) {
+    asm volatile(
+        "s_mov_b32 m0, %0\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dword %1, %2"
+        :
+        : "s"(lds_base), "v"(voff), "s"(gbase)
+        : "memory", "m0");
+}
+
+// Re-score lists of the main kernel, one segment of `cap` entries per wave
+// (segment = 4 * workgroup + wave); compact_lists_kernel joins them.
+struct seg_lists {
+    int32_t* cand_rows;
+    int32_t* cand_count;
+    int32_t* cand_ids;   // MAXC per entry
+    int32_t* coll_rows;
+    double* coll_T;
+    int32_t* counts;     // [2 * segment]: candidate entries, [2 * segment + 1]: collect entries
+    int64_t cap;
+};
+
+// number of set bits of mask in lanes below this one
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
+}
+
+// LDS bytes of assign_bf16_kernel: ring of 32 x dpb bf16 blocks, and per ring
+// slot four 256-byte copies (one per wave) of the block's 32 cnB values
 __host__ inline size_t bf16_lds_bytes(int dpb) {
-    return size_t(BF16_RING) * (size_t(32) * dpb * 2 + 32 * 4);
+    return size_t(BF16_RING) * (size_t(32) * dpb * 2 + 1024);
 }
 
 // ---------------------------------------------------------------------------
@@ -231,6 +289,9 @@ __host__ inline size_t bf16_lds_bytes(int dpb) {
 // c ^ (r & SWZ), so the 32 lanes reading one chunk column hit distinct banks.
 // Lane l = 32*h + r: half h covers features [h*DPB/2, (h+1)*DPB/2), 8 per step.
 // Slots: lane keeps 4 (centroid block index mod 4); a row has 8.
+// The key work of block mb runs while the MFMAs of block mb+1 execute (two
+// accumulator sets), interleaved by scheduling groups.
+// Cb and cnb_g hold bf16_padded_k(k) rows, a multiple of four blocks.
 // COLLECT: rows come from L.coll_rows; every j with v_j <= T is recorded.
 // ---------------------------------------------------------------------------
 template <int DPB, int NB, bool COLLECT>
@@ -238,7 +299,7 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     const uint16_t* __restrict__ Xb, int64_t n, const uint16_t* __restrict__ Cb, int k,
     const float* __restrict__ cnb_g, const float* __restrict__ xnorm_up,
     const float* __restrict__ enorm_up, const bf16_params* __restrict__ params_p,
-    bf16_consts kc, int32_t* __restrict__ labels, refine_lists L) {
+    bf16_consts kc, int32_t* __restrict__ labels, refine_lists L, seg_lists S) {
     constexpr int HALF = DPB / 2;
     constexpr int STEPS = HALF / 8;
     constexpr int ROWS_PER_WAVE = 32 * NB;
@@ -247,70 +308,79 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     constexpr int SWZ = (CPR < 16 ? CPR : 16) - 1;  // chunk swizzle mask
     constexpr int ROWB = DPB * 2;                 // bytes per centroid row
     constexpr int BLKB = 32 * ROWB;               // bytes per block
-    constexpr int NCOPY = BLKB / 1024 + 1;        // 1 KB copies per block (+1: cnB)
-    constexpr int CPW = (NCOPY + 3) / 4;          // copies per wave per block (uniform)
+    constexpr int NDATA = BLKB / 1024;            // 1 KB copies per block
+    constexpr int DPW = NDATA >= 4 ? NDATA / 4 : 1;  // data copies per wave per block
+    constexpr int CPW = DPW + 1;                  // + the wave's quarter of the block's cnB
     constexpr int NSL = 4;
+    // VALU ops of one block's key work (5 per pair of distances) per MFMA
+    constexpr int VPM = (NB * 40 + STEPS * NB - 1) / (STEPS * NB);
+    static_assert(NDATA >= 2 && (NDATA < 4 || NDATA % 4 == 0), "block copy layout");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char* ring = smem_raw;                                 // [RING][BLKB]
-    unsigned char* ring_cn = smem_raw + size_t(BF16_RING) * BLKB;   // [RING][32] floats
+    unsigned char* ring_cn = smem_raw + size_t(BF16_RING) * BLKB;   // [RING][4 waves][64] floats
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int r32 = lane & 31;
     const int h = lane >> 5;
-    const bf16_params P = *params_p;
-    const int mblocks = (k + 31) / 32;
+    const int mblocks = bf16_padded_k(k) / 32;  // a multiple of 4
     const uint32_t kmask = 0x7FFFFFFFu & ~((1u << kc.ib) - 1u);
     const int64_t nrows = COLLECT ? int64_t(L.counters[2]) : n;
     const int64_t ntiles = (nrows + ROWS_PER_WG - 1) / ROWS_PER_WG;
+    // the mask lives in a VGPR so (bits & mask) | tag is one v_and_or_b32
+    // (a VALU op may read only one scalar operand on this target)
+    uint32_t kmv;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(kmv) : "s"(kmask));
 
-    // Direct-to-LDS copies of block mb into ring slot `slot`.  Copy q (q < NCOPY-1)
-    // fills LDS bytes [q KB, (q+1) KB) of the block; lane l writes 16-byte chunk
-    // p = 64 q + l, i.e. physical row p / CPR, physical chunk p % CPR, which holds
-    // logical chunk (p % CPR) ^ (row & SWZ).  Copy NCOPY-1 moves the 32 cnB values.
-    // Every wave issues exactly CPW copies per block (repeating copy 0 when it has
-    // fewer; the repeat writes the same bytes), so vmcnt waits are uniform.
-    auto issue_block = [&](int mb, int slot) {
+    // This wave's direct-to-LDS copies of a block.  Data copy c fills LDS bytes
+    // [q KB, (q+1) KB) of the block, q = wave + 4 c (blocks of 2 KB: both pairs
+    // of waves write the same two copies, the same bytes); lane l moves 16-byte
+    // chunk p = 64 q + l, i.e. physical row p / CPR, physical chunk p % CPR, which
+    // holds logical chunk (p % CPR) ^ (row & SWZ).  Then every wave moves the
+    // block's 32 cnB values into its own 256 bytes (lane l: value l & 31; all
+    // lanes active, so the block loop has no divergent branch).  Every wave issues
+    // CPW copies per block, so the vmcnt waits are uniform.
+    uint32_t voff[DPW], moff[DPW];
 #pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            int q = wave + 4 * c;
-            if (q >= NCOPY) q = 0;
-            if (q < NCOPY - 1) {
-                const int p = q * 64 + lane;
-                const int prow = p / CPR, pc = p % CPR;
-                const int lc = pc ^ (prow & SWZ);
-                const int cj = min(mb * 32 + prow, k - 1);
-                const uint16_t* src = Cb + int64_t(cj) * DPB + lc * 8;
-                lds_dma16(src, ring + slot * BLKB + q * 1024);
-            } else if (lane < 8) {
-                lds_dma16(cnb_g + mb * 32 + lane * 4, ring_cn + slot * 128);
-            }
-        }
+    for (int c = 0; c < DPW; ++c) {
+        const int q = NDATA >= 4 ? wave + 4 * c : wave % NDATA;
+        const int p = q * 64 + lane;
+        const int prow = p / CPR, pc = p % CPR;
+        const int lc = pc ^ (prow & SWZ);
+        voff[c] = uint32_t((prow * DPB + lc * 8) * 2);
+        moff[c] = lds_addr(ring + q * 1024);
+    }
+    const uint32_t voff_cn = uint32_t((lane & 31) * 4);
+    const uint32_t moff_cn = lds_addr(ring_cn + 256 * wave);
+    auto issue_block = [&](int mb, int slot) {
+        const void* gb = uniform_ptr(reinterpret_cast<const unsigned char*>(Cb) + size_t(mb) * BLKB);
+#pragma unroll
+        for (int c = 0; c < DPW; ++c) lds_dma16(gb, voff[c], moff[c] + slot * BLKB);
+        const void* gc = uniform_ptr(cnb_g + mb * 32);
+        lds_dma4(gc, voff_cn, moff_cn + slot * 1024);
     };
-    // wait until at most `pending` blocks' copies of this wave are in flight
-    auto wait_blocks = [&](int pending) {
-        if (COLLECT || pending == 0) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        } else if (pending == 1) {
-            if constexpr (CPW == 1) asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
-            else if constexpr (CPW == 2) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
-            else if constexpr (CPW == 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
-        } else {
-            if constexpr (CPW == 1) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
-            else if constexpr (CPW == 2) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
-            else if constexpr (CPW == 3) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(10)" ::: "memory");
-        }
+    // wait until at most `pending` blocks' copies of this wave are in flight (the
+    // collect kernel's atomics also count, so it waits for everything)
+    // Copies run three blocks ahead, also past the last block (those re-copy the
+    // last block into a slot nobody reads again in this tile), so at every step
+    // the copies of exactly two later blocks may stay in flight.  The collect
+    // kernel's atomics also count, so it waits for everything.
+    auto wait_block = [&]() {
+        if (COLLECT) wait_vm<0>();
+        else wait_vm<2 * CPW>();
     };
-    static_assert(CPW <= 5, "copy count per wave");
 
+    // this wave's list segment (main kernel): at most cap rows pass through a wave
+    const int seg = int(blockIdx.x) * 4 + wave;
+    const int64_t seg_base = int64_t(seg) * S.cap;
+    int n_cand = 0, n_coll = 0;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t w0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
         bf16x8 xr[NB][STEPS];
         int64_t rows[NB];
         double T[NB];
+        float xn[NB], xe[NB];  // ||x||, ||x - bf16(x)|| of the row (for the threshold)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t e = w0 + nb * 32 + r32;
@@ -319,6 +389,9 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             if (COLLECT) {
                 row = ok ? int64_t(L.coll_rows[e]) : 0;
                 T[nb] = ok ? L.coll_T[e] : -1.0;
+            } else {
+                xn[nb] = ok ? xnorm_up[row] : 0.f;
+                xe[nb] = ok ? enorm_up[row] : 0.f;
             }
             rows[nb] = ok ? row : -1;
             const uint16_t* src = Xb + (ok ? row : 0) * DPB + h * HALF;
@@ -338,10 +411,11 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 m2[nb][q] = 0xFFFFFFFFu;
             }
 
-        auto block = [&](int slot, int mb, int sl) {
-            const float* cnb = reinterpret_cast<const float*>(ring_cn + slot * 128);
+        // distances of the block in ring slot `slot`: all A fragments and start
+        // values first, then the MFMA chain
+        auto mfma_block = [&](int slot, floatx16(&acc)[NB]) {
+            const float* cnb = reinterpret_cast<const float*>(ring_cn + slot * 1024 + wave * 256);
             const unsigned char* arow = ring + slot * BLKB + r32 * ROWB;
-            // all A fragments and start values of the block first, then the MFMA chain
             bf16x8 a[STEPS];
 #pragma unroll
             for (int s = 0; s < STEPS; ++s) {
@@ -355,7 +429,6 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
 #pragma unroll
                 for (int t = 0; t < 4; ++t) init[4 * q + t] = c4[t];
             }
-            floatx16 acc[NB];
 #pragma unroll
             for (int s = 0; s < STEPS; ++s) {
 #pragma unroll
@@ -363,6 +436,9 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                     acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], xr[nb][s],
                                                                       s == 0 ? init : acc[nb], 0, 0, 0);
             }
+        };
+        // keys of block mb into slot sl (running top-2), or the collect test
+        auto keys_block = [&](const floatx16(&acc)[NB], int mb, int sl) {
             if (COLLECT) {
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
@@ -379,19 +455,18 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                         }
                     }
             } else {
+                // tag = (block, register) < 2^IB, since IB = 4 + ceil(log2(blocks))
                 const uint32_t tag = uint32_t(mb) << 4;
-                const uint32_t lowmask = ~kmask;
-                // the mask lives in a VGPR so (bits & mask) | tag is one v_and_or_b32
-                // (a VALU op may read only one scalar operand on this target)
-                uint32_t kmv;
-                asm volatile("v_mov_b32 %0, %1" : "=v"(kmv) : "s"(kmask));
+                // register pairs outermost: each (block, register) tag is live briefly
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
+                for (int i = 0; i < 16; i += 2) {
+                    // tags formed on the scalar unit, so a key is one v_and_or_b32
+                    uint32_t ta, tb;
+                    asm("s_or_b32 %0, %1, %2" : "=s"(ta) : "s"(tag), "i"(i));
+                    asm("s_or_b32 %0, %1, %2" : "=s"(tb) : "s"(tag), "i"(i + 1));
 #pragma unroll
-                    for (int i = 0; i < 16; i += 2) {
-                        // bitfield insert: value bits where kmask is set, (block, register) below
-                        const uint32_t ta = (tag | uint32_t(i)) & lowmask;
-                        const uint32_t tb = (tag | uint32_t(i + 1)) & lowmask;
+                    for (int nb = 0; nb < NB; ++nb) {
+                        // value bits where kmask is set, (block, register) below
                         const uint32_t ka = (__float_as_uint(acc[nb][i]) & kmv) | ta;
                         const uint32_t kb = (__float_as_uint(acc[nb][i + 1]) & kmv) | tb;
                         // second smallest of {m1, ka, kb} (m1 <= m2 always)
@@ -399,17 +474,36 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                         m2[nb][sl] = min(m2[nb][sl], med);
                         m1[nb][sl] = min(m1[nb][sl], min(ka, kb));
                     }
+                }
+            }
+        };
+        // LDS reads of the block first; the previous block's key work then fills
+        // the issue slots between the MFMAs
+        auto interleave = [&]() {
+            __builtin_amdgcn_sched_group_barrier(0x100, STEPS + 4, 0);  // LDS reads
+            __builtin_amdgcn_sched_group_barrier(0x002, 2 * VPM, 0);     // VALU
+#pragma unroll
+            for (int i = 0; i < STEPS * NB; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);       // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);     // VALU
             }
         };
 
         // step mb: wait for block mb's copies (blocks mb+1, mb+2 may stay in flight),
         // barrier (all waves' copies visible; everyone is done with block mb-1),
-        // refill the slot of block mb-1 with block mb+3, then run block mb
-        auto step = [&](int mb, int sl) {
-            wait_blocks(min(2, mblocks - 1 - mb));
+        // refill the slot of block mb-1 with block mb+3, then the MFMAs of block mb
+        // (ring slot and top-2 slot mb & 3) beside the keys of block mb-1
+        floatx16 acc0[NB], acc1[NB];
+        auto step = [&](int mb, int slot, floatx16(&cur)[NB], floatx16(&prev)[NB]) {
+            // nothing is scheduled across the wait, the barrier and the copies
+            __builtin_amdgcn_sched_barrier(0);
+            wait_block();
             __syncthreads();
-            if (mb + 3 < mblocks) issue_block(mb + 3, (mb + 3) & (BF16_RING - 1));
-            block(mb & (BF16_RING - 1), mb, sl);
+            issue_block(min(mb + 3, mblocks - 1), (slot + 3) & (BF16_RING - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_block(slot, cur);
+            keys_block(prev, mb - 1, (slot + 3) & (BF16_RING - 1));
+            if (!COLLECT) interleave();
         };
         // row loads complete here (a waitcnt the compiler sees, so it does not re-wait
         // for them inside the block loop, where vmcnt also counts the ring copies);
@@ -417,17 +511,27 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
         issue_block(0, 0);
-        if (1 < mblocks) issue_block(1, 1);
-        if (2 < mblocks) issue_block(2, 2);
+        issue_block(1, 1);
+        issue_block(2, 2);
+        // the first step's "previous block" (-1): all-ones accumulators give keys
+        // 0xFFFFFFFx, above every real key (each slot receives real keys later,
+        // which displace them) and NaN-valued, so never below a threshold
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc1[nb][i] = __uint_as_float(0xFFFFFFFFu);
         for (int mb = 0; mb < mblocks; mb += 4) {
-            step(mb, 0);
-            if (mb + 1 < mblocks) step(mb + 1, 1);
-            if (mb + 2 < mblocks) step(mb + 2, 2);
-            if (mb + 3 < mblocks) step(mb + 3, 3);
+            step(mb, 0, acc0, acc1);
+            step(mb + 1, 1, acc1, acc0);
+            step(mb + 2, 2, acc0, acc1);
+            step(mb + 3, 3, acc1, acc0);
         }
+        keys_block(acc1, mblocks - 1, 3);
         if (COLLECT) continue;
 
-        // settle rows: lane r32 (h = 0) merges its 4 slots with lane r32 + 32's 4
+        // settle rows: lane r32 (h = 0) merges its 4 slots with lane r32 + 32's 4.
+        // Rows that need a re-score go to this wave's own segment of the lists,
+        // at positions counted by ballots (no global atomics).
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             uint32_t k1[2 * NSL], k2[2 * NSL];
@@ -439,50 +543,97 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 k2[NSL + q] = __shfl_xor(m2[nb][q], 32);
             }
             const int64_t row = rows[nb];
-            if (h != 0 || row < 0) continue;
             // decode: key low bits = (block << 4) | register; slot half from position
             auto idx_of = [&](uint32_t key, int hh) {
                 const uint32_t low = key & ((1u << kc.ib) - 1u);
                 const int mbk = int(low >> 4), i = int(low & 15u);
                 return mbk * 32 + 8 * (i >> 2) + 4 * hh + (i & 3);
             };
-            uint32_t best = 0xFFFFFFFFu;
-            int bs = 0;
-#pragma unroll
-            for (int s = 0; s < 2 * NSL; ++s)
-                if (k1[s] < best) {
-                    best = k1[s];
-                    bs = s;
-                }
-            const double X = double(xnorm_up[row]), ex = double(enorm_up[row]);
-            const double Tr = bf16_threshold(kc, P, X, ex, double(__uint_as_float(best & kmask)));
-            bool cert = true;
+            int outcome = 0;  // 1: candidate entry, 2: collect entry
             int cnt = 0;
-            int cand[MAXC];
+            int cj[2 * NSL];  // slot s's winner when inside the band, else -1 (static indices)
+            double Tr = 0.0;
+            if (h == 0 && row >= 0) {
+                uint32_t best = 0xFFFFFFFFu;
+                int bs = 0;
 #pragma unroll
-            for (int s = 0; s < 2 * NSL; ++s) {
-                if (k2[s] != 0xFFFFFFFFu && double(__uint_as_float(k2[s] & kmask)) <= Tr) cert = false;
-                if (k1[s] != 0xFFFFFFFFu && double(__uint_as_float(k1[s] & kmask)) <= Tr) {
-                    const int j = idx_of(k1[s], s >= NSL ? 1 : 0);
-                    if (j >= k) cert = false;  // a padded centroid inside the band: collect pass decides
-                    if (cnt < MAXC) cand[cnt] = j;
-                    cnt++;
+                for (int s = 0; s < 2 * NSL; ++s)
+                    if (k1[s] < best) {
+                        best = k1[s];
+                        bs = s;
+                    }
+                Tr = bf16_threshold(kc, *params_p, double(xn[nb]), double(xe[nb]),
+                                    double(__uint_as_float(best & kmask)));
+                bool cert = true;
+#pragma unroll
+                for (int s = 0; s < 2 * NSL; ++s) {
+                    if (k2[s] != 0xFFFFFFFFu && double(__uint_as_float(k2[s] & kmask)) <= Tr) cert = false;
+                    cj[s] = -1;
+                    if (k1[s] != 0xFFFFFFFFu && double(__uint_as_float(k1[s] & kmask)) <= Tr) {
+                        const int j = idx_of(k1[s], s >= NSL ? 1 : 0);
+                        if (j >= k) cert = false;  // a padded centroid inside the band: collect pass decides
+                        cj[s] = j;
+                        cnt++;
+                    }
                 }
+                if (cert && cnt == 1)
+                    labels[row] = idx_of(best, bs >= NSL ? 1 : 0);
+                else
+                    outcome = cert ? 1 : 2;
             }
-            if (cert && cnt == 1) {
-                labels[row] = idx_of(best, bs >= NSL ? 1 : 0);
-            } else if (cert && cnt <= MAXC) {
-                const int e = atomicAdd(&L.counters[0], 1);
-                L.cand_rows[e] = int32_t(row);
-                L.cand_count[e] = cnt;
-                for (int c = 0; c < cnt; ++c) L.cand_ids[int64_t(e) * MAXC + c] = cand[c];
-            } else {
-                const int e = atomicAdd(&L.counters[2], 1);
-                L.coll_rows[e] = int32_t(row);
-                L.coll_T[e] = Tr;
-                L.coll_count[e] = 0;
+            const uint64_t mc = __ballot(outcome == 1);
+            const uint64_t ml = __ballot(outcome == 2);
+            if (outcome == 1) {
+                const int64_t e = seg_base + n_cand + lanes_below(mc);
+                S.cand_rows[e] = int32_t(row);
+                S.cand_count[e] = cnt;
+                int c = 0;
+#pragma unroll
+                for (int s = 0; s < 2 * NSL; ++s)
+                    if (cj[s] >= 0) S.cand_ids[e * MAXC + c++] = cj[s];
+            } else if (outcome == 2) {
+                const int64_t e = seg_base + n_coll + lanes_below(ml);
+                S.coll_rows[e] = int32_t(row);
+                S.coll_T[e] = Tr;
             }
+            n_cand += int(__popcll(mc));
+            n_coll += int(__popcll(ml));
         }
+    }
+    if (!COLLECT && lane == 0) {
+        S.counts[2 * seg] = n_cand;
+        S.counts[2 * seg + 1] = n_coll;
+    }
+}
+
+// Concatenates the per-wave segments of the main kernel into the dense candidate
+// and collect lists (one wave per segment; the order of entries is not significant:
+// each entry re-scores its own row).
+__global__ __launch_bounds__(256) void compact_lists_kernel(seg_lists S, int nseg, refine_lists L) {
+    const int lane = threadIdx.x & 63;
+    const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (seg >= nseg) return;
+    const int nc = S.counts[2 * seg], nl = S.counts[2 * seg + 1];
+    int bc = 0, bl = 0;
+    if (lane == 0) {
+        if (nc) bc = atomicAdd(&L.counters[0], nc);
+        if (nl) bl = atomicAdd(&L.counters[2], nl);
+    }
+    bc = __shfl(bc, 0);
+    bl = __shfl(bl, 0);
+    const int64_t base = int64_t(seg) * S.cap;
+    for (int i = lane; i < nc; i += 64) {
+        const int64_t src = base + i, dst = int64_t(bc) + i;
+        const int cnt = S.cand_count[src];
+        L.cand_rows[dst] = S.cand_rows[src];
+        L.cand_count[dst] = cnt;
+        for (int c = 0; c < cnt; ++c) L.cand_ids[dst * MAXC + c] = S.cand_ids[src * MAXC + c];
+    }
+    for (int i = lane; i < nl; i += 64) {
+        const int64_t src = base + i, dst = int64_t(bl) + i;
+        L.coll_rows[dst] = S.coll_rows[src];
+        L.coll_T[dst] = S.coll_T[src];
+        L.coll_count[dst] = 0;
     }
 }
 

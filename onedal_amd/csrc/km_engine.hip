@@ -96,6 +96,7 @@ struct km_ctx {
 
     // workspace
     km::dbuf x_bf16, enorm, xmax, c_bf16, cnb, params, coll_rows, coll_T, coll_count, coll_ids;
+    km::dbuf seg_cand_rows, seg_cand_cnt, seg_cand_ids, seg_coll_rows, seg_coll_T, seg_counts;
     km::dbuf x_dev, x_pad, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
@@ -187,7 +188,7 @@ static band_consts make_band(int d) {
 static bf16_consts make_bf16_consts(int d, int k) {
     bf16_consts c;
     int mbits = 0;
-    while ((1 << mbits) < (k + 31) / 32) mbits++;
+    while ((1 << mbits) < bf16_padded_k(k) / 32) mbits++;
     c.ib = 4 + mbits;
     c.u = fp_traits<float>::u;
     c.g_acc = gamma_m(double(d + 1), fp_traits<float>::u_acc);
@@ -203,14 +204,14 @@ static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, int path) {
     if constexpr (std::is_same<F, float>::value) {
         if (path == 3) {
             fill<unsigned long long>(ctx, w.cmax_bits, 3, 0ull);
-            const int grid = grid_for(ctx, int64_t(w.k) * 64, 256, 4);
+            const int kp = bf16_padded_k(w.k);
+            const int grid = grid_for(ctx, int64_t(kp) * 64, 256, 4);
             hipLaunchKernelGGL(centroid_prep_bf16_kernel, dim3(grid), dim3(256), 0, ctx->stream, C,
-                               w.k, w.d, w.dpad, w.Cb, w.cn, w.cmax_bits);
+                               w.k, kp, w.d, w.dpad, w.Cb, w.cn, w.cmax_bits);
             KM_LAUNCH_CHECK();
             hipLaunchKernelGGL(bf16_params_kernel, dim3(1), dim3(64), 0, ctx->stream, w.cmax_bits,
                                w.xmax_bits, make_bf16_consts(w.d, w.k), w.params);
             KM_LAUNCH_CHECK();
-            const int kp = (w.k + 31) / 32 * 32;
             hipLaunchKernelGGL(cnb_kernel, dim3(grid_for(ctx, kp, 256, 1)), dim3(256), 0, ctx->stream,
                                w.cn, w.k, kp, w.params, w.cnb);
             KM_LAUNCH_CHECK();
@@ -257,14 +258,28 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
     if (!collect) {
         const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
         const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
+        // one list segment per wave, as long as the rows that wave can see
+        const int nseg = grid * 4;
+        seg_lists S;
+        S.cap = (tiles + grid - 1) / grid * (32 * NB);
+        const int64_t ents = S.cap * nseg;
+        S.cand_rows = ctx->seg_cand_rows.get<int32_t>(ents);
+        S.cand_count = ctx->seg_cand_cnt.get<int32_t>(ents);
+        S.cand_ids = ctx->seg_cand_ids.get<int32_t>(ents * MAXC);
+        S.coll_rows = ctx->seg_coll_rows.get<int32_t>(ents);
+        S.coll_T = ctx->seg_coll_T.get<double>(ents);
+        S.counts = ctx->seg_counts.get<int32_t>(2 * int64_t(nseg));
         hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, false>), dim3(grid), dim3(256), lds,
                            ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params, kc,
-                           w.labels, w.L);
+                           w.labels, w.L, S);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(compact_lists_kernel, dim3((nseg + 3) / 4), dim3(256), 0, ctx->stream, S,
+                           nseg, w.L);
     } else {
         // row count is on the device; a fixed grid strides over it
         hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, true>), dim3(ctx->cu_count * 2), dim3(256),
                            lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params,
-                           kc, w.labels, w.L);
+                           kc, w.labels, w.L, seg_lists{});
     }
     KM_LAUNCH_CHECK();
 }
@@ -531,8 +546,8 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
             w.Xb = ctx->x_bf16.get<uint16_t>(n * int64_t(w.dpad));
             w.enorm = ctx->enorm.get<float>(n);
             w.xmax_bits = ctx->xmax.get<unsigned long long>(2);
-            w.Cb = ctx->c_bf16.get<uint16_t>(int64_t(k) * w.dpad);
-            w.cnb = ctx->cnb.get<float>((k + 31) / 32 * 32);
+            w.Cb = ctx->c_bf16.get<uint16_t>(int64_t(bf16_padded_k(k)) * w.dpad);
+            w.cnb = ctx->cnb.get<float>(bf16_padded_k(k));
             w.params = ctx->params.get<bf16_params>(1);
             w.L.coll_rows = ctx->coll_rows.get<int32_t>(n);
             w.L.coll_T = ctx->coll_T.get<double>(n);
@@ -705,6 +720,15 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
     KM_HIP(hipStreamSynchronize(ctx->stream));
     collect_refine_stats(ctx, pin + 2);
     if (want_obj && obj_out) std::memcpy(obj_out, pin + 6, sizeof(double));
+    if (ctx->timing) {
+        double tot = 0.0;
+        for (size_t i = 0; i + 1 < ctx->ev_used; i += 2) {
+            float a = 0.f;
+            KM_HIP(hipEventElapsedTime(&a, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
+            tot += a;
+        }
+        ctx->stats.assign_kernel_ms = tot;
+    }
 }
 
 // --- precondition checks (reference order: train_ops.hpp:40-61, common.cpp:82-115) ---
@@ -812,7 +836,9 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->offsets, &ctx->chunk_base, &ctx->empty_list, &ctx->meta, &ctx->perm,
                         &ctx->keys_out, &ctx->sort_tmp, &ctx->part, &ctx->d64, &ctx->obj_chunks,
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
-                        &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev};
+                        &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
+                        &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
+                        &ctx->seg_counts};
     for (auto* b : bufs) b->release();
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
