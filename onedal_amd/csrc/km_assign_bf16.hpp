@@ -325,6 +325,7 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     const int r32 = lane & 31;
     const int h = lane >> 5;
     const int mblocks = bf16_padded_k(k) / 32;  // a multiple of 4
+    const bf16_params P = *params_p;
     const uint32_t kmask = 0x7FFFFFFFu & ~((1u << kc.ib) - 1u);
     const int64_t nrows = COLLECT ? int64_t(L.counters[2]) : n;
     const int64_t ntiles = (nrows + ROWS_PER_WG - 1) / ROWS_PER_WG;
@@ -375,12 +376,13 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     const int seg = int(blockIdx.x) * 4 + wave;
     const int64_t seg_base = int64_t(seg) * S.cap;
     int n_cand = 0, n_coll = 0;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // rows of a tile: B operand fragments, ids, and the per-row threshold inputs
+    bf16x8 xr[NB][STEPS];
+    int64_t rows[NB];
+    double T[NB];
+    float xn[NB], xe[NB];  // ||x||, ||x - bf16(x)|| of the row (for the threshold)
+    auto load_tile = [&](int64_t tile) {
         const int64_t w0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
-        bf16x8 xr[NB][STEPS];
-        int64_t rows[NB];
-        double T[NB];
-        float xn[NB], xe[NB];  // ||x||, ||x - bf16(x)|| of the row (for the threshold)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t e = w0 + nb * 32 + r32;
@@ -402,6 +404,10 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 xr[nb][s] = __builtin_bit_cast(bf16x8, v);
             }
         }
+    };
+    if (int64_t(blockIdx.x) < ntiles) load_tile(blockIdx.x);
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t w0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
         uint32_t m1[NB][NSL], m2[NB][NSL];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
@@ -527,57 +533,74 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             step(mb + 3, 3, acc1, acc0);
         }
         keys_block(acc1, mblocks - 1, 3);
-        if (COLLECT) continue;
-
-        // settle rows: lane r32 (h = 0) merges its 4 slots with lane r32 + 32's 4.
-        // Rows that need a re-score go to this wave's own segment of the lists,
-        // at positions counted by ballots (no global atomics).
+        // the next tile's rows load while this tile settles (the fragments are dead)
+        int64_t rows_s[NB];
+        float xn_s[NB], xe_s[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            uint32_t k1[2 * NSL], k2[2 * NSL];
+            rows_s[nb] = rows[nb];
+            xn_s[nb] = xn[nb];
+            xe_s[nb] = xe[nb];
+        }
+        if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
+        if (COLLECT) continue;
+
+        // settle: with two row blocks, lane (r32, h) settles row r32 of block h (both
+        // halves busy), with its own 4 slots of that block and the partner lane's 4
+        // (one shuffle); with one block the lanes h = 0 settle.  Rows that need a
+        // re-score go to this wave's own segment of the lists, at positions counted
+        // by ballots (no global atomics).
+        {
+            const int nbs = NB == 2 ? h : 0;
+            uint32_t k1[2 * NSL], k2[2 * NSL];  // own half's slots, then the partner's
 #pragma unroll
             for (int q = 0; q < NSL; ++q) {
-                k1[q] = m1[nb][q];
-                k2[q] = m2[nb][q];
-                k1[NSL + q] = __shfl_xor(m1[nb][q], 32);
-                k2[NSL + q] = __shfl_xor(m2[nb][q], 32);
+                k1[q] = nbs ? m1[NB - 1][q] : m1[0][q];
+                k2[q] = nbs ? m2[NB - 1][q] : m2[0][q];
+                const uint32_t s1 = (NB == 2 && !h) ? m1[NB - 1][q] : m1[0][q];
+                const uint32_t s2 = (NB == 2 && !h) ? m2[NB - 1][q] : m2[0][q];
+                k1[NSL + q] = __shfl_xor(s1, 32);
+                k2[NSL + q] = __shfl_xor(s2, 32);
             }
-            const int64_t row = rows[nb];
-            // decode: key low bits = (block << 4) | register; slot half from position
+            const int64_t row = nbs ? rows_s[NB - 1] : rows_s[0];
+            const float xnr = nbs ? xn_s[NB - 1] : xn_s[0];
+            const float xer = nbs ? xe_s[NB - 1] : xe_s[0];
+            const bool active = (NB == 2 || h == 0) && row >= 0;
+            // decode: key low bits = (block << 4) | register; hh = half that holds it
             auto idx_of = [&](uint32_t key, int hh) {
                 const uint32_t low = key & ((1u << kc.ib) - 1u);
                 const int mbk = int(low >> 4), i = int(low & 15u);
                 return mbk * 32 + 8 * (i >> 2) + 4 * hh + (i & 3);
             };
-            int outcome = 0;  // 1: candidate entry, 2: collect entry
+            uint32_t bo = k1[0], bp = k1[NSL];
+#pragma unroll
+            for (int q = 1; q < NSL; ++q) {
+                bo = min(bo, k1[q]);
+                bp = min(bp, k1[NSL + q]);
+            }
+            const uint32_t best = min(bo, bp);
+            const int hb = bo <= bp ? h : 1 - h;
+            const double Tr = bf16_threshold(kc, P, double(xnr), double(xer),
+                                             double(__uint_as_float(best & kmask)));
+            // value(key) <= Tr  <=>  key <= TK: the threshold rounded down to a float,
+            // with the index bits set (positive floats order like their bits; the
+            // all-ones sentinels stay above)
+            float trf = float(Tr);
+            if (double(trf) > Tr) trf = __uint_as_float(__float_as_uint(trf) - 1u);
+            const uint32_t TK = (__float_as_uint(trf) & kmask) | ((1u << kc.ib) - 1u);
+            // padding centroids have keys >= 0x7F000000: a threshold that high cannot
+            // certify (the collect pass decides)
+            bool cert = TK < 0x7F000000u;
             int cnt = 0;
-            int cj[2 * NSL];  // slot s's winner when inside the band, else -1 (static indices)
-            double Tr = 0.0;
-            if (h == 0 && row >= 0) {
-                uint32_t best = 0xFFFFFFFFu;
-                int bs = 0;
 #pragma unroll
-                for (int s = 0; s < 2 * NSL; ++s)
-                    if (k1[s] < best) {
-                        best = k1[s];
-                        bs = s;
-                    }
-                Tr = bf16_threshold(kc, *params_p, double(xn[nb]), double(xe[nb]),
-                                    double(__uint_as_float(best & kmask)));
-                bool cert = true;
-#pragma unroll
-                for (int s = 0; s < 2 * NSL; ++s) {
-                    if (k2[s] != 0xFFFFFFFFu && double(__uint_as_float(k2[s] & kmask)) <= Tr) cert = false;
-                    cj[s] = -1;
-                    if (k1[s] != 0xFFFFFFFFu && double(__uint_as_float(k1[s] & kmask)) <= Tr) {
-                        const int j = idx_of(k1[s], s >= NSL ? 1 : 0);
-                        if (j >= k) cert = false;  // a padded centroid inside the band: collect pass decides
-                        cj[s] = j;
-                        cnt++;
-                    }
-                }
+            for (int q = 0; q < 2 * NSL; ++q) {
+                cert = cert && k2[q] > TK;
+                cnt += k1[q] <= TK ? 1 : 0;
+            }
+            int outcome = 0;  // 1: candidate entry, 2: collect entry
+            if (active) {
                 if (cert && cnt == 1)
-                    labels[row] = idx_of(best, bs >= NSL ? 1 : 0);
+                    labels[row] = idx_of(best, hb);
                 else
                     outcome = cert ? 1 : 2;
             }
@@ -589,8 +612,8 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 S.cand_count[e] = cnt;
                 int c = 0;
 #pragma unroll
-                for (int s = 0; s < 2 * NSL; ++s)
-                    if (cj[s] >= 0) S.cand_ids[e * MAXC + c++] = cj[s];
+                for (int q = 0; q < 2 * NSL; ++q)
+                    if (k1[q] <= TK) S.cand_ids[e * MAXC + c++] = idx_of(k1[q], q < NSL ? h : 1 - h);
             } else if (outcome == 2) {
                 const int64_t e = seg_base + n_coll + lanes_below(ml);
                 S.coll_rows[e] = int32_t(row);
