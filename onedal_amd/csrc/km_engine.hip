@@ -289,8 +289,9 @@ template <typename F>
 static void refine_pairs(km_ctx* ctx, work<F>& w, const F* C, const int32_t* rows,
                          const int32_t* counts, const int32_t* ids, int stride,
                          const int32_t* count_p) {
-    const int rb = int(std::max<int64_t>(1, std::min<int64_t>(16, 16384 / (int64_t(w.d) * sizeof(F)))));
-    const size_t lds = size_t(4) * rb * w.d * sizeof(F);
+    const int64_t row_bytes = int64_t(refine_row_stride<F>(w.d)) * sizeof(F);
+    const int rb = int(std::max<int64_t>(1, std::min<int64_t>(16, 16384 / row_bytes)));
+    const size_t lds = size_t(4) * rb * row_bytes;
     auto kern = refine_pairs_kernel<F>;
     KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL(kern, dim3(ctx->cu_count * 4), dim3(256), lds, ctx->stream, w.X, w.d, C, rows,

@@ -15,29 +15,41 @@ namespace km {
 // candidates from the assignment kernels, or ids gathered by the BF16 collect
 // pass).  Entry e: row rows[e], counts[e] ids at ids[e * stride].  An entry with
 // no ids or more than `stride` (a collect overflow) goes to the full list.
-// Each wave takes RB entries: their rows are staged in LDS with coalesced loads,
-// the (row, id) pairs are spread over the 64 lanes, each lane forms the exact
-// distance of its pair (features in order; centroid read as 4-wide vectors when
-// rows are 16-byte aligned), and lane r keeps the minimum of entry r
+// Each wave takes rb entries: their rows are staged in LDS by all 64 lanes with
+// 16-byte loads (row stride d + 16 bytes, so lanes reading different rows hit
+// different banks), the (row, id) pairs are spread over the lanes, each lane
+// forms the exact distance of its pair (features in order), reading the centroid
+// one 32-feature chunk ahead of the sum, and lane r keeps the minimum of entry r
 // (ties to the lowest id).
+template <typename F>
+__host__ __device__ inline int refine_row_stride(int d) {
+    return d + int(16 / sizeof(F));
+}
+
 template <typename F>
 __global__ __launch_bounds__(256) void refine_pairs_kernel(
     const F* __restrict__ X, int d, const F* __restrict__ C, const int32_t* __restrict__ rows,
     const int32_t* __restrict__ counts, const int32_t* __restrict__ ids, int stride,
     const int32_t* __restrict__ count_p, int rb, int32_t* __restrict__ full_rows,
     int32_t* __restrict__ full_counter, int32_t* __restrict__ labels) {
+    typedef F F4 __attribute__((ext_vector_type(4)));
+    constexpr int CH = 32 / int(sizeof(F));  // 4-vectors per prefetched centroid chunk
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ F pd[4][64];
     __shared__ int pj[4][64];
     __shared__ int pstart[4][64];
+    __shared__ int prow[4][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    F* xs = reinterpret_cast<F*>(smem_raw) + size_t(wave) * rb * d;
+    const int ds = refine_row_stride<F>(d);
+    F* xs = reinterpret_cast<F*>(smem_raw) + size_t(wave) * rb * ds;
     const int total = *count_p;
     const int nbatch = (total + rb - 1) / rb;
     const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0;
+    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0 &&
+                     (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(F))) == 0;
+    const int nv = d / 4;
     for (int b = gw; b < nbatch; b += nw) {
         const int e0 = b * rb;
         const int nr = min(rb, total - e0);
@@ -50,10 +62,20 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(
                 my_cnt = 0;
             }
         }
-        // stage the rows (lanes over features)
-        for (int r = 0; r < nr; ++r) {
-            const int64_t row = __shfl(my_row, r);
-            for (int f = lane; f < d; f += 64) xs[r * d + f] = X[row * d + f];
+        prow[wave][lane] = my_row;
+        __builtin_amdgcn_wave_barrier();
+        // stage the rows: every lane moves 4-vectors (or elements) of any row
+        if (vec) {
+            for (int i = lane; i < nr * nv; i += 64) {
+                const int r = i / nv, q = i - r * nv;
+                const F4 v = *reinterpret_cast<const F4*>(X + int64_t(prow[wave][r]) * d + 4 * q);
+                *reinterpret_cast<F4*>(xs + r * ds + 4 * q) = v;
+            }
+        } else {
+            for (int i = lane; i < nr * d; i += 64) {
+                const int r = i / d, f = i - r * d;
+                xs[r * ds + f] = X[int64_t(prow[wave][r]) * d + f];
+            }
         }
         // exclusive scan of the pair counts over lanes 0..nr-1
         int incl = my_cnt;
@@ -78,19 +100,33 @@ __global__ __launch_bounds__(256) void refine_pairs_kernel(
                 for (int q = 1; q < nr; ++q)
                     if (pstart[wave][q] <= p) r = q;
                 j = ids[int64_t(e0 + r) * stride + (p - pstart[wave][r])];
-                const F* x = xs + r * d;
+                const F* x = xs + r * ds;
                 const F* c = C + int64_t(j) * d;
                 F acc = F(0);
                 if (vec) {
-                    for (int f = 0; f < d; f += 4) {
-                        typedef F F4 __attribute__((ext_vector_type(4)));
-                        const F4 c4 = *reinterpret_cast<const F4*>(c + f);
+                    F4 cur[CH], nxt[CH];
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const F diff = x[f + t] - c4[t];
-                            const F sq = diff * diff;
-                            acc = acc + sq;
+                    for (int q = 0; q < CH; ++q)
+                        cur[q] = q < nv ? *reinterpret_cast<const F4*>(c + 4 * q) : F4{};
+                    for (int v0 = 0; v0 < nv; v0 += CH) {
+#pragma unroll
+                        for (int q = 0; q < CH; ++q)
+                            nxt[q] = v0 + CH + q < nv ? *reinterpret_cast<const F4*>(c + 4 * (v0 + CH + q))
+                                                      : F4{};
+#pragma unroll
+                        for (int q = 0; q < CH; ++q) {
+                            if (v0 + q < nv) {
+                                const F4 x4 = *reinterpret_cast<const F4*>(x + 4 * (v0 + q));
+#pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const F diff = x4[t] - cur[q][t];
+                                    const F sq = diff * diff;
+                                    acc = acc + sq;
+                                }
+                            }
                         }
+#pragma unroll
+                        for (int q = 0; q < CH; ++q) cur[q] = nxt[q];
                     }
                 } else {
                     for (int f = 0; f < d; ++f) {
