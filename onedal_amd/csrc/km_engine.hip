@@ -377,6 +377,24 @@ template <typename F>
 static void launch_reduce(km_ctx* ctx, work<F>& w, int32_t* perm, int64_t* offsets,
                           int32_t* chunk_base, int32_t* meta, double* part) {
     const int64_t max_chunks = w.n / REDUCE_CHUNK + w.k + 1;
+    if (w.d > 32 && w.d <= 512 && w.d % 4 == 0 &&
+        reinterpret_cast<uintptr_t>(w.X) % (4 * sizeof(F)) == 0) {
+        // sub-wave per chunk with 4-wide loads: lpc lanes cover the row V times
+        const int V = w.d <= 128 ? 1 : (w.d <= 256 ? 2 : 4);
+        int lpc = 1;
+        while (lpc * 4 * V < w.d) lpc <<= 1;
+        const int subs = 64 / lpc;
+        const int grid = grid_for(ctx, (max_chunks + subs - 1) / subs * 64, 256, 8);
+#define KM_REDV(V_)                                                                                 \
+    hipLaunchKernelGGL((reduce_chunks_vec_kernel<F, V_>), dim3(grid), dim3(256), 0, ctx->stream,    \
+                       w.X, w.d, w.k, perm, offsets, chunk_base, meta, part, lpc)
+        if (V == 1) KM_REDV(1);
+        else if (V == 2) KM_REDV(2);
+        else KM_REDV(4);
+#undef KM_REDV
+        KM_LAUNCH_CHECK();
+        return;
+    }
     const int grid = grid_for(ctx, max_chunks * 64, 256, 8);
     const int cpl = w.d <= 64 ? 1 : (w.d + 63) / 64;
 #define KM_RED(C_)                                                                                  \

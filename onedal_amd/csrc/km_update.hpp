@@ -269,6 +269,83 @@ __global__ __launch_bounds__(256) void reduce_chunks_kernel(
     }
 }
 
+// Same chunk sums for 32 < d <= 512 with d % 4 == 0 and 16-byte aligned rows:
+// a sub-wave of lpc lanes (a power of two, lpc * 4 * V >= d) per chunk, lane l
+// owning the 4-vectors l, l + lpc, ..., V of them, so a member's row is read with
+// 16-byte loads (eight members' loads in flight).  Each feature is still summed
+// over the chunk's members in order, so the partials equal reduce_chunks_kernel's.
+template <typename F, int V>
+__global__ __launch_bounds__(256) void reduce_chunks_vec_kernel(
+    const F* __restrict__ X, int d, int k, const int32_t* __restrict__ perm,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ chunk_base,
+    const int32_t* __restrict__ meta, double* __restrict__ part, int lpc) {
+    typedef F F4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / lpc, sl = lane - sub * lpc;
+    const int subs = 64 / lpc;
+    const int64_t gsub = int64_t((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * subs + sub;
+    const int64_t nsub = int64_t((gridDim.x * blockDim.x) >> 6) * subs;
+    const int total = meta[1];
+    const int d4 = d / 4;
+    for (int64_t q = gsub; q < total; q += nsub) {
+        // cluster owning chunk q: last j with chunk_base[j] <= q
+        int lo = 0, hi = k;  // chunk_base[k] = total > q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_base[mid] <= q) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const int64_t m_begin = offsets[j] + int64_t(q - chunk_base[j]) * REDUCE_CHUNK;
+        const int64_t m_end = min(m_begin + REDUCE_CHUNK, offsets[j + 1]);
+        double acc[V][4];
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[v][e] = 0.0;
+        int64_t m = m_begin;
+        for (; m + 8 <= m_end; m += 8) {
+            int64_t r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r[u] = perm[m + u];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int c4 = sl + lpc * v;
+                if (c4 < d4) {
+                    F4 xv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) xv[u] = *reinterpret_cast<const F4*>(X + r[u] * d + 4 * c4);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) acc[v][e] = acc[v][e] + double(xv[u][e]);
+                }
+            }
+        }
+        for (; m < m_end; ++m) {
+            const int64_t r0 = perm[m];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int c4 = sl + lpc * v;
+                if (c4 < d4) {
+                    const F4 x4 = *reinterpret_cast<const F4*>(X + r0 * d + 4 * c4);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[v][e] = acc[v][e] + double(x4[e]);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int c4 = sl + lpc * v;
+            if (c4 < d4) {
+                double* o = part + int64_t(q) * d + 4 * c4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) o[e] = acc[v][e];
+            }
+        }
+    }
+}
+
+
 // ---------------------------------------------------------------------------
 // merge chunk sums per cluster (chunk order) and divide: C_j = (F)(S1_j / n_j).
 // Empty clusters are left for fill_empty_kernel.
@@ -326,8 +403,28 @@ __global__ __launch_bounds__(256) void objective_chunks_kernel(
             const F* x = X + row * d;
             const F* c = C + int64_t(labels[row]) * d;
             double acc = 0.0;
+            int f = 0;
             if (vec) {
-                for (int f = 0; f < d; f += 4) {
+                // whole 128-byte lines of the row per step (8 loads issued together),
+                // summed in feature order
+                constexpr int LV = 128 / (4 * int(sizeof(F)));  // 4-vectors per line
+                for (; f + 4 * LV <= d; f += 4 * LV) {
+                    F4 xv[LV], cv[LV];
+#pragma unroll
+                    for (int u = 0; u < LV; ++u) {
+                        xv[u] = *reinterpret_cast<const F4*>(x + f + 4 * u);
+                        cv[u] = *reinterpret_cast<const F4*>(c + f + 4 * u);
+                    }
+#pragma unroll
+                    for (int u = 0; u < LV; ++u)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const double diff = double(xv[u][e]) - double(cv[u][e]);
+                            const double sq = diff * diff;
+                            acc = acc + sq;
+                        }
+                }
+                for (; f < d; f += 4) {
                     const F4 x4 = *reinterpret_cast<const F4*>(x + f);
                     const F4 c4 = *reinterpret_cast<const F4*>(c + f);
 #pragma unroll
@@ -338,7 +435,7 @@ __global__ __launch_bounds__(256) void objective_chunks_kernel(
                     }
                 }
             } else {
-                for (int f = 0; f < d; ++f) {
+                for (; f < d; ++f) {
                     const double diff = double(x[f]) - double(c[f]);
                     const double sq = diff * diff;
                     acc = acc + sq;
