@@ -259,10 +259,16 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // lds_base + 16 l (wave-uniform, through M0).  Issued as inline asm so the
 // compiler does not insert its own conservative vmcnt(0) before every LDS read;
 // ordering is by the explicit vmcnt waits and barriers of the caller.
+// The compiler's hazard recognizer does not look inside inline asm, so the asm
+// carries its own wait states: the base SGPRs typically come from
+// v_readfirstlane just before, and a VMEM read of an SGPR written by a VALU op
+// needs 5 wait states (otherwise the load may use the register's previous
+// contents as the address); s_nop 4 gives 5, which also covers the one that
+// M0 -> LDS DMA needs.
 __device__ __forceinline__ void lds_dma16(const void* gbase, uint32_t voff, uint32_t lds_base) {
     asm volatile(
         "s_mov_b32 m0, %0\n\t"
-        "s_nop 0\n\t"
+        "s_nop 4\n\t"
         "global_load_lds_dwordx4 %1, %2"
         :
         : "s"(lds_base), "v"(voff), "s"(gbase)
@@ -276,11 +282,11 @@ __device__ __forceinline__ void wait_vm() {
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-// 4 bytes per lane, as lds_dma16: lane l writes lds_base + 4 l
+// 4 bytes per lane, as lds_dma16 (same wait states): lane l writes lds_base + 4 l
 __device__ __forceinline__ void lds_dma4(const void* gbase, uint32_t voff, uint32_t lds_base) {
     asm volatile(
         "s_mov_b32 m0, %0\n\t"
-        "s_nop 0\n\t"
+        "s_nop 4\n\t"
         "global_load_lds_dword %1, %2"
         :
         : "s"(lds_base), "v"(voff), "s"(gbase)
