@@ -260,17 +260,19 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // compiler does not insert its own conservative vmcnt(0) before every LDS read;
 // ordering is by the explicit vmcnt waits and barriers of the caller.
 // The compiler's hazard recognizer does not look inside inline asm, so the asm
-// carries its own wait states: the base SGPRs typically come from
-// v_readfirstlane just before, and a VMEM read of an SGPR written by a VALU op
-// needs 5 wait states (otherwise the load may use the register's previous
-// contents as the address); s_nop 4 gives 5, which also covers the one that
+// avoids the hazards itself: the base SGPRs typically come from v_readfirstlane
+// just before, and a VMEM read of an SGPR written by a VALU op needs 5 wait
+// states (otherwise the load may use the register's previous contents as the
+// address).  The load therefore reads a copy made by a scalar move (no wait
+// states after an SALU write), and that move is also the one wait state that
 // M0 -> LDS DMA needs.
 __device__ __forceinline__ void lds_dma16(const void* gbase, uint32_t voff, uint32_t lds_base) {
+    uint64_t base;
     asm volatile(
-        "s_mov_b32 m0, %0\n\t"
-        "s_nop 4\n\t"
-        "global_load_lds_dwordx4 %1, %2"
-        :
+        "s_mov_b32 m0, %1\n\t"
+        "s_mov_b64 %0, %3\n\t"
+        "global_load_lds_dwordx4 %2, %0"
+        : "=&s"(base)
         : "s"(lds_base), "v"(voff), "s"(gbase)
         : "memory", "m0");
 }
@@ -282,13 +284,14 @@ __device__ __forceinline__ void wait_vm() {
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-// 4 bytes per lane, as lds_dma16 (same wait states): lane l writes lds_base + 4 l
+// 4 bytes per lane, as lds_dma16 (same hazard handling): lane l writes lds_base + 4 l
 __device__ __forceinline__ void lds_dma4(const void* gbase, uint32_t voff, uint32_t lds_base) {
+    uint64_t base;
     asm volatile(
-        "s_mov_b32 m0, %0\n\t"
-        "s_nop 4\n\t"
-        "global_load_lds_dword %1, %2"
-        :
+        "s_mov_b32 m0, %1\n\t"
+        "s_mov_b64 %0, %3\n\t"
+        "global_load_lds_dword %2, %0"
+        : "=&s"(base)
         : "s"(lds_base), "v"(voff), "s"(gbase)
         : "memory", "m0");
 }
