@@ -57,7 +57,7 @@ def make_data(torch, n, d, k, center_range, sigma, anisotropic, seed, device):
     return x, c0
 
 
-def cpu_baseline(x_host, c0_host, iters, budget_s=20.0):
+def cpu_baseline(x_host, c0_host, iters, budget_s=15.0):
     """The oracle's restatement of the reference CPU Lloyd, timed on a bounded
     sample of the workload (first rows, same initial centroids)."""
     from oracle import oracle as O
@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP32-path and FP64-path comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, copy a 1 GiB device buffer three times (a plain copy "
+                         "of known size, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -181,8 +184,18 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        xs = x[: min(n, 400_000)].cpu().numpy()
-        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters)
+        xs = x[: min(n, 4_000_000)].cpu().numpy()
+        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters, budget_s=15.0)
+        del xs
+
+    if args.copy_calibration:
+        log("copy calibration: 3 x 1 GiB device-to-device copies ...")
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     out = {
         "metric": METRIC,

@@ -314,9 +314,13 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 }
 
 // LDS bytes of assign_bf16_kernel: ring of 32 x dpb bf16 blocks, and per ring
-// slot four 256-byte copies (one per wave) of the block's 32 cnB values
+// slot four 256-byte copies (one per wave) of the block's 32 cnB values; the
+// collect kernel adds per-row hit lists (a count and MAXC2 ids per tile row)
 __host__ inline size_t bf16_lds_bytes(int dpb) {
     return size_t(BF16_RING) * (size_t(32) * dpb * 2 + 1024);
+}
+__host__ inline size_t bf16_collect_lds_bytes(int dpb, int rows_per_wg) {
+    return bf16_lds_bytes(dpb) + size_t(rows_per_wg) * (MAXC2 + 1) * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -330,7 +334,9 @@ __host__ inline size_t bf16_lds_bytes(int dpb) {
 // The key work of block mb runs while the MFMAs of block mb+1 execute (two
 // accumulator sets), interleaved by scheduling groups.
 // Cb and cnb_g hold bf16_padded_k(k) rows, a multiple of four blocks.
-// COLLECT: rows come from L.coll_rows; every j with v_j <= T is recorded.
+// COLLECT: rows come from L.coll_rows; every j with v_j <= T is recorded in
+// per-row LDS lists (the wave that owns a row is the only writer, so no
+// barrier guards them) and flushed to L.coll_count / L.coll_ids once per tile.
 // ---------------------------------------------------------------------------
 template <int DPB, int NB, bool COLLECT>
 __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
@@ -356,6 +362,9 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char* ring = smem_raw;                                 // [RING][BLKB]
     unsigned char* ring_cn = smem_raw + size_t(BF16_RING) * BLKB;   // [RING][4 waves][64] floats
+    // collect kernel: hit count and ids of each tile row (local row = wave, block, lane)
+    int* hit_cnt = reinterpret_cast<int*>(ring_cn + BF16_RING * 1024);  // [ROWS_PER_WG]
+    int* hit_ids = hit_cnt + ROWS_PER_WG;                                // [ROWS_PER_WG][MAXC2]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -399,16 +408,11 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         const void* gc = uniform_ptr(cnb_g + mb * 32);
         lds_dma4(gc, voff_cn, moff_cn + slot * 1024);
     };
-    // wait until at most `pending` blocks' copies of this wave are in flight (the
-    // collect kernel's atomics also count, so it waits for everything)
     // Copies run three blocks ahead, also past the last block (those re-copy the
     // last block into a slot nobody reads again in this tile), so at every step
-    // the copies of exactly two later blocks may stay in flight.  The collect
-    // kernel's atomics also count, so it waits for everything.
-    auto wait_block = [&]() {
-        if (COLLECT) wait_vm<0>();
-        else wait_vm<2 * CPW>();
-    };
+    // the copies of exactly two later blocks may stay in flight (the block loop
+    // issues no other vector-memory operation: collect hits go to LDS).
+    auto wait_block = [&]() { wait_vm<2 * CPW>(); };
 
     // this wave's list segment (main kernel): at most cap rows pass through a wave
     const int seg = int(blockIdx.x) * 4 + wave;
@@ -417,7 +421,7 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     // rows of a tile: B operand fragments, ids, and the per-row threshold inputs
     bf16x8 xr[NB][STEPS];
     int64_t rows[NB];
-    double T[NB];
+    uint32_t TKc[NB];  // collect: value(key) <= T  <=>  key <= TKc (0: inactive row)
     float xn[NB], xe[NB];  // ||x||, ||x - bf16(x)|| of the row (for the threshold)
     auto load_tile = [&](int64_t tile) {
         const int64_t w0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
@@ -428,7 +432,19 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             int64_t row = e;
             if (COLLECT) {
                 row = ok ? int64_t(L.coll_rows[e]) : 0;
-                T[nb] = ok ? L.coll_T[e] : -1.0;
+                // keys are positive-float bits with the low IB bits clear, so
+                // value(key) <= T  <=>  key <= bits(largest float <= T); a NaN or
+                // negative T collects nothing (the row then goes to the full list)
+                uint32_t tkc = 0u;
+                if (ok) {
+                    const double T = L.coll_T[e];
+                    if (T >= 0.0) {
+                        float tf = float(T);
+                        if (double(tf) > T) tf = __uint_as_float(__float_as_uint(tf) - 1u);
+                        tkc = __float_as_uint(tf);
+                    }
+                }
+                TKc[nb] = tkc;
             } else {
                 xn[nb] = ok ? xnorm_up[row] : 0.f;
                 xe[nb] = ok ? enorm_up[row] : 0.f;
@@ -489,12 +505,12 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint32_t key = (__float_as_uint(acc[nb][i]) & kmask);
-                        if (rows[nb] >= 0 && double(__uint_as_float(key)) <= T[nb]) {
+                        if (key <= TKc[nb]) {
                             const int cj = mb * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
                             if (cj < k) {
-                                const int64_t e = w0 + nb * 32 + r32;
-                                const int pos = atomicAdd(&L.coll_count[e], 1);
-                                if (pos < MAXC2) L.coll_ids[e * MAXC2 + pos] = cj;
+                                const int lr = wave * ROWS_PER_WAVE + nb * 32 + r32;
+                                const int pos = atomicAdd(&hit_cnt[lr], 1);
+                                if (pos < MAXC2) hit_ids[lr * MAXC2 + pos] = cj;
                             }
                         }
                     }
@@ -553,6 +569,11 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         // for them inside the block loop, where vmcnt also counts the ring copies);
         // the previous tile's last readers are done before the prologue refills the ring
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        if (COLLECT) {
+            // this wave's rows start the tile with empty hit lists (the previous
+            // tile's flush, by this wave, read them before in LDS order)
+            for (int r = lane; r < ROWS_PER_WAVE; r += 64) hit_cnt[wave * ROWS_PER_WAVE + r] = 0;
+        }
         __syncthreads();
         issue_block(0, 0);
         issue_block(1, 1);
@@ -581,7 +602,23 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             xe_s[nb] = xe[nb];
         }
         if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
-        if (COLLECT) continue;
+        if (COLLECT) {
+            // flush this wave's hit lists: count (may exceed MAXC2: overflow) and
+            // the first MAXC2 ids of each row, row after row
+            for (int idx = lane; idx < ROWS_PER_WAVE * (MAXC2 + 1); idx += 64) {
+                const int r = idx / (MAXC2 + 1), q = idx - r * (MAXC2 + 1);
+                const int64_t e = w0 + r;
+                if (e < nrows) {
+                    const int lr = wave * ROWS_PER_WAVE + r;
+                    const int c = hit_cnt[lr];
+                    if (q == MAXC2)
+                        L.coll_count[e] = c;
+                    else if (q < c)
+                        L.coll_ids[e * MAXC2 + q] = hit_ids[lr * MAXC2 + q];
+                }
+            }
+            continue;
+        }
 
         // settle: with two row blocks, lane (r32, h) settles row r32 of block h (both
         // halves busy), with its own 4 slots of that block and the partner lane's 4
@@ -694,7 +731,6 @@ __global__ __launch_bounds__(256) void compact_lists_kernel(seg_lists S, int nse
         const int64_t src = base + i, dst = int64_t(bl) + i;
         L.coll_rows[dst] = S.coll_rows[src];
         L.coll_T[dst] = S.coll_T[src];
-        L.coll_count[dst] = 0;
     }
 }
 

@@ -55,7 +55,7 @@ struct fp_traits<double> {
 };
 
 // Candidate ids gathered per row by the BF16 collect pass.
-constexpr int MAXC2 = 16;
+constexpr int MAXC2 = 32;
 
 // Device-side work lists filled by the assignment kernels.
 struct refine_lists {

@@ -277,9 +277,13 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
                            nseg, w.L);
     } else {
         // row count is on the device; a fixed grid strides over it
-        hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, true>), dim3(ctx->cu_count * 2), dim3(256),
-                           lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params,
-                           kc, w.labels, w.L, seg_lists{});
+        const size_t clds = bf16_collect_lds_bytes(DPB, ROWS_PER_WG);
+        auto ckern = assign_bf16_kernel<DPB, NB, true>;
+        KM_HIP(hipFuncSetAttribute((const void*)ckern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(clds)));
+        hipLaunchKernelGGL(ckern, dim3(ctx->cu_count * 2), dim3(256), clds, ctx->stream, w.Xb, w.n,
+                           w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params, kc, w.labels, w.L,
+                           seg_lists{});
     }
     KM_LAUNCH_CHECK();
 }
@@ -353,8 +357,13 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
                             w.L.counters + 2);
         }
     }
-    hipLaunchKernelGGL(refine_full_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d, C,
-                       w.k, w.L, w.labels);
+    if (w.d % 4 == 0 && reinterpret_cast<uintptr_t>(w.X) % (4 * sizeof(F)) == 0 &&
+        reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F)) == 0)
+        hipLaunchKernelGGL(refine_full_vec_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X,
+                           w.d, C, w.k, w.L, w.labels);
+    else
+        hipLaunchKernelGGL(refine_full_kernel<F>, dim3(rgrid), dim3(256), 0, ctx->stream, w.X, w.d,
+                           C, w.k, w.L, w.labels);
     KM_LAUNCH_CHECK();
 }
 

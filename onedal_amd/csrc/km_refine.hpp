@@ -250,4 +250,124 @@ __global__ __launch_bounds__(256) void refine_full_kernel(const F* __restrict__ 
     }
 }
 
+// refine_full_kernel for d % 4 == 0 with 4-vector aligned X and C.  A workgroup
+// takes 32 listed rows (8 per wave), staged in LDS as 4-vectors (feature chunks of
+// 32 KB when the rows do not fit); lane l carries centroid j0 + l of a 64-centroid
+// tile with its features in registers, a register chunk of 32 bytes per feature
+// group loaded one chunk ahead of use, and forms the exact-form sums of the
+// wave's 8 rows in feature order, reading the rows as LDS broadcasts.  Same scan
+// semantics as refine_full_kernel.
+template <typename F>
+__global__ __launch_bounds__(256) void refine_full_vec_kernel(const F* __restrict__ X, int d,
+                                                              const F* __restrict__ C, int k,
+                                                              const refine_lists L,
+                                                              int32_t* __restrict__ labels) {
+    typedef F F4 __attribute__((ext_vector_type(4)));
+    constexpr int RW = 8, RB = 4 * RW;
+    constexpr int VT = 32768 / (RB * int(sizeof(F4)));  // 4-vectors per staged row chunk
+    constexpr int CQ = 32 / int(sizeof(F));             // 4-vectors per register chunk
+    __shared__ F4 Xs[RB][VT];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int total = L.counters[1];
+    const int nbatch = (total + RB - 1) / RB;
+    const int nv = d / 4;
+    const bool one_chunk = nv <= VT;
+    const F4* X4 = reinterpret_cast<const F4*>(X);
+    auto stage = [&](int b, int v0, int vt) {
+        for (int e = tid; e < RB * vt; e += 256) {
+            const int r = e / vt, q = e - r * vt;
+            const int le = b * RB + r;
+            const int64_t row = le < total ? int64_t(L.full_rows[le]) : 0;
+            Xs[r][q] = X4[row * nv + v0 + q];
+        }
+    };
+    for (int b = blockIdx.x; b < nbatch; b += gridDim.x) {
+        int64_t rows[RW];
+        F best[RW];
+        int bi[RW];
+        bool nan0[RW];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int e = b * RB + wave * RW + r;
+            rows[r] = e < total ? int64_t(L.full_rows[e]) : -1;
+            best[r] = fp_traits<F>::inf;
+            bi[r] = 0x7fffffff;
+            nan0[r] = false;
+        }
+        if (one_chunk) {
+            __syncthreads();
+            stage(b, 0, nv);
+            __syncthreads();
+        }
+        for (int j0 = 0; j0 < k; j0 += 64) {
+            const int j = j0 + lane;
+            const F4* c = reinterpret_cast<const F4*>(C + int64_t(j < k ? j : 0) * d);
+            F acc[RW];
+#pragma unroll
+            for (int r = 0; r < RW; ++r) acc[r] = F(0);
+            for (int v0 = 0; v0 < nv; v0 += VT) {
+                const int vt = min(VT, nv - v0);
+                if (!one_chunk) {
+                    __syncthreads();
+                    stage(b, v0, vt);
+                    __syncthreads();
+                }
+                F4 cur[CQ], nxt[CQ];
+#pragma unroll
+                for (int q = 0; q < CQ; ++q) cur[q] = q < vt ? c[v0 + q] : F4{};
+                for (int u0 = 0; u0 < vt; u0 += CQ) {
+#pragma unroll
+                    for (int q = 0; q < CQ; ++q)
+                        nxt[q] = u0 + CQ + q < vt ? c[v0 + u0 + CQ + q] : F4{};
+#pragma unroll
+                    for (int q = 0; q < CQ; ++q) {
+                        if (u0 + q < vt) {
+#pragma unroll
+                            for (int r = 0; r < RW; ++r) {
+                                const F4 x4 = Xs[wave * RW + r][u0 + q];
+#pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const F diff = x4[t] - cur[q][t];
+                                    const F sq = diff * diff;
+                                    acc[r] = acc[r] + sq;
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < CQ; ++q) cur[q] = nxt[q];
+                }
+            }
+            if (j < k) {
+#pragma unroll
+                for (int r = 0; r < RW; ++r) {
+                    if (j == 0 && acc[r] != acc[r]) nan0[r] = true;
+                    if (acc[r] < best[r]) {
+                        best[r] = acc[r];
+                        bi[r] = j;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            F bv = best[r];
+            int bj = bi[r];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const F ob = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bj, off);
+                if (ob < bv || (ob == bv && oi < bj)) {
+                    bv = ob;
+                    bj = oi;
+                }
+            }
+            const bool n0 = __shfl(nan0[r] ? 1 : 0, 0) != 0;
+            if (lane == 0 && rows[r] >= 0) labels[rows[r]] = (n0 || bj == 0x7fffffff) ? 0 : bj;
+        }
+    }
+}
+
 }  // namespace km
