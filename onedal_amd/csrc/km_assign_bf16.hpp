@@ -237,9 +237,15 @@ __global__ void cnb_kernel(const float* __restrict__ cn, int k, int kp,
         cnb[j] = j < k ? add_round_up(cn[j], Bf) : __uint_as_float(0x7F000000u);
 }
 
-// Centroid blocks live in a ring of BF16_RING LDS buffers, filled BF16_RING-1
-// blocks ahead with direct-to-LDS loads.
-constexpr int BF16_RING = 4;
+// Centroid blocks live in a ring of LDS slots, filled ahead of use with
+// direct-to-LDS loads.  The main kernel uses 4 slots (one barrier per block,
+// copies three blocks ahead; 8 slots with a barrier per two blocks measured
+// slower).  The collect kernel, whose LDS also holds hit lists, uses 4 slots, or
+// 2 for 16 KB blocks (copies one ahead).  A ring of 8 would use one barrier per
+// two blocks (bf16 kernel GRP).
+__host__ __device__ constexpr int bf16_ring(int dpb, bool collect) {
+    return collect ? (dpb <= 128 ? 4 : 2) : 4;
+}
 
 // a wave-uniform pointer moved to scalar registers (an SGPR asm operand)
 __device__ __forceinline__ const void* uniform_ptr(const void* p) {
@@ -314,20 +320,21 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 }
 
 // LDS bytes of assign_bf16_kernel: ring of 32 x dpb bf16 blocks, and per ring
-// slot four 256-byte copies (one per wave) of the block's 32 cnB values; the
-// collect kernel adds per-row hit lists (a count and MAXC2 ids per tile row)
+// slot 256 bytes holding the block's 32 cnB values twice; the collect kernel
+// adds per-row hit lists (a count and MAXC2 ids per tile row).  Every layout stays
+// below 68 KB (67584 bytes at most).
 __host__ inline size_t bf16_lds_bytes(int dpb) {
-    return size_t(BF16_RING) * (size_t(32) * dpb * 2 + 1024);
+    return size_t(bf16_ring(dpb, false)) * (size_t(32) * dpb * 2 + 256);
 }
 __host__ inline size_t bf16_collect_lds_bytes(int dpb, int rows_per_wg) {
-    return bf16_lds_bytes(dpb) + size_t(rows_per_wg) * (MAXC2 + 1) * 4;
+    return size_t(bf16_ring(dpb, true)) * (size_t(32) * dpb * 2 + 256) +
+           size_t(rows_per_wg) * (MAXC2 + 1) * 4;
 }
 
 // ---------------------------------------------------------------------------
 // Main / collect kernel.  Workgroup = 4 waves; each wave owns NB blocks of 32
 // rows (B operand in VGPRs, 8 bf16 per step); centroid blocks of 32 rows (A
-// operand) stream through an LDS ring by global_load_lds_dwordx4, three blocks
-// ahead of use.  A block is stored unpadded, 16-byte chunk c of row r at chunk
+// operand) stream through an LDS ring by global_load_lds_dwordx4 (bf16_ring).  A block is stored unpadded, 16-byte chunk c of row r at chunk
 // c ^ (r & SWZ), so the 32 lanes reading one chunk column hit distinct banks.
 // Lane l = 32*h + r: half h covers features [h*DPB/2, (h+1)*DPB/2), 8 per step.
 // Slots: lane keeps 4 (centroid block index mod 4); a row has 8.
@@ -356,14 +363,17 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     constexpr int DPW = NDATA >= 4 ? NDATA / 4 : 1;  // data copies per wave per block
     constexpr int CPW = DPW + 1;                  // + the wave's quarter of the block's cnB
     constexpr int NSL = 4;
+    constexpr int RING = bf16_ring(DPB, COLLECT);  // LDS slots
+    constexpr int GRP = RING >= 8 ? 2 : 1;         // blocks per barrier
+    constexpr int AHEAD = RING - GRP;              // blocks copied ahead of use
     // VALU ops of one block's key work (5 per pair of distances) per MFMA
     constexpr int VPM = (NB * 40 + STEPS * NB - 1) / (STEPS * NB);
     static_assert(NDATA >= 2 && (NDATA < 4 || NDATA % 4 == 0), "block copy layout");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char* ring = smem_raw;                                 // [RING][BLKB]
-    unsigned char* ring_cn = smem_raw + size_t(BF16_RING) * BLKB;   // [RING][4 waves][64] floats
+    unsigned char* ring_cn = smem_raw + size_t(RING) * BLKB;   // [RING][64] floats
     // collect kernel: hit count and ids of each tile row (local row = wave, block, lane)
-    int* hit_cnt = reinterpret_cast<int*>(ring_cn + BF16_RING * 1024);  // [ROWS_PER_WG]
+    int* hit_cnt = reinterpret_cast<int*>(ring_cn + RING * 256);  // [ROWS_PER_WG]
     int* hit_ids = hit_cnt + ROWS_PER_WG;                                // [ROWS_PER_WG][MAXC2]
 
     const int tid = threadIdx.x;
@@ -386,9 +396,10 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     // of waves write the same two copies, the same bytes); lane l moves 16-byte
     // chunk p = 64 q + l, i.e. physical row p / CPR, physical chunk p % CPR, which
     // holds logical chunk (p % CPR) ^ (row & SWZ).  Then every wave moves the
-    // block's 32 cnB values into its own 256 bytes (lane l: value l & 31; all
-    // lanes active, so the block loop has no divergent branch).  Every wave issues
-    // CPW copies per block, so the vmcnt waits are uniform.
+    // block's 32 cnB values into the slot's 256 bytes (lane l: value l & 31; all
+    // four waves write the same bytes, all lanes active, so the block loop has no
+    // divergent branch).  Every wave issues CPW copies per block, so the vmcnt
+    // waits are uniform.
     uint32_t voff[DPW], moff[DPW];
 #pragma unroll
     for (int c = 0; c < DPW; ++c) {
@@ -400,19 +411,20 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         moff[c] = lds_addr(ring + q * 1024);
     }
     const uint32_t voff_cn = uint32_t((lane & 31) * 4);
-    const uint32_t moff_cn = lds_addr(ring_cn + 256 * wave);
+    const uint32_t moff_cn = lds_addr(ring_cn);
     auto issue_block = [&](int mb, int slot) {
         const void* gb = uniform_ptr(reinterpret_cast<const unsigned char*>(Cb) + size_t(mb) * BLKB);
 #pragma unroll
         for (int c = 0; c < DPW; ++c) lds_dma16(gb, voff[c], moff[c] + slot * BLKB);
         const void* gc = uniform_ptr(cnb_g + mb * 32);
-        lds_dma4(gc, voff_cn, moff_cn + slot * 1024);
+        lds_dma4(gc, voff_cn, moff_cn + slot * 256);
     };
-    // Copies run three blocks ahead, also past the last block (those re-copy the
-    // last block into a slot nobody reads again in this tile), so at every step
-    // the copies of exactly two later blocks may stay in flight (the block loop
-    // issues no other vector-memory operation: collect hits go to LDS).
-    auto wait_block = [&]() { wait_vm<2 * CPW>(); };
+    // Copies run AHEAD blocks ahead, also past the last block (those re-copy the
+    // last block into a slot nobody reads again in this tile), GRP at a time, so at
+    // every barrier the copies of exactly AHEAD - GRP later blocks may stay in
+    // flight (the block loop issues no other vector-memory operation: collect hits
+    // go to LDS).
+    auto wait_block = [&]() { wait_vm<(AHEAD - GRP) * CPW>(); };
 
     // this wave's list segment (main kernel): at most cap rows pass through a wave
     const int seg = int(blockIdx.x) * 4 + wave;
@@ -474,7 +486,7 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         // distances of the block in ring slot `slot`: all A fragments and start
         // values first, then the MFMA chain
         auto mfma_block = [&](int slot, floatx16(&acc)[NB]) {
-            const float* cnb = reinterpret_cast<const float*>(ring_cn + slot * 1024 + wave * 256);
+            const float* cnb = reinterpret_cast<const float*>(ring_cn + slot * 256);
             const unsigned char* arow = ring + slot * BLKB + r32 * ROWB;
             bf16x8 a[STEPS];
 #pragma unroll
@@ -520,10 +532,12 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 // register pairs outermost: each (block, register) tag is live briefly
 #pragma unroll
                 for (int i = 0; i < 16; i += 2) {
-                    // tags formed on the scalar unit, so a key is one v_and_or_b32
+                    // tags formed on the scalar unit, so a key is one v_and_or_b32;
+                    // s_or_b32 writes SCC, which the compiler must not keep a
+                    // branch condition in across this asm (hence the clobber)
                     uint32_t ta, tb;
-                    asm("s_or_b32 %0, %1, %2" : "=s"(ta) : "s"(tag), "i"(i));
-                    asm("s_or_b32 %0, %1, %2" : "=s"(tb) : "s"(tag), "i"(i + 1));
+                    asm("s_or_b32 %0, %1, %2" : "=s"(ta) : "s"(tag), "i"(i) : "scc");
+                    asm("s_or_b32 %0, %1, %2" : "=s"(tb) : "s"(tag), "i"(i + 1) : "scc");
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
                         // value bits where kmask is set, (block, register) below
@@ -549,20 +563,26 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             }
         };
 
-        // step mb: wait for block mb's copies (blocks mb+1, mb+2 may stay in flight),
-        // barrier (all waves' copies visible; everyone is done with block mb-1),
-        // refill the slot of block mb-1 with block mb+3, then the MFMAs of block mb
-        // (ring slot and top-2 slot mb & 3) beside the keys of block mb-1
+        // step mb (u = mb mod 4): at the first block of a group, wait for the
+        // group's copies (the next AHEAD - GRP blocks may stay in flight), barrier
+        // (all waves' copies visible; everyone is done with the previous group),
+        // refill the previous group's slots with blocks mb+AHEAD .. ; then the
+        // MFMAs of block mb (ring slot mb mod RING) beside the keys of block mb-1
+        // (top-2 slot (mb-1) mod 4)
         floatx16 acc0[NB], acc1[NB];
-        auto step = [&](int mb, int slot, floatx16(&cur)[NB], floatx16(&prev)[NB]) {
-            // nothing is scheduled across the wait, the barrier and the copies
-            __builtin_amdgcn_sched_barrier(0);
-            wait_block();
-            __syncthreads();
-            issue_block(min(mb + 3, mblocks - 1), (slot + 3) & (BF16_RING - 1));
-            __builtin_amdgcn_sched_barrier(0);
-            mfma_block(slot, cur);
-            keys_block(prev, mb - 1, (slot + 3) & (BF16_RING - 1));
+        auto step = [&](int mb, int u, floatx16(&cur)[NB], floatx16(&prev)[NB]) {
+            if (u % GRP == 0) {
+                // nothing is scheduled across the wait, the barrier and the copies
+                __builtin_amdgcn_sched_barrier(0);
+                wait_block();
+                __syncthreads();
+#pragma unroll
+                for (int g = 0; g < GRP; ++g)
+                    issue_block(min(mb + AHEAD + g, mblocks - 1), (mb + AHEAD + g) & (RING - 1));
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            mfma_block(mb & (RING - 1), cur);
+            keys_block(prev, mb - 1, (u + 3) & 3);
             if (!COLLECT) interleave();
         };
         // row loads complete here (a waitcnt the compiler sees, so it does not re-wait
@@ -575,9 +595,8 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             for (int r = lane; r < ROWS_PER_WAVE; r += 64) hit_cnt[wave * ROWS_PER_WAVE + r] = 0;
         }
         __syncthreads();
-        issue_block(0, 0);
-        issue_block(1, 1);
-        issue_block(2, 2);
+#pragma unroll
+        for (int b = 0; b < AHEAD; ++b) issue_block(min(b, mblocks - 1), b);
         // the first step's "previous block" (-1): all-ones accumulators give keys
         // 0xFFFFFFFx, above every real key (each slot receives real keys later,
         // which displace them) and NaN-valued, so never below a threshold
@@ -671,6 +690,9 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
             for (int q = 0; q < 2 * NSL; ++q) {
                 cert = cert && k2[q] > TK;
                 cnt += k1[q] <= TK ? 1 : 0;
+                // an id past k (a padding centroid) is never certified; such a row
+                // goes to the collect pass, which records ids below k only
+                cert = cert && (k1[q] > TK || idx_of(k1[q], q < NSL ? h : 1 - h) < k);
             }
             int outcome = 0;  // 1: candidate entry, 2: collect entry
             if (active) {

@@ -269,9 +269,11 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
         S.coll_rows = ctx->seg_coll_rows.get<int32_t>(ents);
         S.coll_T = ctx->seg_coll_T.get<double>(ents);
         S.counts = ctx->seg_counts.get<int32_t>(2 * int64_t(nseg));
-        hipLaunchKernelGGL((assign_bf16_kernel<DPB, NB, false>), dim3(grid), dim3(256), lds,
-                           ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb, w.xnorm, w.enorm, w.params, kc,
-                           w.labels, w.L, S);
+        auto kern = assign_bf16_kernel<DPB, NB, false>;
+        KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(lds)));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, w.Xb, w.n, w.Cb, w.k, w.cnb,
+                           w.xnorm, w.enorm, w.params, kc, w.labels, w.L, S);
         KM_LAUNCH_CHECK();
         hipLaunchKernelGGL(compact_lists_kernel, dim3((nseg + 3) / 4), dim3(256), 0, ctx->stream, S,
                            nseg, w.L);
