@@ -19,6 +19,13 @@ sys.path.insert(0, ROOT)
 METRIC = ("k-means point-iters/s + %roofline, bf16+refine vs fp32 path; p50/p99 job latency")
 PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 
+# HBM bytes per launch of the main assignment kernel, from two rocprofv3 counter
+# passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy of the same run
+# (profiles/traffic_precision_n20M_d128_k1024.md); hardware counters cannot be read
+# from inside this process.
+MEASURED_TRAFFIC = {"precision_c3": {"bytes": 5.537e9, "read": 5.299e9, "write": 0.238e9,
+                                     "source": "profiles/traffic_precision_n20M_d128_k1024.md"}}
+
 WORKLOADS = {
     # Precision form: n=2e7, d=128, k=1024, 1024 anisotropic Gaussians (per-dimension
     # scale log-uniform in [0.1,10], centers U[-50,50]^d), random-row init, 20 iterations,
@@ -143,9 +150,14 @@ def main():
     flops_per_launch = 2.0 * k * d * n
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     peak = PEAK["bf16_mfma_tflops"] if wl["bf16"] else PEAK["fp32_mfma_tflops"]
+    mt = MEASURED_TRAFFIC.get(args.workload)
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4), "traffic": mt["bytes"] if mt else None,
             "kernel": "assign (fused distance GEMM + argmin)", "avg_launch_ms": round(avg_ms, 4)}
+    if mt:
+        roof["traffic_detail"] = {"unit": "bytes per launch", "read": mt["read"],
+                                  "write": mt["write"], "source": mt["source"],
+                                  "hbm_gbs_at_avg_launch": round(mt["bytes"] / (avg_ms * 1e-3) / 1e9, 1)}
 
     compare = None
     if wl["bf16"] and not args.no_compare:
