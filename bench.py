@@ -197,7 +197,7 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         xs = x[: min(n, 4_000_000)].cpu().numpy()
-        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters, budget_s=15.0)
+        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters, budget_s=35.0)
         del xs
 
     if args.copy_calibration:

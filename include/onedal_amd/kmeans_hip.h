@@ -54,6 +54,8 @@ typedef enum {
                                         data and d > 256. */
 #define KM_FLAG_DEVICE_POINTERS 0x4u /* X, C0/C and every output are device pointers
                                         on the context's device */
+#define KM_FLAG_GRAPH 0x8u           /* batched training: capture one iteration as a
+                                        HIP graph and replay it */
 
 /* context options (km_context_set_option) */
 #define KM_OPT_ASSIGN_PATH 1 /* 0 auto, 1 generic tile kernel, 2 FP32 matrix core */
@@ -99,6 +101,30 @@ km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, 
                              int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
 km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
                              int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
+
+/* One instance of a batched Lloyd training; all pointers are device memory. */
+typedef struct {
+    const float* X;      /* n x d row-major points */
+    const float* C0;     /* k x d row-major initial centroids */
+    float* C_out;        /* k x d final centroids */
+    int32_t* labels_out; /* n final labels */
+    int64_t n, d, k;
+} km_batch_instance_f32;
+
+/* Batched Lloyd training: `count` independent instances (each its own n, d, k),
+ * max_iter iterations each (accuracy threshold 0: the reference GPU driver then
+ * runs every iteration, train_kernel_lloyd_dense_dpc.cpp:147-168).  Each
+ * instance's centroids, labels and objective equal a km_lloyd_train_f32 call on
+ * that instance alone, bit for bit.  Every stage of an iteration is one launch over
+ * all instances and the iteration needs no host synchronisation; with
+ * KM_FLAG_GRAPH the iteration is captured once as a HIP graph and replayed.
+ * obj_out and iters_out are host arrays of `count` entries.  The total row count
+ * of a batch must stay below 2^31.  No counterpart in the reference, which trains
+ * one instance per call (train_kernel_gpu<F, lloyd_dense>::operator(),
+ * backend/gpu/train_kernel.hpp:24-29); this is the batched form of that call. */
+km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances, int64_t count,
+                             int64_t max_iter, uint32_t flags, double* obj_out,
+                             int64_t* iters_out);
 
 #ifdef __cplusplus
 }

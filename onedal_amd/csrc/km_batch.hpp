@@ -1,0 +1,622 @@
+// km_batch.hpp -- batched Lloyd: thousands of independent instances per launch.
+//
+// The reference runs one instance per train() call
+// (cpp/oneapi/dal/algo/kmeans/backend/gpu/train_kernel_lloyd_dense_dpc.cpp:76-207).
+// Here every stage of one Lloyd iteration is one launch over all instances of a
+// batch, so the iteration has no host synchronisation and no per-instance launch
+// cost; the stages are the single-instance ones, and every instance's result is
+// the single-instance result bit for bit (same arithmetic order):
+//   exact-form assignment (features in order, ties to the lowest index)
+//   -> label histogram -> per-instance scan (member offsets, chunk bases, empty
+//   list) -> stable segmented sort of row ids by label -> chunk sums in FP64
+//   (REDUCE_CHUNK members, reduce_groups interleave) -> merge/divide
+//   -> empty clusters refilled on the device with the farthest rows (largest d64,
+//   ties to the lowest row) in ascending cluster order.
+// Work items (row batches, chunks, centroid elements) of all instances form one
+// index space; a workgroup or wave finds its instance by binary search in a
+// prefix table.
+#pragma once
+
+#include "km_common.hpp"
+
+namespace km {
+
+// One instance as the kernels see it.  Per-row arrays (labels, sorted row ids,
+// d64) are concatenated over the batch (row0); centroid arrays too (c0, a
+// multiple of 4 floats); cluster arrays (counts, empty lists) at k0, with the
+// k + 1 entries of offsets and chunk bases at k0 + index; chunk partial sums at p0
+// (room for n / REDUCE_CHUNK + k + 1 chunks of d doubles); objective chunk sums
+// at o0.
+struct batch_inst {
+    const float* X;
+    const float* C0;
+    float* C_out;
+    int32_t* labels_out;
+    int64_t n;
+    int64_t row0;
+    int64_t c0;
+    int64_t k0;
+    int64_t p0;
+    int64_t o0;
+    int32_t d;
+    int32_t k;
+    int32_t index;
+    int32_t vec;  // d % 4 == 0 and X 16-byte aligned
+};
+
+constexpr int BATCH_RW = 8;               // assignment rows per wave
+constexpr int BATCH_RB = 4 * BATCH_RW;    // assignment rows per workgroup
+constexpr int BATCH_CNT_ROWS = 4096;      // rows per histogram workgroup
+constexpr int BATCH_D64_ROWS = 256;       // rows per d64 workgroup
+constexpr int BATCH_HIST_MAX = 4096;      // LDS histogram bins
+
+// instance owning work item w: the last i with pre[i] <= w (pre[B] > w)
+__device__ __forceinline__ int batch_find(const int64_t* __restrict__ pre, int B, int64_t w) {
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= w) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Exact-form assignment of BATCH_RB rows of one instance per workgroup (8 per
+// wave) against all its centroids: lane l carries centroid j0 + l of a
+// 64-centroid tile, its features in registers (32 at a time, the next 32 loaded
+// ahead), the rows are staged in LDS (feature chunks of 256) and read as
+// broadcasts; each (row, centroid) sum runs over the features in order with
+// sub, mul and add rounded separately.  Scan semantics of
+// select_min_distance (kernels_fp_impl.hpp:198-202): strict < in centroid order,
+// a NaN distance to centroid 0 keeps label 0.
+// labels: concatenated labels (row0) or, with to_user, each instance's output.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __restrict__ I, int B,
+                                                           const int64_t* __restrict__ pre,
+                                                           const float* __restrict__ C,
+                                                           int32_t* __restrict__ labels,
+                                                           int to_user) {
+    constexpr int RW = BATCH_RW, RB = BATCH_RB;
+    constexpr int FT = 256;  // staged features per row chunk
+    constexpr int CQ = 8;    // 4-vectors per register chunk
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) float Xs[RB][FT];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t total = pre[B];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int ii = batch_find(pre, B, b);
+        const batch_inst in = I[ii];
+        const int d = in.d, k = in.k;
+        const int64_t r0 = (b - pre[ii]) * RB;
+        const int nr = int(min<int64_t>(RB, in.n - r0));
+        const float* Ci = C + in.c0;
+        const bool vec = in.vec != 0;
+        float best[RW];
+        int bi[RW];
+        bool nan0[RW];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            best[r] = __builtin_inff();
+            bi[r] = 0x7fffffff;
+            nan0[r] = false;
+        }
+        for (int j0 = 0; j0 < k; j0 += 64) {
+            const int j = j0 + lane;
+            const float* c = Ci + int64_t(j < k ? j : 0) * d;
+            float acc[RW];
+#pragma unroll
+            for (int r = 0; r < RW; ++r) acc[r] = 0.f;
+            for (int f0 = 0; f0 < d; f0 += FT) {
+                const int ft = min(FT, d - f0);
+                __syncthreads();
+                if (vec) {
+                    const int nv = ft / 4;
+                    for (int e = tid; e < RB * nv; e += 256) {
+                        const int r = e / nv, q = e - r * nv;
+                        F4 v = F4{0.f, 0.f, 0.f, 0.f};
+                        if (r < nr)
+                            v = *reinterpret_cast<const F4*>(in.X + (r0 + r) * d + f0 + 4 * q);
+                        *reinterpret_cast<F4*>(&Xs[r][4 * q]) = v;
+                    }
+                } else {
+                    for (int e = tid; e < RB * ft; e += 256) {
+                        const int r = e / ft, q = e - r * ft;
+                        Xs[r][q] = r < nr ? in.X[(r0 + r) * d + f0 + q] : 0.f;
+                    }
+                }
+                __syncthreads();
+                if (vec) {
+                    const int nv = ft / 4;
+                    const F4* c4 = reinterpret_cast<const F4*>(c + f0);
+                    F4 cur[CQ], nxt[CQ];
+#pragma unroll
+                    for (int q = 0; q < CQ; ++q) cur[q] = q < nv ? c4[q] : F4{};
+                    for (int u0 = 0; u0 < nv; u0 += CQ) {
+#pragma unroll
+                        for (int q = 0; q < CQ; ++q) nxt[q] = u0 + CQ + q < nv ? c4[u0 + CQ + q] : F4{};
+#pragma unroll
+                        for (int q = 0; q < CQ; ++q) {
+                            if (u0 + q < nv) {
+#pragma unroll
+                                for (int r = 0; r < RW; ++r) {
+                                    const F4 x4 = *reinterpret_cast<const F4*>(&Xs[wave * RW + r][4 * (u0 + q)]);
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t) {
+                                        const float diff = x4[t] - cur[q][t];
+                                        const float sq = diff * diff;
+                                        acc[r] = acc[r] + sq;
+                                    }
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < CQ; ++q) cur[q] = nxt[q];
+                    }
+                } else {
+                    for (int f = 0; f < ft; ++f) {
+                        const float cf = c[f0 + f];
+#pragma unroll
+                        for (int r = 0; r < RW; ++r) {
+                            const float diff = Xs[wave * RW + r][f] - cf;
+                            const float sq = diff * diff;
+                            acc[r] = acc[r] + sq;
+                        }
+                    }
+                }
+            }
+            if (j < k) {
+#pragma unroll
+                for (int r = 0; r < RW; ++r) {
+                    if (j == 0 && acc[r] != acc[r]) nan0[r] = true;
+                    if (acc[r] < best[r]) {
+                        best[r] = acc[r];
+                        bi[r] = j;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            float bv = best[r];
+            int bj = bi[r];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const float ob = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bj, off);
+                if (ob < bv || (ob == bv && oi < bj)) {
+                    bv = ob;
+                    bj = oi;
+                }
+            }
+            const bool n0 = __shfl(nan0[r] ? 1 : 0, 0) != 0;
+            const int rr = wave * RW + r;
+            if (lane == 0 && rr < nr) {
+                const int lab = (n0 || bj == 0x7fffffff) ? 0 : bj;
+                if (to_user)
+                    in.labels_out[r0 + rr] = lab;
+                else
+                    labels[in.row0 + r0 + rr] = lab;
+            }
+        }
+    }
+}
+
+// label histogram: BATCH_CNT_ROWS rows of one instance per workgroup
+__global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __restrict__ I, int B,
+                                                          const int64_t* __restrict__ pre,
+                                                          const int32_t* __restrict__ labels,
+                                                          int32_t* __restrict__ counts) {
+    __shared__ int hist[BATCH_HIST_MAX];
+    const int64_t total = pre[B];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int ii = batch_find(pre, B, b);
+        const batch_inst in = I[ii];
+        const int64_t r0 = (b - pre[ii]) * BATCH_CNT_ROWS;
+        const int nr = int(min<int64_t>(BATCH_CNT_ROWS, in.n - r0));
+        const bool lds = in.k <= BATCH_HIST_MAX;
+        __syncthreads();
+        if (lds)
+            for (int j = threadIdx.x; j < in.k; j += 256) hist[j] = 0;
+        __syncthreads();
+        for (int r = threadIdx.x; r < nr; r += 256) {
+            const int lab = labels[in.row0 + r0 + r];
+            if (lab >= 0 && lab < in.k) {
+                if (lds) atomicAdd(&hist[lab], 1);
+                else atomicAdd(&counts[in.k0 + lab], 1);
+            }
+        }
+        __syncthreads();
+        if (lds)
+            for (int j = threadIdx.x; j < in.k; j += 256)
+                if (hist[j]) atomicAdd(&counts[in.k0 + j], hist[j]);
+    }
+}
+
+// Per instance (one workgroup each): member offsets, chunk bases and the list of
+// empty clusters from the counts, as scan_clusters_kernel; meta[2i] = #empty,
+// meta[2i+1] = #chunks.
+__global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __restrict__ I, int B,
+                                                          const int32_t* __restrict__ counts,
+                                                          int64_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ chunk_base,
+                                                          int32_t* __restrict__ empty_list,
+                                                          int32_t* __restrict__ meta) {
+    __shared__ int64_t s_off[1024];
+    __shared__ int32_t s_chk[1024];
+    __shared__ int32_t s_emp[1024];
+    __shared__ int64_t carry_off;
+    __shared__ int32_t carry_chk, carry_emp;
+    const int t = threadIdx.x;
+    for (int ii = blockIdx.x; ii < B; ii += gridDim.x) {
+        const batch_inst in = I[ii];
+        const int k = in.k;
+        const int32_t* cnt = counts + in.k0;
+        int64_t* off = offsets + in.k0 + ii;
+        int32_t* cb = chunk_base + in.k0 + ii;
+        int32_t* emp = empty_list + in.k0;
+        __syncthreads();
+        if (t == 0) {
+            carry_off = 0;
+            carry_chk = 0;
+            carry_emp = 0;
+        }
+        __syncthreads();
+        for (int base = 0; base < k; base += 1024) {
+            const int j = base + t;
+            const int32_t c = j < k ? cnt[j] : 0;
+            s_off[t] = c;
+            s_chk[t] = j < k ? (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK : 0;
+            s_emp[t] = (j < k && c == 0) ? 1 : 0;
+            __syncthreads();
+            for (int o = 1; o < 1024; o <<= 1) {
+                int64_t a = 0;
+                int32_t bch = 0, e = 0;
+                if (t >= o) {
+                    a = s_off[t - o];
+                    bch = s_chk[t - o];
+                    e = s_emp[t - o];
+                }
+                __syncthreads();
+                s_off[t] += a;
+                s_chk[t] += bch;
+                s_emp[t] += e;
+                __syncthreads();
+            }
+            if (j < k) {
+                off[j] = carry_off + s_off[t] - c;
+                cb[j] = carry_chk + s_chk[t] - (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK;
+                if (c == 0) emp[carry_emp + s_emp[t] - 1] = j;
+            }
+            __syncthreads();
+            if (t == 1023) {
+                carry_off += s_off[1023];
+                carry_chk += s_chk[1023];
+                carry_emp += s_emp[1023];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            off[k] = carry_off;
+            cb[k] = carry_chk;
+            meta[2 * ii] = carry_emp;
+            meta[2 * ii + 1] = carry_chk;
+        }
+    }
+}
+
+// exclusive prefix over instances of their chunk counts (one workgroup):
+// qpre[i] = first global chunk of instance i, qpre[B] = total
+__global__ __launch_bounds__(1024) void batch_chunk_prefix_kernel(const int32_t* __restrict__ meta,
+                                                                  int B, int64_t* __restrict__ qpre) {
+    __shared__ int64_t s[1024];
+    __shared__ int64_t carry;
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < B; base += 1024) {
+        const int i = base + t;
+        const int64_t v = i < B ? int64_t(meta[2 * i + 1]) : 0;
+        s[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int64_t a = t >= o ? s[t - o] : 0;
+            __syncthreads();
+            s[t] += a;
+            __syncthreads();
+        }
+        if (i < B) qpre[i] = carry + s[t] - v;
+        __syncthreads();
+        if (t == 1023) carry += s[1023];
+        __syncthreads();
+    }
+    if (t == 0) qpre[B] = carry;
+}
+
+// Chunk sums, one wave per chunk of REDUCE_CHUNK members of one cluster of one
+// instance, in the order of reduce_chunks_kernel: d <= 32 uses reduce_groups(d)
+// interleaved partials added in group order; otherwise each feature is summed
+// over the members in order (64 features per pass).  perm holds global row ids
+// (row0 + local row) sorted by label within each instance.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __restrict__ I, int B,
+                                                           const int64_t* __restrict__ qpre,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int64_t* __restrict__ offsets,
+                                                           const int32_t* __restrict__ chunk_base,
+                                                           double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
+    const int64_t total = qpre[B];
+    for (int64_t q = gw; q < total; q += nw) {
+        const int ii = batch_find(qpre, B, q);
+        const batch_inst in = I[ii];
+        const int d = in.d, k = in.k;
+        const int32_t lq = int32_t(q - qpre[ii]);
+        const int64_t* off = offsets + in.k0 + ii;
+        const int32_t* cb = chunk_base + in.k0 + ii;
+        int lo = 0, hi = k;  // cb[k] = chunks > lq
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cb[mid] <= lq) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const int64_t m_begin = off[j] + int64_t(lq - cb[j]) * REDUCE_CHUNK;
+        const int64_t m_end = min(m_begin + REDUCE_CHUNK, off[j + 1]);
+        const int32_t* pm = perm + in.row0;
+        double* out = part + in.p0 + int64_t(lq) * d;
+        if (d <= 32) {
+            const int G = reduce_groups(d);
+            const int g = lane / d, f0 = lane % d;
+            const bool active = g < G;
+            double acc = 0.0;
+            for (int64_t m = m_begin; m < m_end; m += G) {
+                const int64_t mm = m + g;
+                if (active && mm < m_end) {
+                    const int64_t row = int64_t(pm[mm]) - in.row0;
+                    acc = acc + double(in.X[row * d + f0]);
+                }
+            }
+            double tot = acc;
+            for (int gg = 1; gg < G; ++gg) {
+                const double o = __shfl(acc, gg * d + f0);
+                tot = tot + o;
+            }
+            if (g == 0) out[f0] = tot;
+        } else {
+            for (int fc = 0; fc < d; fc += 64) {
+                const int f = fc + lane;
+                double acc = 0.0;
+                int64_t m = m_begin;
+                for (; m + 8 <= m_end; m += 8) {
+                    int64_t r[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) r[u] = int64_t(pm[m + u]) - in.row0;
+                    if (f < d) {
+                        double v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) v[u] = double(in.X[r[u] * d + f]);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+                    }
+                }
+                for (; m < m_end; ++m) {
+                    const int64_t r1 = int64_t(pm[m]) - in.row0;
+                    if (f < d) acc = acc + double(in.X[r1 * d + f]);
+                }
+                if (f < d) out[f] = acc;
+            }
+        }
+    }
+}
+
+// C_next[j][f] = (float)(sum of the cluster's chunk partials in order / count);
+// empty clusters are left for batch_fill_empty_kernel.  One thread per centroid
+// element (prefix table over k * d).
+__global__ __launch_bounds__(256) void batch_finalize_kernel(const batch_inst* __restrict__ I, int B,
+                                                             const int64_t* __restrict__ pre,
+                                                             const int32_t* __restrict__ counts,
+                                                             const int32_t* __restrict__ chunk_base,
+                                                             const double* __restrict__ part,
+                                                             float* __restrict__ Cnext) {
+    const int64_t total = pre[B];
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int ii = batch_find(pre, B, e);
+        const batch_inst in = I[ii];
+        const int64_t le = e - pre[ii];
+        const int j = int(le / in.d), f = int(le - int64_t(j) * in.d);
+        const int32_t cnt = counts[in.k0 + j];
+        if (cnt == 0) continue;
+        const int32_t* cb = chunk_base + in.k0 + ii;
+        const double* pp = part + in.p0;
+        double s = 0.0;
+        int q = cb[j];
+        const int q1 = cb[j + 1];
+        for (; q + 8 <= q1; q += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = pp[int64_t(q + u) * in.d + f];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s = s + v[u];
+        }
+        for (; q < q1; ++q) s = s + pp[int64_t(q) * in.d + f];
+        Cnext[in.c0 + le] = float(s / double(cnt));
+    }
+}
+
+// exact per-row d64 = sum_f ((double)x_f - (double)c_f)^2 (features in order)
+__device__ __forceinline__ double batch_row_d64(const float* __restrict__ x,
+                                                const float* __restrict__ c, int d) {
+    double acc = 0.0;
+    for (int f = 0; f < d; ++f) {
+        const double diff = double(x[f]) - double(c[f]);
+        const double sq = diff * diff;
+        acc = acc + sq;
+    }
+    return acc;
+}
+
+// d64 of every row of the instances that have empty clusters (meta[2i] > 0);
+// BATCH_D64_ROWS rows per workgroup, one row per thread
+__global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __restrict__ I, int B,
+                                                        const int64_t* __restrict__ pre,
+                                                        const int32_t* __restrict__ meta,
+                                                        const float* __restrict__ C,
+                                                        const int32_t* __restrict__ labels,
+                                                        double* __restrict__ d64) {
+    const int64_t total = pre[B];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int ii = batch_find(pre, B, b);
+        if (meta[2 * ii] == 0) continue;
+        const batch_inst in = I[ii];
+        const int64_t r = (b - pre[ii]) * BATCH_D64_ROWS + threadIdx.x;
+        if (r >= in.n) continue;
+        const int lab = labels[in.row0 + r];
+        const float* c = C + in.c0 + int64_t(lab >= 0 && lab < in.k ? lab : 0) * in.d;
+        d64[in.row0 + r] = batch_row_d64(in.X + r * in.d, c, in.d);
+    }
+}
+
+// Empty clusters of one instance per workgroup: pick e = 0 .. m-1 takes the row
+// with the largest d64 (ties: lowest row) after pick e-1 in that order, and the
+// e-th empty cluster (ascending index) receives that row -- the policy of
+// empty_cluster_handling_dpc.cpp:26-114 with its tie order pinned.
+__global__ __launch_bounds__(256) void batch_fill_empty_kernel(const batch_inst* __restrict__ I, int B,
+                                                               const int32_t* __restrict__ meta,
+                                                               const int32_t* __restrict__ empty_list,
+                                                               const double* __restrict__ d64,
+                                                               float* __restrict__ Cnext) {
+    __shared__ double sd[256];
+    __shared__ int64_t sr[256];
+    const int t = threadIdx.x;
+    for (int ii = blockIdx.x; ii < B; ii += gridDim.x) {
+        const int m = meta[2 * ii];
+        if (m == 0) continue;
+        const batch_inst in = I[ii];
+        const double* dd = d64 + in.row0;
+        double pd = __builtin_inf();
+        int64_t pr = -1;
+        for (int e = 0; e < m; ++e) {
+            double bd = -1.0;
+            int64_t br = -1;
+            for (int64_t r = t; r < in.n; r += 256) {
+                const double v = dd[r];
+                const bool after = v < pd || (v == pd && r > pr);
+                if (after && (br < 0 || v > bd)) {  // rows ascend per thread: first max wins
+                    bd = v;
+                    br = r;
+                }
+            }
+            sd[t] = bd;
+            sr[t] = br;
+            __syncthreads();
+            for (int s = 128; s > 0; s >>= 1) {
+                if (t < s) {
+                    const double od = sd[t + s];
+                    const int64_t orr = sr[t + s];
+                    if (orr >= 0 && (sr[t] < 0 || od > sd[t] || (od == sd[t] && orr < sr[t]))) {
+                        sd[t] = od;
+                        sr[t] = orr;
+                    }
+                }
+                __syncthreads();
+            }
+            const double wd = sd[0];
+            const int64_t wr = sr[0];
+            __syncthreads();
+            if (wr >= 0) {
+                const int j = empty_list[in.k0 + e];
+                for (int f = t; f < in.d; f += 256)
+                    Cnext[in.c0 + int64_t(j) * in.d + f] = in.X[wr * in.d + f];
+            }
+            pd = wd;
+            pr = wr;
+        }
+    }
+}
+
+// Objective chunk sums: OBJ_CHUNK rows of one instance per workgroup, in the
+// order of objective_chunks_kernel (thread t sums rows t, t+256, ...; tree of
+// the 256 partials).
+__global__ __launch_bounds__(256) void batch_objective_kernel(const batch_inst* __restrict__ I, int B,
+                                                              const int64_t* __restrict__ pre,
+                                                              const float* __restrict__ C,
+                                                              double* __restrict__ chunk_sums) {
+    __shared__ double part[256];
+    const int t = threadIdx.x;
+    const int64_t total = pre[B];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int ii = batch_find(pre, B, b);
+        const batch_inst in = I[ii];
+        const int64_t lq = b - pre[ii];
+        const int64_t r0 = lq * OBJ_CHUNK;
+        const int rows = int(min<int64_t>(OBJ_CHUNK, in.n - r0));
+        double p = 0.0;
+        for (int r = t; r < rows; r += 256) {
+            const int64_t row = r0 + r;
+            const int lab = in.labels_out[row];
+            const float* c = C + in.c0 + int64_t(lab >= 0 && lab < in.k ? lab : 0) * in.d;
+            p = p + batch_row_d64(in.X + row * in.d, c, in.d);
+        }
+        __syncthreads();
+        part[t] = p;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) part[t] = part[t] + part[t + s];
+            __syncthreads();
+        }
+        if (t == 0) chunk_sums[in.o0 + lq] = part[0];
+    }
+}
+
+// per instance: objective = chunk sums added in chunk order (sum_sequential_kernel)
+__global__ void batch_objective_sum_kernel(const batch_inst* __restrict__ I, int B,
+                                           const double* __restrict__ chunk_sums,
+                                           double* __restrict__ out) {
+    for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < B; ii += gridDim.x * blockDim.x) {
+        const batch_inst in = I[ii];
+        const int64_t count = (in.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        const double* v = chunk_sums + in.o0;
+        double s = 0.0;
+        int64_t i = 0;
+        for (; i + 16 <= count; i += 16) {
+            double w[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) w[u] = v[i + u];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s = s + w[u];
+        }
+        for (; i < count; ++i) s = s + v[i];
+        out[ii] = s;
+    }
+}
+
+// centroid copies over the k * d elements of every instance (prefix table):
+// dir 0: C0 -> C (start), dir 1: C -> C_out (result)
+__global__ __launch_bounds__(256) void batch_centroid_copy_kernel(const batch_inst* __restrict__ I,
+                                                                  int B,
+                                                                  const int64_t* __restrict__ pre,
+                                                                  float* __restrict__ C, int dir) {
+    const int64_t total = pre[B];
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int ii = batch_find(pre, B, e);
+        const batch_inst in = I[ii];
+        const int64_t le = e - pre[ii];
+        if (dir == 0)
+            C[in.c0 + le] = in.C0[le];
+        else
+            in.C_out[le] = C[in.c0 + le];
+    }
+}
+
+template <typename T>
+__global__ void copy_kernel(const T* __restrict__ src, T* __restrict__ dst, int64_t count) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace km

@@ -16,6 +16,7 @@
 
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
@@ -32,6 +33,7 @@
 #include "../../include/onedal_amd/kmeans_hip.h"
 #include "km_assign.hpp"
 #include "km_assign_bf16.hpp"
+#include "km_batch.hpp"
 #include "km_common.hpp"
 #include "km_refine.hpp"
 #include "km_update.hpp"
@@ -101,6 +103,9 @@ struct km_ctx {
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
         c_out_dev;
+    // batched training
+    km::dbuf b_inst, b_pre, b_labels, b_keys, b_perm, b_d64, b_ccur, b_cnext, b_counts, b_offsets,
+        b_cb, b_empty, b_meta, b_qpre, b_part, b_objc, b_obj, b_segoff, b_sort_tmp;
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -797,6 +802,205 @@ static void check_infer_args(const void* X, int64_t n, int64_t d, const void* C,
     if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
 }
 
+// ---------------------------------------------------------------------------
+// Batched training (km_batch_train_f32).  Instance tables and prefix tables are
+// built on the host; each iteration is a fixed sequence of launches over all
+// instances (optionally captured once as a HIP graph and replayed).
+// ---------------------------------------------------------------------------
+static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
+                        int64_t max_iter, uint32_t flags, double* obj_out, int64_t* iters_out) {
+    if (!inst || count <= 0) throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
+    if (count > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Instance count is greater than max int32 value");
+    if (max_iter < 0)
+        throw status_error(KM_DOMAIN_ERROR, "Max iteration count should be >= 0");
+    if (!obj_out || !iters_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    const int B = int(count);
+    std::vector<batch_inst> hi(B);
+    // prefix tables: assignment batches, histogram chunks, centroid elements,
+    // d64 batches, objective chunks
+    enum { P_ASG = 0, P_CNT, P_EL, P_D64, P_OBJ, P_NUM };
+    std::vector<int64_t> pre(size_t(P_NUM) * (B + 1), 0);
+    auto P = [&](int t, int i) -> int64_t& { return pre[size_t(t) * (B + 1) + i]; };
+    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
+    int maxk = 1;
+    for (int i = 0; i < B; ++i) {
+        const km_batch_instance_f32& q = inst[i];
+        if (!q.X || !q.C0 || !q.C_out || !q.labels_out)
+            throw status_error(KM_INVALID_ARGUMENT, "Batch instance " + std::to_string(i) +
+                                                        ": input and output buffers are required");
+        if (q.n <= 0 || q.d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
+        if (q.k <= 0 || q.k > std::numeric_limits<int32_t>::max())
+            throw status_error(KM_DOMAIN_ERROR, "Cluster count should be > 0 and <= max int32 value");
+        if (q.k > q.n)
+            throw status_error(KM_INVALID_ARGUMENT,
+                               "Cluster count should be less than or equal to the row count");
+        if (q.d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
+        batch_inst& b = hi[i];
+        b.X = q.X;
+        b.C0 = q.C0;
+        b.C_out = q.C_out;
+        b.labels_out = q.labels_out;
+        b.n = q.n;
+        b.d = int32_t(q.d);
+        b.k = int32_t(q.k);
+        b.index = i;
+        b.vec = (q.d % 4 == 0 && reinterpret_cast<uintptr_t>(q.X) % 16 == 0) ? 1 : 0;
+        b.row0 = rows;
+        b.c0 = cfl;
+        b.k0 = kk;
+        b.p0 = pp;
+        b.o0 = oo;
+        rows += q.n;
+        cfl += (q.k * q.d + 3) / 4 * 4;
+        kk += q.k;
+        pp += (q.n / REDUCE_CHUNK + q.k + 1) * q.d;
+        oo += (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        maxk = std::max<int>(maxk, int(q.k));
+        P(P_ASG, i + 1) = P(P_ASG, i) + (q.n + BATCH_RB - 1) / BATCH_RB;
+        P(P_CNT, i + 1) = P(P_CNT, i) + (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
+        P(P_EL, i + 1) = P(P_EL, i) + q.k * q.d;
+        P(P_D64, i + 1) = P(P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
+        P(P_OBJ, i + 1) = P(P_OBJ, i) + (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+    }
+    if (rows >= (int64_t(1) << 31))
+        throw status_error(KM_UNIMPLEMENTED, "a batch holds at most 2^31 - 1 rows in total");
+    ctx->stats = km_stats{};
+
+    // device tables and workspace
+    batch_inst* dI = ctx->b_inst.get<batch_inst>(B);
+    int64_t* dpre = ctx->b_pre.get<int64_t>(pre.size());
+    auto DP = [&](int t) { return dpre + size_t(t) * (B + 1); };
+    int32_t* labels = ctx->b_labels.get<int32_t>(rows);
+    int32_t* keys = ctx->b_keys.get<int32_t>(rows);
+    int32_t* perm = ctx->b_perm.get<int32_t>(rows);
+    double* d64 = ctx->b_d64.get<double>(rows);
+    float* Ccur = ctx->b_ccur.get<float>(cfl);
+    float* Cnext = ctx->b_cnext.get<float>(cfl);
+    int32_t* counts = ctx->b_counts.get<int32_t>(kk);
+    int64_t* offsets = ctx->b_offsets.get<int64_t>(kk + B);
+    int32_t* cb = ctx->b_cb.get<int32_t>(kk + B);
+    int32_t* empty = ctx->b_empty.get<int32_t>(kk);
+    int32_t* meta = ctx->b_meta.get<int32_t>(2 * int64_t(B));
+    int64_t* qpre = ctx->b_qpre.get<int64_t>(B + 1);
+    double* part = ctx->b_part.get<double>(pp);
+    double* objc = ctx->b_objc.get<double>(oo);
+    double* objd = ctx->b_obj.get<double>(B);
+    unsigned int* segoff = ctx->b_segoff.get<unsigned int>(B + 1);
+    std::vector<unsigned int> hseg(B + 1);
+    for (int i = 0; i < B; ++i) hseg[i] = unsigned(hi[i].row0);
+    hseg[B] = unsigned(rows);
+    unsigned bits = 1;
+    while ((1ull << bits) < (unsigned long long)maxk) bits++;
+    size_t tmp_bytes = 0;
+    rocprim::counting_iterator<int32_t> ids(0);
+    KM_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, labels, keys, ids, perm,
+                                               unsigned(rows), unsigned(B), segoff, segoff + 1, 0,
+                                               bits, ctx->stream));
+    void* tmp = ctx->b_sort_tmp.get<char>(tmp_bytes);
+    KM_HIP(hipMemcpyAsync(dI, hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
+    KM_HIP(hipMemcpyAsync(dpre, pre.data(), sizeof(int64_t) * pre.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    KM_HIP(hipMemcpyAsync(segoff, hseg.data(), sizeof(unsigned int) * (B + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));  // the host tables go out of scope below
+
+    const int cap = ctx->cu_count * 8;
+    auto g = [&](int64_t items) { return int(std::max<int64_t>(1, std::min<int64_t>(items, cap))); };
+    hipLaunchKernelGGL(batch_centroid_copy_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
+                       ctx->stream, dI, B, DP(P_EL), Ccur, 0);
+    KM_LAUNCH_CHECK();
+
+    auto iteration = [&]() {
+        hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI,
+                           B, DP(P_ASG), Ccur, labels, 0);
+        KM_LAUNCH_CHECK();
+        fill<int32_t>(ctx, counts, kk, 0);
+        hipLaunchKernelGGL(batch_count_kernel, dim3(g(P(P_CNT, B))), dim3(256), 0, ctx->stream, dI, B,
+                           DP(P_CNT), labels, counts);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, dI, B, counts,
+                           offsets, cb, empty, meta);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, meta, B, qpre);
+        KM_LAUNCH_CHECK();
+        KM_HIP(rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, labels, keys, ids, perm,
+                                                   unsigned(rows), unsigned(B), segoff, segoff + 1,
+                                                   0, bits, ctx->stream));
+        hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, dI, B, qpre, perm,
+                           offsets, cb, part);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_finalize_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dI, B, DP(P_EL), counts, cb, part, Cnext);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_d64_kernel, dim3(g(P(P_D64, B))), dim3(256), 0, ctx->stream, dI, B,
+                           DP(P_D64), meta, Ccur, labels, d64);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_fill_empty_kernel, dim3(g(B)), dim3(256), 0, ctx->stream, dI, B, meta,
+                           empty, d64, Cnext);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(copy_kernel<float>, dim3(g((cfl + 255) / 256)), dim3(256), 0, ctx->stream,
+                           Cnext, Ccur, cfl);
+        KM_LAUNCH_CHECK();
+    };
+
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        ctx->ev_used = 0;
+        e0 = ctx->event();
+        e1 = ctx->event();
+        KM_HIP(hipEventRecord(e0, ctx->stream));
+    }
+    if ((flags & KM_FLAG_GRAPH) && max_iter > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        KM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            iteration();
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        KM_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            KM_HIP(ie);
+        }
+        hipError_t le = hipSuccess;
+        for (int64_t it = 0; it < max_iter && le == hipSuccess; ++it) le = hipGraphLaunch(exec, ctx->stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        KM_HIP(le);
+    } else {
+        for (int64_t it = 0; it < max_iter; ++it) iteration();
+    }
+    // final assignment (into the callers' label buffers), objective, centroids out
+    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI, B,
+                       DP(P_ASG), Ccur, labels, 1);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_objective_kernel, dim3(g(P(P_OBJ, B))), dim3(256), 0, ctx->stream, dI, B,
+                       DP(P_OBJ), Ccur, objc);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_objective_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, dI,
+                       B, objc, objd);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_centroid_copy_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
+                       ctx->stream, dI, B, DP(P_EL), Ccur, 1);
+    KM_LAUNCH_CHECK();
+    if (ctx->timing) KM_HIP(hipEventRecord(e1, ctx->stream));
+    KM_HIP(hipMemcpyAsync(obj_out, objd, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < B; ++i) iters_out[i] = max_iter;
+    if (ctx->timing) {
+        float ms = 0.f;
+        KM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ctx->stats.iterations_ms = ms;
+    }
+}
+
 template <typename Fn>
 static km_status guarded(km_ctx* ctx, Fn fn) {
     if (!ctx) return KM_INVALID_ARGUMENT;
@@ -868,7 +1072,10 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
                         &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
-                        &ctx->seg_counts};
+                        &ctx->seg_counts, &ctx->b_inst, &ctx->b_pre, &ctx->b_labels, &ctx->b_keys,
+                        &ctx->b_perm, &ctx->b_d64, &ctx->b_ccur, &ctx->b_cnext, &ctx->b_counts,
+                        &ctx->b_offsets, &ctx->b_cb, &ctx->b_empty, &ctx->b_meta, &ctx->b_qpre,
+                        &ctx->b_part, &ctx->b_objc, &ctx->b_obj, &ctx->b_segoff, &ctx->b_sort_tmp};
     for (auto* b : bufs) b->release();
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -935,6 +1142,14 @@ km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d,
     return km::guarded(ctx, [&] {
         km::check_infer_args(X, n, d, C, k, flags, labels_out, obj_out);
         km::infer<double>(ctx, X, n, d, C, k, flags, labels_out, obj_out);
+    });
+}
+
+km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances, int64_t count,
+                             int64_t max_iter, uint32_t flags, double* obj_out,
+                             int64_t* iters_out) {
+    return km::guarded(ctx, [&] {
+        km::batch_train(ctx, instances, count, max_iter, flags, obj_out, iters_out);
     });
 }
 

@@ -23,6 +23,7 @@ KM_INTERNAL_ERROR = 6
 KM_FLAG_EXACT_OBJECTIVE = 0x1
 KM_FLAG_BF16_REFINE = 0x2
 KM_FLAG_DEVICE_POINTERS = 0x4
+KM_FLAG_GRAPH = 0x8
 
 KM_OPT_ASSIGN_PATH = 1
 KM_OPT_TIMING = 2
@@ -32,8 +33,15 @@ PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64",
+    "km_lloyd_infer_f64", "km_batch_train_f32",
 ]
+
+
+class KmBatchInstance(ctypes.Structure):
+    """km_batch_instance_f32 (include/onedal_amd/kmeans_hip.h)."""
+    _fields_ = [("X", ctypes.c_void_p), ("C0", ctypes.c_void_p), ("C_out", ctypes.c_void_p),
+                ("labels_out", ctypes.c_void_p), ("n", ctypes.c_int64), ("d", ctypes.c_int64),
+                ("k", ctypes.c_int64)]
 
 
 class KmStats(ctypes.Structure):
@@ -86,6 +94,14 @@ def load_library():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libkmeans_hip.so is not built (%s); run python -m onedal_amd.build"
                            % LIB_PATH)
+    # PyTorch-ROCm bundles its own HIP runtime under the same soname as the one the
+    # engine links.  Loaded first, the engine would map the system runtime and torch
+    # would later map its own copy, which then finds no device.  Importing torch
+    # first (when it is installed) leaves one HIP runtime in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P, I64, D, U32, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32, ctypes.c_int
     L.km_context_create.argtypes = [I, ctypes.POINTER(P)]
@@ -105,6 +121,9 @@ def load_library():
         f = getattr(L, "km_lloyd_infer_" + sfx)
         f.argtypes = [P, P, I64, I64, P, I64, U32, P, P]
         f.restype = I
+    L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
+                                     ctypes.POINTER(D), ctypes.POINTER(I64)]
+    L.km_batch_train_f32.restype = I
     _lib = L
     return L
 
@@ -234,6 +253,35 @@ class Engine:
             ctypes.byref(obj), ctypes.byref(iters))
         self._check(st)
         return c, labels, obj.value, iters.value
+
+    def train_batch_device(self, xs, c0s, max_iter, graph=False, outs=None):
+        """Batched training of independent instances: xs[i] (n_i x d_i) and c0s[i]
+        (k_i x d_i) contiguous float32 CUDA tensors.  Returns lists of device
+        centroids and labels and host lists of objectives and iteration counts."""
+        import torch
+
+        count = len(xs)
+        arr = (KmBatchInstance * count)()
+        cs, ls = [], []
+        for i, (x, c0) in enumerate(zip(xs, c0s)):
+            assert x.is_contiguous() and c0.is_contiguous() and x.dtype == torch.float32
+            n, d = x.shape
+            k = c0.shape[0]
+            if outs is None:
+                c = torch.empty((k, d), dtype=torch.float32, device=x.device)
+                lab = torch.empty(n, dtype=torch.int32, device=x.device)
+            else:
+                c, lab = outs[0][i], outs[1][i]
+            cs.append(c)
+            ls.append(lab)
+            arr[i] = KmBatchInstance(x.data_ptr(), c0.data_ptr(), c.data_ptr(), lab.data_ptr(),
+                                     n, d, k)
+        obj = (ctypes.c_double * count)()
+        its = (ctypes.c_int64 * count)()
+        flags = KM_FLAG_DEVICE_POINTERS | (KM_FLAG_GRAPH if graph else 0)
+        st = self.lib.km_batch_train_f32(self.ctx, arr, count, int(max_iter), flags, obj, its)
+        self._check(st)
+        return cs, ls, list(obj), list(its)
 
     def infer_device(self, x, c, objective=False, bf16=False):
         import torch

@@ -1,0 +1,108 @@
+"""Batched training (km_batch_train_f32) on the GPU: every instance of a batch must
+equal the oracle's canonical Lloyd on that instance alone, bit for bit (labels,
+centroid bytes, objective), in eager and graph-replay mode."""
+
+import numpy as np
+import pytest
+
+from conftest import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(seed, count):
+    rng = np.random.default_rng(seed)
+    dims = [1, 2, 3, 5, 8, 13, 16, 17, 32, 33, 48, 64, 100, 128]
+    inst = []
+    for i in range(count):
+        d = dims[i % len(dims)]
+        n = int(rng.integers(300, 9000))
+        k = int(rng.integers(1, 131))
+        k = min(k, n)
+        x = gaussian_mixture(n, d, max(1, k // 2 + 1), seed=seed * 1000 + i, center_range=10.0,
+                             sigma=(0.5, 2.0))
+        if i % 5 == 0:
+            # initial centroids with repeats: clusters that start empty
+            idx = rng.choice(n, size=max(1, k // 2), replace=False)
+            c0 = x[np.resize(idx, k)].copy()
+        else:
+            c0 = x[rng.choice(n, size=k, replace=False)].copy()
+        inst.append((x, c0))
+    # edge shapes: one cluster, n == k, a single column
+    x = gaussian_mixture(500, 7, 3, seed=seed + 1)
+    inst.append((x, x[:1].copy()))
+    x = gaussian_mixture(64, 5, 4, seed=seed + 2)
+    inst.append((x, x.copy()))
+    x = gaussian_mixture(2000, 1, 4, seed=seed + 3)
+    inst.append((x, x[:9].copy()))
+    return inst
+
+
+def run_batch(engine, inst, iters, graph):
+    import torch
+
+    xs = [torch.from_numpy(x).cuda() for x, _ in inst]
+    cs = [torch.from_numpy(c).cuda() for _, c in inst]
+    c_out, l_out, obj, its = engine.train_batch_device(xs, cs, iters, graph=graph)
+    torch.cuda.synchronize()
+    return ([c.cpu().numpy() for c in c_out], [l.cpu().numpy() for l in l_out], obj, its)
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_batch_bit_exact_vs_canonical(engine, oracle, graph):
+    iters = 6
+    inst = make_batch(11, 40)
+    c_out, l_out, obj, its = run_batch(engine, inst, iters, graph)
+    for i, (x, c0) in enumerate(inst):
+        wc, wl, wo, wi = oracle.canonical_lloyd(x, c0, iters, 0.0)[:4]
+        assert its[i] == wi == iters
+        bad = np.flatnonzero(l_out[i] != wl)
+        assert bad.size == 0, "instance %d (n=%d d=%d k=%d): labels differ at %s" % (
+            i, x.shape[0], x.shape[1], c0.shape[0], bad[:10])
+        assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), "instance %d centroids" % i
+        assert obj[i] == wo, (i, obj[i], wo)
+
+
+def test_batch_graph_replays_match_eager(engine):
+    """The captured iteration replayed max_iter times gives the eager result, and a
+    second call (fresh capture) gives it again."""
+    inst = make_batch(23, 24)
+    a = run_batch(engine, inst, 5, graph=False)
+    b = run_batch(engine, inst, 5, graph=True)
+    c = run_batch(engine, inst, 5, graph=True)
+    for i in range(len(inst)):
+        assert np.array_equal(a[1][i], b[1][i]) and np.array_equal(a[1][i], c[1][i])
+        assert np.array_equal(a[0][i].view(np.uint8), b[0][i].view(np.uint8))
+        assert np.array_equal(a[0][i].view(np.uint8), c[0][i].view(np.uint8))
+        assert a[2][i] == b[2][i] == c[2][i]
+
+
+def test_batch_matches_single_instance_engine(engine):
+    """Batched and one-instance-per-call training agree (zero iterations included)."""
+    import torch
+
+    inst = make_batch(5, 12)
+    for iters in (0, 3):
+        c_out, l_out, obj, its = run_batch(engine, inst, iters, graph=False)
+        for i, (x, c0) in enumerate(inst):
+            c, lab, o, it = engine.train(x, c0, iters, 0.0)
+            assert np.array_equal(lab, l_out[i]), i
+            assert np.array_equal(c.view(np.uint8), c_out[i].view(np.uint8)), i
+            assert o == obj[i] and it == its[i]
+    torch.cuda.synchronize()
+
+
+def test_batch_badargs(engine):
+    import torch
+
+    from onedal_amd.engine import DomainError, InvalidArgument
+
+    x = torch.zeros((10, 3), dtype=torch.float32, device="cuda")
+    c0 = torch.zeros((4, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(InvalidArgument):
+        engine.train_batch_device([x[:3]], [c0], 2)  # k > n
+    with pytest.raises(DomainError):
+        engine.train_batch_device([x], [c0], -1)
+    # usable afterwards
+    c, l, o, it = engine.train_batch_device([x], [c0], 1)
+    assert it == [1]
