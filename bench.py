@@ -9,6 +9,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fp32_c2]
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -37,6 +38,13 @@ WORKLOADS = {
     "fp32_c2": dict(n=10_000_000, d=64, k=256, iters=20, center_range=20.0, sigma=(0.5, 2.0),
                     anisotropic=False, bf16=False),
 }
+
+# Batched form: independent instances, n log-uniform in [1e3, 1e6], d from
+# {8, 16, 32, 64, 128}, k uniform in [4, 128], k Gaussians per instance (centers
+# U[-10,10]^d, sigma 1), random distinct rows as initial centroids, 30 iterations,
+# all instances in one batched call (each iteration replayed as a HIP graph).
+BATCHED = {"batched_c4": dict(count=4096, n_lo=1e3, n_hi=1e6, dims=(8, 16, 32, 64, 128), k_lo=4,
+                              k_hi=128, iters=30)}
 
 
 def log(*a):
@@ -91,12 +99,114 @@ def cpu_baseline(x_host, c0_host, iters, budget_s=15.0):
                       "restatement (float32, %d threads, %.1f s)" % (sample, cores, dt)}
 
 
+def make_batch(torch, wl, count, seed, device):
+    """Instances of the batched workload as lists of device tensors (x_i, c0_i)."""
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    gd = torch.Generator(device=device)
+    gd.manual_seed(seed)
+    xs, cs = [], []
+    lo, hi = math.log(wl["n_lo"]), math.log(wl["n_hi"])
+    for i in range(count):
+        n = int(math.exp(lo + (hi - lo) * float(torch.rand(1, generator=g))))
+        d = wl["dims"][int(torch.randint(0, len(wl["dims"]), (1,), generator=g))]
+        k = int(torch.randint(wl["k_lo"], wl["k_hi"] + 1, (1,), generator=g))
+        centers = (torch.rand((k, d), generator=gd, device=device) * 2 - 1) * 10.0
+        lab = torch.randint(0, k, (n,), generator=gd, device=device)
+        x = centers[lab] + torch.randn((n, d), generator=gd, device=device)
+        perm = torch.randperm(n, generator=gd, device=device)[:k]
+        xs.append(x.contiguous())
+        cs.append(x[perm].contiguous())
+    return xs, cs
+
+
+def run_batched(torch, args, wl):
+    from onedal_amd.engine import Engine, KM_OPT_TIMING
+
+    count = args.instances or wl["count"]
+    iters = wl["iters"]
+    dev = torch.device("cuda:0")
+    log("generating %s: %d instances" % (args.workload, count))
+    xs, cs = make_batch(torch, wl, count, 0, dev)
+    torch.cuda.synchronize()
+    rows = sum(x.shape[0] for x in xs)
+    flops_pit = sum(2.0 * x.shape[0] * c.shape[0] * x.shape[1] for x, c in zip(xs, cs))  # per iteration
+    log("rows %d, X %.1f GB" % (rows, sum(x.numel() for x in xs) * 4 / 1e9))
+    eng = Engine(0)
+    eng.set_option(KM_OPT_TIMING, 1)
+    outs = ([torch.empty_like(c) for c in cs], [torch.empty(x.shape[0], dtype=torch.int32, device=dev)
+                                                for x in xs])
+    for w in range(args.warmup):
+        eng.train_batch_device(xs, cs, iters, graph=True, outs=outs)
+        log("warmup %d done" % w)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    loop_ms = []
+    for s in range(args.steps):
+        _, _, obj, its = eng.train_batch_device(xs, cs, iters, graph=True, outs=outs)
+        torch.cuda.synchronize()
+        loop_ms.append(eng.stats()["iterations_ms"])
+        log("step %d: loop %.1f ms" % (s, loop_ms[-1]))
+    elapsed = time.perf_counter() - t_all
+    pit = rows * iters * args.steps / elapsed
+    achieved = flops_pit * iters * args.steps / elapsed / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        from oracle import oracle as O
+
+        try:
+            cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        except AttributeError:
+            cores = 16
+        t0 = time.perf_counter()
+        done = 0
+        used = 0
+        for x, c in zip(xs, cs):
+            xh, ch = x.cpu().numpy(), c.cpu().numpy()
+            O.daal_lloyd(xh, ch, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+            done += xh.shape[0]
+            used += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": done / dt, "unit": "point-iterations/s", "cores": cores, "kind": "port",
+               "sample": "first %d instances of the batch (%d rows), 1 Lloyd iteration each, oracle "
+                         "DAAL-Lloyd restatement (float32, %d threads, %.1f s incl. copies)" % (
+                             used, done, cores, dt)}
+    out = {
+        "metric": METRIC,
+        "value": pit,
+        "unit": "point-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "instances": count, "rows": rows, "iterations": iters,
+                   "path": "batched exact-form assignment, HIP graph per iteration"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
+                     "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK["fp32_mfma_tflops"], 4), "traffic": None,
+                     "kernel": "whole batched iteration (2kd flops per point-iteration)"},
+        "cpu_baseline": cpu,
+        "loop_ms_per_step": sum(loop_ms) / len(loop_ms),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="precision_c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="precision_c3", choices=sorted(WORKLOADS) + sorted(BATCHED))
+    ap.add_argument("--instances", type=int, default=None,
+                    help="batched workloads: number of instances (default: the workload's)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP32-path and FP64-path comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -114,6 +224,8 @@ def main():
 
     from onedal_amd.engine import Engine, KM_OPT_TIMING
 
+    if args.workload in BATCHED:
+        return run_batched(torch, args, BATCHED[args.workload])
     wl = WORKLOADS[args.workload]
     n, d, k, iters = wl["n"], wl["d"], wl["k"], wl["iters"]
     dev = torch.device("cuda:0")

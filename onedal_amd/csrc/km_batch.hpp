@@ -8,7 +8,7 @@
 // the single-instance result bit for bit (same arithmetic order):
 //   exact-form assignment (features in order, ties to the lowest index)
 //   -> label histogram -> per-instance scan (member offsets, chunk bases, empty
-//   list) -> stable segmented sort of row ids by label -> chunk sums in FP64
+//   list) -> one stable radix sort of row ids by (instance, label) -> chunk sums in FP64
 //   (REDUCE_CHUNK members, reduce_groups interleave) -> merge/divide
 //   -> empty clusters refilled on the device with the farthest rows (largest d64,
 //   ties to the lowest row) in ascending cluster order.
@@ -69,12 +69,16 @@ __device__ __forceinline__ int batch_find(const int64_t* __restrict__ pre, int B
 // sub, mul and add rounded separately.  Scan semantics of
 // select_min_distance (kernels_fp_impl.hpp:198-202): strict < in centroid order,
 // a NaN distance to centroid 0 keeps label 0.
-// labels: concatenated labels (row0) or, with to_user, each instance's output.
+// labels: concatenated labels (row0) or, with to_user, each instance's output;
+// skeys (when labels are concatenated): the row's sort key (index << lbits) |
+// label, so one stable radix sort over the batch groups every instance's rows
+// by label in row order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __restrict__ I, int B,
                                                            const int64_t* __restrict__ pre,
                                                            const float* __restrict__ C,
                                                            int32_t* __restrict__ labels,
+                                                           uint32_t* __restrict__ skeys, int lbits,
                                                            int to_user) {
     constexpr int RW = BATCH_RW, RB = BATCH_RB;
     constexpr int FT = 256;  // staged features per row chunk
@@ -194,10 +198,12 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             const int rr = wave * RW + r;
             if (lane == 0 && rr < nr) {
                 const int lab = (n0 || bj == 0x7fffffff) ? 0 : bj;
-                if (to_user)
+                if (to_user) {
                     in.labels_out[r0 + rr] = lab;
-                else
+                } else {
                     labels[in.row0 + r0 + rr] = lab;
+                    skeys[in.row0 + r0 + rr] = (uint32_t(in.index) << lbits) | uint32_t(lab);
+                }
             }
         }
     }

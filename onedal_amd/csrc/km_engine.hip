@@ -16,7 +16,6 @@
 
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
@@ -105,7 +104,7 @@ struct km_ctx {
         c_out_dev;
     // batched training
     km::dbuf b_inst, b_pre, b_labels, b_keys, b_perm, b_d64, b_ccur, b_cnext, b_counts, b_offsets,
-        b_cb, b_empty, b_meta, b_qpre, b_part, b_objc, b_obj, b_segoff, b_sort_tmp;
+        b_cb, b_empty, b_meta, b_qpre, b_part, b_objc, b_obj, b_sort_tmp;
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -872,7 +871,7 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
     int64_t* dpre = ctx->b_pre.get<int64_t>(pre.size());
     auto DP = [&](int t) { return dpre + size_t(t) * (B + 1); };
     int32_t* labels = ctx->b_labels.get<int32_t>(rows);
-    int32_t* keys = ctx->b_keys.get<int32_t>(rows);
+    uint32_t* skeys = ctx->b_keys.get<uint32_t>(2 * rows);  // keys, then their sorted copy
     int32_t* perm = ctx->b_perm.get<int32_t>(rows);
     double* d64 = ctx->b_d64.get<double>(rows);
     float* Ccur = ctx->b_ccur.get<float>(cfl);
@@ -886,22 +885,23 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
     double* part = ctx->b_part.get<double>(pp);
     double* objc = ctx->b_objc.get<double>(oo);
     double* objd = ctx->b_obj.get<double>(B);
-    unsigned int* segoff = ctx->b_segoff.get<unsigned int>(B + 1);
-    std::vector<unsigned int> hseg(B + 1);
-    for (int i = 0; i < B; ++i) hseg[i] = unsigned(hi[i].row0);
-    hseg[B] = unsigned(rows);
-    unsigned bits = 1;
-    while ((1ull << bits) < (unsigned long long)maxk) bits++;
+    // rows are grouped by (instance, label) with one stable radix sort of the keys
+    // (index << lbits) | label; instances are contiguous, so each instance's rows
+    // come out by label in row order
+    unsigned lbits = 0, ibits = 0;
+    while ((1ull << lbits) < (unsigned long long)maxk) lbits++;
+    while ((1ull << ibits) < (unsigned long long)B) ibits++;
+    if (lbits + ibits > 32)
+        throw status_error(KM_UNIMPLEMENTED, "batch too large for 32-bit sort keys "
+                                             "(instances x max cluster count above 2^32)");
+    const unsigned kbits = std::max(1u, lbits + ibits);
     size_t tmp_bytes = 0;
     rocprim::counting_iterator<int32_t> ids(0);
-    KM_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, labels, keys, ids, perm,
-                                               unsigned(rows), unsigned(B), segoff, segoff + 1, 0,
-                                               bits, ctx->stream));
+    KM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, skeys, skeys + rows, ids, perm,
+                                     unsigned(rows), 0, kbits, ctx->stream));
     void* tmp = ctx->b_sort_tmp.get<char>(tmp_bytes);
     KM_HIP(hipMemcpyAsync(dI, hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
     KM_HIP(hipMemcpyAsync(dpre, pre.data(), sizeof(int64_t) * pre.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-    KM_HIP(hipMemcpyAsync(segoff, hseg.data(), sizeof(unsigned int) * (B + 1), hipMemcpyHostToDevice,
                           ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));  // the host tables go out of scope below
 
@@ -913,7 +913,7 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
 
     auto iteration = [&]() {
         hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI,
-                           B, DP(P_ASG), Ccur, labels, 0);
+                           B, DP(P_ASG), Ccur, labels, skeys, int(lbits), 0);
         KM_LAUNCH_CHECK();
         fill<int32_t>(ctx, counts, kk, 0);
         hipLaunchKernelGGL(batch_count_kernel, dim3(g(P(P_CNT, B))), dim3(256), 0, ctx->stream, dI, B,
@@ -924,9 +924,8 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         KM_LAUNCH_CHECK();
         hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, meta, B, qpre);
         KM_LAUNCH_CHECK();
-        KM_HIP(rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, labels, keys, ids, perm,
-                                                   unsigned(rows), unsigned(B), segoff, segoff + 1,
-                                                   0, bits, ctx->stream));
+        KM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, skeys, skeys + rows, ids, perm,
+                                         unsigned(rows), 0, kbits, ctx->stream));
         hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, dI, B, qpre, perm,
                            offsets, cb, part);
         KM_LAUNCH_CHECK();
@@ -979,7 +978,7 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
     }
     // final assignment (into the callers' label buffers), objective, centroids out
     hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI, B,
-                       DP(P_ASG), Ccur, labels, 1);
+                       DP(P_ASG), Ccur, labels, skeys, int(lbits), 1);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_objective_kernel, dim3(g(P(P_OBJ, B))), dim3(256), 0, ctx->stream, dI, B,
                        DP(P_OBJ), Ccur, objc);
@@ -1075,7 +1074,7 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->seg_counts, &ctx->b_inst, &ctx->b_pre, &ctx->b_labels, &ctx->b_keys,
                         &ctx->b_perm, &ctx->b_d64, &ctx->b_ccur, &ctx->b_cnext, &ctx->b_counts,
                         &ctx->b_offsets, &ctx->b_cb, &ctx->b_empty, &ctx->b_meta, &ctx->b_qpre,
-                        &ctx->b_part, &ctx->b_objc, &ctx->b_obj, &ctx->b_segoff, &ctx->b_sort_tmp};
+                        &ctx->b_part, &ctx->b_objc, &ctx->b_obj, &ctx->b_sort_tmp};
     for (auto* b : bufs) b->release();
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
