@@ -5,6 +5,10 @@ synthetic Gaussian-mixture matrix, through the C ABI with device pointers.
 value = point-iterations/s = n * iterations * steps / elapsed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fp32_c2]
+
+The batched workload (batched_c4) trains thousands of instances per call; the
+serving workload (serving_c5) sweeps offered request rates through the km_server
+request loop and reports throughput with p50/p99 job latency per rate.
 """
 
 import argparse
@@ -45,6 +49,18 @@ WORKLOADS = {
 # all instances in one batched call (each iteration replayed as a HIP graph).
 BATCHED = {"batched_c4": dict(count=4096, n_lo=1e3, n_hi=1e6, dims=(8, 16, 32, 64, 128), k_lo=4,
                               k_hi=128, iters=30)}
+
+# Serving form: a request loop with Poisson arrivals, swept over offered rates;
+# 70% fit jobs (20 Lloyd iterations) and 30% infer jobs (labels and objective
+# against resident centroids); job sizes as the batched form plus 5% large jobs
+# (n=5e6, d=128, k=512); float32 data, BF16+refine distances requested (the
+# large-job lane runs them; the batched lane's exact form gives the same labels);
+# admission by a 200 GB HBM budget.  Datasets are generated once into HBM and
+# each request names one of them.
+SERVING = {"serving_c5": dict(rates=(50, 100, 200, 400, 1000, 2000), window_s=2.0, min_jobs=200,
+                              max_jobs=4000, fit_frac=0.7, large_frac=0.05, iters=20, budget=200e9,
+                              pool_small=96, pool_large=2, large=(5_000_000, 128, 512), n_lo=1e3,
+                              n_hi=1e6, dims=(8, 16, 32, 64, 128), k_lo=4, k_hi=128)}
 
 
 def log(*a):
@@ -199,12 +215,168 @@ def run_batched(torch, args, wl):
     print(json.dumps(out), flush=True)
 
 
+def _pct(v, q):
+    if not v:
+        return None
+    v = sorted(v)
+    return v[min(len(v) - 1, int(math.ceil(q / 100.0 * len(v))) - 1)]
+
+
+def run_serving(torch, args, wl):
+    import queue
+    import random
+    import threading
+
+    from onedal_amd.engine import Server
+
+    dev = torch.device("cuda:0")
+    log("generating %s pools: %d small datasets, %d large" % (args.workload, wl["pool_small"],
+                                                              wl["pool_large"]))
+    xs, cs = make_batch(torch, wl, wl["pool_small"], 1, dev)
+    nl, dl, kl = wl["large"]
+    large = [make_data(torch, nl, dl, kl, 10.0, (1.0, 1.0), False, 100 + i, dev)
+             for i in range(wl["pool_large"])]
+    torch.cuda.synchronize()
+    log("pool X %.1f GB" % ((sum(x.numel() for x in xs) + sum(x.numel() for x, _ in large)) * 4 / 1e9))
+    iters = wl["iters"]
+    srv = Server(budget_bytes=int(wl["budget"]))
+
+    def run(rate, count, seed):
+        rng = random.Random(seed)
+        reqs = []
+        t = 0.0
+        for _ in range(count):
+            t += rng.expovariate(rate)
+            big = rng.random() < wl["large_frac"]
+            fit = rng.random() < wl["fit_frac"]
+            if big:
+                x, c = large[rng.randrange(len(large))]
+            else:
+                i = rng.randrange(len(xs))
+                x, c = xs[i], cs[i]
+            reqs.append((t, big, fit, x, c))
+        ids = queue.Queue()
+        done = []
+        t_last = [0.0]
+
+        def waiter():
+            while True:
+                item = ids.get()
+                if item is None:
+                    return
+                jid, big, fit, n, k, d = item
+                r = srv.wait(jid)
+                t_last[0] = time.perf_counter()
+                done.append((big, fit, n, k, d, r["latency_ms"], r["queue_ms"], r["lane"]))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        st0 = srv.stats()
+        t0 = time.perf_counter()
+        for (ta, big, fit, x, c) in reqs:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            if fit:
+                jid = srv.submit_fit(x, c, iters, bf16=True)
+            else:
+                jid = srv.submit_infer(x, c, bf16=True)
+            ids.put((jid, big, fit, x.shape[0], c.shape[0], x.shape[1]))
+        t_sub = time.perf_counter() - t0
+        ids.put(None)
+        th.join()
+        st1 = srv.stats()
+        el = t_last[0] - t0
+        pit = sum(n * (iters if fit else 1) for (_, fit, n, _, _, _, _, _) in done)
+        flops = sum(2.0 * n * k * d * (iters + 1 if fit else 1) for (_, fit, n, k, d, _, _, _) in done)
+        lat = [r[5] for r in done]
+        small_lat = [r[5] for r in done if not r[0]]
+        big_lat = [r[5] for r in done if r[0]]
+        steps = st1["steps"] - st0["steps"]
+        return {"offered_rate": rate, "jobs": count, "arrival_span_s": round(reqs[-1][0], 3),
+                "submit_span_s": round(t_sub, 3), "elapsed_s": round(el, 3),
+                "jobs_per_s": round(count / el, 1), "point_iters_per_s": pit / el,
+                "tflops": round(flops / el / 1e12, 2),
+                "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                "p50_small_ms": _pct(small_lat, 50), "p99_small_ms": _pct(small_lat, 99),
+                "p50_large_ms": _pct(big_lat, 50), "p99_large_ms": _pct(big_lat, 99),
+                "mean_queue_ms": sum(r[6] for r in done) / len(done),
+                "large_jobs": len(big_lat), "batched_steps": steps,
+                "mean_step_jobs": round((st1["step_jobs"] - st0["step_jobs"]) / max(1, steps), 1),
+                "max_step_jobs": st1["max_step_jobs"],
+                "peak_admitted_gb": round(st1["peak_bytes_admitted"] / 1e9, 2)}
+
+    log("warmup ...")
+    run(200.0, 60, 12345)
+    sweep = []
+    rates = [float(r) for r in (args.rates.split(",") if args.rates else wl["rates"])]
+    for r in rates:
+        count = int(min(wl["max_jobs"], max(wl["min_jobs"], r * wl["window_s"])))
+        res = run(r, count, int(r))
+        sweep.append(res)
+        log("rate %6.0f/s: %d jobs in %.2f s, %.0f jobs/s, %.3g pit/s, p50 %.1f ms, p99 %.1f ms, "
+            "mean step jobs %.1f" % (r, count, res["elapsed_s"], res["jobs_per_s"],
+                                      res["point_iters_per_s"], res["p50_ms"], res["p99_ms"],
+                                      res["mean_step_jobs"]))
+    srv.close()
+    best = max(sweep, key=lambda z: z["point_iters_per_s"])
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        from oracle import oracle as O
+
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        t0 = time.perf_counter()
+        done_rows = used = 0
+        for x, c in zip(xs, cs):
+            O.daal_lloyd(x.cpu().numpy(), c.cpu().numpy(), 1, 0.0, stop_rule=2, block_size=512,
+                         nthreads=cores)
+            done_rows += x.shape[0]
+            used += 1
+            if time.perf_counter() - t0 > 10.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": done_rows / dt, "unit": "point-iterations/s", "cores": cores, "kind": "port",
+               "sample": "first %d small datasets of the pool (%d rows), 1 Lloyd iteration each, "
+                         "oracle DAAL-Lloyd restatement (float32, %d threads, %.1f s incl. copies)" % (
+                             used, done_rows, cores, dt)}
+    out = {
+        "metric": METRIC,
+        "value": best["point_iters_per_s"],
+        "unit": "point-iterations/s",
+        "n_gpus": 1,
+        "steps": len(sweep),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (batched lane), bf16+f32 (large-job lane)",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "fit_fraction": wl["fit_frac"],
+                   "large_fraction": wl["large_frac"], "fit_iterations": iters,
+                   "budget_bytes": wl["budget"], "large_job": {"n": nl, "d": dl, "k": kl}},
+        "latency": {"p50_ms_at_best": best["p50_ms"], "p99_ms_at_best": best["p99_ms"],
+                    "rate_at_best": best["offered_rate"]},
+        "roofline": {"bound": "mfma", "achieved": best["tflops"], "peak": PEAK["fp32_mfma_tflops"],
+                     "unit": "TFLOP/s", "frac": round(best["tflops"] / PEAK["fp32_mfma_tflops"], 4),
+                     "traffic": None,
+                     "kernel": "whole request loop (2kd flops per point-iteration and final pass)"},
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="precision_c3", choices=sorted(WORKLOADS) + sorted(BATCHED))
+    ap.add_argument("--workload", default="precision_c3",
+                    choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING))
+    ap.add_argument("--rates", default=None,
+                    help="serving workloads: comma-separated offered rates in jobs/s")
     ap.add_argument("--instances", type=int, default=None,
                     help="batched workloads: number of instances (default: the workload's)")
     ap.add_argument("--no-compare", action="store_true",
@@ -226,6 +398,8 @@ def main():
 
     if args.workload in BATCHED:
         return run_batched(torch, args, BATCHED[args.workload])
+    if args.workload in SERVING:
+        return run_serving(torch, args, SERVING[args.workload])
     wl = WORKLOADS[args.workload]
     n, d, k, iters = wl["n"], wl["d"], wl["k"], wl["iters"]
     dev = torch.device("cuda:0")

@@ -42,7 +42,8 @@ typedef enum {
     KM_DEVICE_BAD_ALLOC = 3,  /* oneapi::dal::device_bad_alloc */
     KM_HOST_BAD_ALLOC = 4,    /* oneapi::dal::host_bad_alloc */
     KM_UNIMPLEMENTED = 5,     /* oneapi::dal::unimplemented */
-    KM_INTERNAL_ERROR = 6     /* oneapi::dal::internal_error (HIP runtime failure) */
+    KM_INTERNAL_ERROR = 6,    /* oneapi::dal::internal_error (HIP runtime failure) */
+    KM_PENDING = 7            /* km_server_wait: the job has not finished yet (not an error) */
 } km_status;
 
 /* call flags */
@@ -125,6 +126,92 @@ typedef struct {
 km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances, int64_t count,
                              int64_t max_iter, uint32_t flags, double* obj_out,
                              int64_t* iters_out);
+
+/* One step of a continuous batch, for schedulers that change the set of
+ * instances between iterations.  First `iterations` Lloyd iterations of every
+ * instance of `iterated` (centroids read from C0, written to C_out, which may
+ * alias C0; labels_out unused), then a final pass over every instance of
+ * `finished`: labels against C0 into labels_out, objective into obj_out[i]
+ * (host array of n_finished entries) and, where C_out is not NULL, the centroids
+ * into C_out.  The final pass runs after the iterations, so an instance may be in
+ * both sets, its final pass reading the centroids its last iteration wrote.
+ * Iterated instances follow the training preconditions, finished ones the
+ * inference preconditions; every instance's result equals the single-instance
+ * calls bit for bit.  All pointers except obj_out are device memory; flags must be
+ * 0; returns after the step has completed.  No counterpart in the reference: it is
+ * cluster_updater<F>::update (cluster_updater.hpp:82-179) applied to many
+ * instances at once, plus the final assignment of train_kernel_lloyd_dense_dpc.cpp:
+ * 169-197. */
+km_status km_batch_step_f32(km_ctx* ctx, const km_batch_instance_f32* iterated, int64_t n_iterated,
+                            int64_t iterations, const km_batch_instance_f32* finished,
+                            int64_t n_finished, uint32_t flags, double* obj_out);
+
+/* ---- Serving --------------------------------------------------------------
+ * A request loop over one device.  Fit jobs (fixed iteration count, accuracy
+ * threshold 0) and infer jobs are submitted from any thread and return a job id
+ * at once.  Jobs are admitted in submission order while the sum of the admitted
+ * jobs' HBM footprints stays within the budget; host-memory jobs are then staged
+ * into device memory the server owns.  Admitted jobs join the batched lane at its
+ * next iteration (continuous batching: each step iterates every active fit job,
+ * and jobs finish and leave between steps), except jobs with n*k*d at or above
+ * large_job_work, which run one at a time on a second stream through the
+ * single-instance engine (BF16 matrix cores when the job asks for
+ * KM_FLAG_BF16_REFINE), overlapping the batched lane on the device.  Every job's
+ * labels, centroids and objective equal km_lloyd_train_f32 / km_lloyd_infer_f32
+ * on that job alone, bit for bit, whatever else runs.  No counterpart in the
+ * reference, which serves one train()/infer() call at a time. */
+typedef struct km_server km_server;
+
+typedef struct {
+    int32_t device;           /* HIP device */
+    int32_t max_batch_jobs;   /* active jobs of the batched lane; 0: 4096, at most 65536 */
+    int64_t hbm_budget_bytes; /* admission budget over the admitted jobs' footprints */
+    int64_t large_job_work;   /* n*k*d from which a job takes the large-job lane;
+                                 0: 2^34, negative: never */
+} km_server_config;
+
+typedef struct {
+    int64_t iterations;  /* Lloyd iterations run (fit), 0 (infer) */
+    double objective;    /* sum of squared distances to the final centroids */
+    double queue_ms;     /* submission -> admission */
+    double latency_ms;   /* submission -> results in place */
+    int32_t lane;        /* 0 batched lane, 1 large-job lane */
+    int32_t reserved;
+} km_job_result;
+
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t queued;             /* jobs waiting for admission now */
+    int64_t steps;              /* batched-lane steps run */
+    int64_t step_jobs;          /* jobs summed over those steps */
+    int64_t max_step_jobs;      /* most jobs in one step */
+    int64_t large_jobs;         /* jobs run on the large-job lane */
+    int64_t bytes_admitted;     /* footprints of the admitted, unfinished jobs now */
+    int64_t peak_bytes_admitted;
+} km_server_stats;
+
+km_status km_server_create(const km_server_config* cfg, km_server** out);
+/* Runs every submitted job to completion, then frees the server. */
+km_status km_server_destroy(km_server* server);
+/* Message of this thread's last failed km_server_* call (or failed job's wait). */
+const char* km_server_last_error(void);
+
+/* Fit job: max_iter Lloyd iterations from C0.  With KM_FLAG_DEVICE_POINTERS, X,
+ * C0, C_out and labels_out are device memory used in place; otherwise host
+ * memory, copied in at admission and out at completion.  Every buffer must stay
+ * valid until the job has finished.  Preconditions as km_lloyd_train_f32; a job
+ * whose footprint exceeds the budget fails with KM_DEVICE_BAD_ALLOC. */
+km_status km_server_submit_fit_f32(km_server* server, const float* X, int64_t n, int64_t d,
+                                   const float* C0, int64_t k, int64_t max_iter, uint32_t flags,
+                                   float* C_out, int32_t* labels_out, int64_t* job_out);
+/* Infer job: labels of X against the centroids C (k x d) and the objective. */
+km_status km_server_submit_infer_f32(km_server* server, const float* X, int64_t n, int64_t d,
+                                     const float* C, int64_t k, uint32_t flags, int32_t* labels_out,
+                                     int64_t* job_out);
+/* Wait up to timeout_ms (negative: no limit) for a job.  Returns KM_PENDING if it
+ * is still running; otherwise the job's own status, fills *out and forgets the job. */
+km_status km_server_wait(km_server* server, int64_t job, int64_t timeout_ms, km_job_result* out);
+km_status km_server_get_stats(km_server* server, km_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libkmeans_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = [os.path.join(CSRC, "km_engine.hip")]
+SOURCES = [os.path.join(CSRC, "km_engine.hip"), os.path.join(CSRC, "km_server.cpp")]
 HEADERS = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
 
 FLAGS = [

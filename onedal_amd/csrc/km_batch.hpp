@@ -600,7 +600,8 @@ __global__ void batch_objective_sum_kernel(const batch_inst* __restrict__ I, int
 }
 
 // centroid copies over the k * d elements of every instance (prefix table):
-// dir 0: C0 -> C (start), dir 1: C -> C_out (result)
+// dir 0: C0 -> C (start), dir 1: C -> C_out (result; instances without C_out are
+// skipped)
 __global__ __launch_bounds__(256) void batch_centroid_copy_kernel(const batch_inst* __restrict__ I,
                                                                   int B,
                                                                   const int64_t* __restrict__ pre,
@@ -613,7 +614,7 @@ __global__ __launch_bounds__(256) void batch_centroid_copy_kernel(const batch_in
         const int64_t le = e - pre[ii];
         if (dir == 0)
             C[in.c0 + le] = in.C0[le];
-        else
+        else if (in.C_out != nullptr)
             in.C_out[le] = C[in.c0 + le];
     }
 }

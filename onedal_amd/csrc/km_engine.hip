@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/onedal_amd/kmeans_hip.h"
+#include "km_args.hpp"
 #include "km_assign.hpp"
 #include "km_assign_bf16.hpp"
 #include "km_batch.hpp"
@@ -38,11 +39,6 @@
 #include "km_update.hpp"
 
 namespace km {
-
-struct status_error : std::runtime_error {
-    km_status status;
-    status_error(km_status s, const std::string& m) : std::runtime_error(m), status(s) {}
-};
 
 #define KM_HIP(expr)                                                                       \
     do {                                                                                   \
@@ -84,6 +80,18 @@ struct dbuf {
     }
 };
 
+// workspace of one batched plan: instance and prefix tables, concatenated
+// per-row, per-cluster and per-chunk arrays, sort scratch
+struct batch_ws {
+    dbuf inst, pre, labels, keys, perm, d64, ccur, cnext, counts, offsets, cb, empty, meta, qpre,
+        part, objc, obj, sort_tmp;
+    void release() {
+        dbuf* all[] = {&inst, &pre,     &labels, &keys, &perm, &d64,  &ccur, &cnext, &counts,
+                       &offsets, &cb, &empty,  &meta, &qpre, &part, &objc, &obj,  &sort_tmp};
+        for (dbuf* b : all) b->release();
+    }
+};
+
 }  // namespace km
 
 struct km_ctx {
@@ -102,9 +110,8 @@ struct km_ctx {
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
         sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
         c_out_dev;
-    // batched training
-    km::dbuf b_inst, b_pre, b_labels, b_keys, b_perm, b_d64, b_ccur, b_cnext, b_counts, b_offsets,
-        b_cb, b_empty, b_meta, b_qpre, b_part, b_objc, b_obj, b_sort_tmp;
+    // batched training and steps: the iterated set, and the finished set of a step
+    km::batch_ws bws, bws_fin;
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -766,76 +773,77 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
 }
 
 // --- precondition checks (reference order: train_ops.hpp:40-61, common.cpp:82-115) ---
-static void check_train_args(const void* X, int64_t n, int64_t d, const void* C0, int64_t k,
-                             int64_t max_iter, double thr, const void* C_out, const void* labels) {
-    if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
-    if (k > std::numeric_limits<int32_t>::max())
-        throw status_error(KM_DOMAIN_ERROR, "Cluster count is greater than max int32 value");
-    if (max_iter < 0) throw status_error(KM_DOMAIN_ERROR, "Max iteration count lower than zero");
-    if (thr < 0.0) throw status_error(KM_DOMAIN_ERROR, "Accuracy_threshold is lower than zero");
-    if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
-    if (n > std::numeric_limits<int32_t>::max())
-        throw status_error(KM_DOMAIN_ERROR, "Row count is greater than max int32 value");
-    if (k > n) throw status_error(KM_INVALID_ARGUMENT, "Cluster count exceeds data row count");
-    if (C0 == nullptr)
-        throw status_error(KM_UNIMPLEMENTED,
-                           "Initial centroids are required: default k-means++ initialisation is "
-                           "not part of this engine");
-    if (C_out == nullptr || labels == nullptr)
-        throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
-    if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
-}
-
-static void check_infer_args(const void* X, int64_t n, int64_t d, const void* C, int64_t k,
-                             uint32_t flags, const void* labels, const void* obj) {
-    if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
-    if (k > std::numeric_limits<int32_t>::max())
-        throw status_error(KM_DOMAIN_ERROR, "Cluster count is greater than max int32 value");
-    if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
-    if (C == nullptr) throw status_error(KM_DOMAIN_ERROR, "Input model centroids are empty");
-    if (n > std::numeric_limits<int32_t>::max())
-        throw status_error(KM_DOMAIN_ERROR, "Row count is greater than max int32 value");
-    if (labels == nullptr) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
-    if ((flags & KM_FLAG_EXACT_OBJECTIVE) && obj == nullptr)
-        throw status_error(KM_INVALID_ARGUMENT, "Objective output is required");
-    if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
-}
-
 // ---------------------------------------------------------------------------
-// Batched training (km_batch_train_f32).  Instance tables and prefix tables are
-// built on the host; each iteration is a fixed sequence of launches over all
-// instances (optionally captured once as a HIP graph and replayed).
+// Batched plans (km_batch_train_f32, km_batch_step_f32).  A plan is a set of
+// instances with its instance table and prefix tables, built on the host; each
+// stage of an iteration is one launch over every instance of the plan, so an
+// iteration needs no host synchronisation and can be captured as a HIP graph.
 // ---------------------------------------------------------------------------
-static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
-                        int64_t max_iter, uint32_t flags, double* obj_out, int64_t* iters_out) {
-    if (!inst || count <= 0) throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
-    if (count > std::numeric_limits<int32_t>::max())
-        throw status_error(KM_DOMAIN_ERROR, "Instance count is greater than max int32 value");
-    if (max_iter < 0)
-        throw status_error(KM_DOMAIN_ERROR, "Max iteration count should be >= 0");
-    if (!obj_out || !iters_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
-    const int B = int(count);
-    std::vector<batch_inst> hi(B);
+struct batch_plan {
     // prefix tables: assignment batches, histogram chunks, centroid elements,
     // d64 batches, objective chunks
     enum { P_ASG = 0, P_CNT, P_EL, P_D64, P_OBJ, P_NUM };
-    std::vector<int64_t> pre(size_t(P_NUM) * (B + 1), 0);
-    auto P = [&](int t, int i) -> int64_t& { return pre[size_t(t) * (B + 1) + i]; };
+    int B = 0;
+    bool train = false;
+    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
+    unsigned kbits = 1, lbits = 0;
+    std::vector<batch_inst> hi;
+    std::vector<int64_t> pre;
+    // device views (plan_upload)
+    batch_inst* dI = nullptr;
+    int64_t* dpre = nullptr;
+    int32_t* labels = nullptr;
+    uint32_t* skeys = nullptr;
+    int32_t* perm = nullptr;
+    double* d64 = nullptr;
+    float* Ccur = nullptr;
+    float* Cnext = nullptr;
+    int32_t* counts = nullptr;
+    int64_t* offsets = nullptr;
+    int32_t* cb = nullptr;
+    int32_t* empty = nullptr;
+    int32_t* meta = nullptr;
+    int64_t* qpre = nullptr;
+    double* part = nullptr;
+    double* objc = nullptr;
+    double* objd = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+
+    int64_t P(int t, int i) const { return pre[size_t(t) * (B + 1) + i]; }
+    const int64_t* DP(int t) const { return dpre + size_t(t) * (B + 1); }
+};
+
+// Instance table and prefix tables of `count` instances.  train: the instances
+// are iterated (training preconditions, C_out required); otherwise they only get
+// a final pass (inference preconditions; labels_out required, C_out optional).
+static void plan_build(batch_plan& p, const km_batch_instance_f32* inst, int64_t count, bool train) {
+    if (!inst || count <= 0) throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
+    if (count > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Instance count is greater than max int32 value");
+    const int B = int(count);
+    p.B = B;
+    p.train = train;
+    p.hi.assign(B, batch_inst{});
+    p.pre.assign(size_t(batch_plan::P_NUM) * (B + 1), 0);
+    auto P = [&](int t, int i) -> int64_t& { return p.pre[size_t(t) * (B + 1) + i]; };
     int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
     int maxk = 1;
     for (int i = 0; i < B; ++i) {
         const km_batch_instance_f32& q = inst[i];
-        if (!q.X || !q.C0 || !q.C_out || !q.labels_out)
+        if (!q.X || !q.C0 || (train && !q.C_out) || (!train && !q.labels_out))
             throw status_error(KM_INVALID_ARGUMENT, "Batch instance " + std::to_string(i) +
                                                         ": input and output buffers are required");
         if (q.n <= 0 || q.d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
         if (q.k <= 0 || q.k > std::numeric_limits<int32_t>::max())
             throw status_error(KM_DOMAIN_ERROR, "Cluster count should be > 0 and <= max int32 value");
-        if (q.k > q.n)
+        if (q.n > std::numeric_limits<int32_t>::max())
+            throw status_error(KM_DOMAIN_ERROR, "Row count is greater than max int32 value");
+        if (train && q.k > q.n)
             throw status_error(KM_INVALID_ARGUMENT,
                                "Cluster count should be less than or equal to the row count");
         if (q.d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
-        batch_inst& b = hi[i];
+        batch_inst& b = p.hi[i];
         b.X = q.X;
         b.C0 = q.C0;
         b.C_out = q.C_out;
@@ -856,92 +864,148 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         pp += (q.n / REDUCE_CHUNK + q.k + 1) * q.d;
         oo += (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
         maxk = std::max<int>(maxk, int(q.k));
-        P(P_ASG, i + 1) = P(P_ASG, i) + (q.n + BATCH_RB - 1) / BATCH_RB;
-        P(P_CNT, i + 1) = P(P_CNT, i) + (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
-        P(P_EL, i + 1) = P(P_EL, i) + q.k * q.d;
-        P(P_D64, i + 1) = P(P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
-        P(P_OBJ, i + 1) = P(P_OBJ, i) + (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        P(batch_plan::P_ASG, i + 1) = P(batch_plan::P_ASG, i) + (q.n + BATCH_RB - 1) / BATCH_RB;
+        P(batch_plan::P_CNT, i + 1) = P(batch_plan::P_CNT, i) + (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
+        P(batch_plan::P_EL, i + 1) = P(batch_plan::P_EL, i) + q.k * q.d;
+        P(batch_plan::P_D64, i + 1) = P(batch_plan::P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
+        P(batch_plan::P_OBJ, i + 1) = P(batch_plan::P_OBJ, i) + (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     }
     if (rows >= (int64_t(1) << 31))
         throw status_error(KM_UNIMPLEMENTED, "a batch holds at most 2^31 - 1 rows in total");
-    ctx->stats = km_stats{};
-
-    // device tables and workspace
-    batch_inst* dI = ctx->b_inst.get<batch_inst>(B);
-    int64_t* dpre = ctx->b_pre.get<int64_t>(pre.size());
-    auto DP = [&](int t) { return dpre + size_t(t) * (B + 1); };
-    int32_t* labels = ctx->b_labels.get<int32_t>(rows);
-    uint32_t* skeys = ctx->b_keys.get<uint32_t>(2 * rows);  // keys, then their sorted copy
-    int32_t* perm = ctx->b_perm.get<int32_t>(rows);
-    double* d64 = ctx->b_d64.get<double>(rows);
-    float* Ccur = ctx->b_ccur.get<float>(cfl);
-    float* Cnext = ctx->b_cnext.get<float>(cfl);
-    int32_t* counts = ctx->b_counts.get<int32_t>(kk);
-    int64_t* offsets = ctx->b_offsets.get<int64_t>(kk + B);
-    int32_t* cb = ctx->b_cb.get<int32_t>(kk + B);
-    int32_t* empty = ctx->b_empty.get<int32_t>(kk);
-    int32_t* meta = ctx->b_meta.get<int32_t>(2 * int64_t(B));
-    int64_t* qpre = ctx->b_qpre.get<int64_t>(B + 1);
-    double* part = ctx->b_part.get<double>(pp);
-    double* objc = ctx->b_objc.get<double>(oo);
-    double* objd = ctx->b_obj.get<double>(B);
+    p.rows = rows;
+    p.cfl = cfl;
+    p.kk = kk;
+    p.pp = pp;
+    p.oo = oo;
     // rows are grouped by (instance, label) with one stable radix sort of the keys
     // (index << lbits) | label; instances are contiguous, so each instance's rows
     // come out by label in row order
     unsigned lbits = 0, ibits = 0;
     while ((1ull << lbits) < (unsigned long long)maxk) lbits++;
     while ((1ull << ibits) < (unsigned long long)B) ibits++;
-    if (lbits + ibits > 32)
+    if (train && lbits + ibits > 32)
         throw status_error(KM_UNIMPLEMENTED, "batch too large for 32-bit sort keys "
                                              "(instances x max cluster count above 2^32)");
-    const unsigned kbits = std::max(1u, lbits + ibits);
-    size_t tmp_bytes = 0;
-    rocprim::counting_iterator<int32_t> ids(0);
-    KM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, skeys, skeys + rows, ids, perm,
-                                     unsigned(rows), 0, kbits, ctx->stream));
-    void* tmp = ctx->b_sort_tmp.get<char>(tmp_bytes);
-    KM_HIP(hipMemcpyAsync(dI, hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
-    KM_HIP(hipMemcpyAsync(dpre, pre.data(), sizeof(int64_t) * pre.size(), hipMemcpyHostToDevice,
+    p.lbits = lbits;
+    p.kbits = std::max(1u, lbits + ibits);
+}
+
+static int batch_grid(const km_ctx* ctx, int64_t items) {
+    const int64_t cap = int64_t(ctx->cu_count) * 8;
+    return int(std::max<int64_t>(1, std::min<int64_t>(items, cap)));
+}
+
+// Device workspace of the plan and the table upload (stream-ordered: the host
+// tables must stay alive until the stream has passed the copies).
+static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
+    const int B = p.B;
+    p.dI = ws.inst.get<batch_inst>(B);
+    p.dpre = ws.pre.get<int64_t>(p.pre.size());
+    p.Ccur = ws.ccur.get<float>(p.cfl);
+    p.objc = ws.objc.get<double>(p.oo);
+    p.objd = ws.obj.get<double>(B);
+    if (p.train) {
+        p.labels = ws.labels.get<int32_t>(p.rows);
+        p.skeys = ws.keys.get<uint32_t>(2 * p.rows);  // keys, then their sorted copy
+        p.perm = ws.perm.get<int32_t>(p.rows);
+        p.d64 = ws.d64.get<double>(p.rows);
+        p.Cnext = ws.cnext.get<float>(p.cfl);
+        p.counts = ws.counts.get<int32_t>(p.kk);
+        p.offsets = ws.offsets.get<int64_t>(p.kk + B);
+        p.cb = ws.cb.get<int32_t>(p.kk + B);
+        p.empty = ws.empty.get<int32_t>(p.kk);
+        p.meta = ws.meta.get<int32_t>(2 * int64_t(B));
+        p.qpre = ws.qpre.get<int64_t>(B + 1);
+        p.part = ws.part.get<double>(p.pp);
+        p.tmp_bytes = 0;
+        rocprim::counting_iterator<int32_t> ids(0);
+        KM_HIP(rocprim::radix_sort_pairs(nullptr, p.tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
+                                         unsigned(p.rows), 0, p.kbits, ctx->stream));
+        p.tmp = ws.sort_tmp.get<char>(p.tmp_bytes);
+    }
+    KM_HIP(hipMemcpyAsync(p.dI, p.hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
+    KM_HIP(hipMemcpyAsync(p.dpre, p.pre.data(), sizeof(int64_t) * p.pre.size(), hipMemcpyHostToDevice,
                           ctx->stream));
-    KM_HIP(hipStreamSynchronize(ctx->stream));  // the host tables go out of scope below
+}
 
-    const int cap = ctx->cu_count * 8;
-    auto g = [&](int64_t items) { return int(std::max<int64_t>(1, std::min<int64_t>(items, cap))); };
-    hipLaunchKernelGGL(batch_centroid_copy_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
-                       ctx->stream, dI, B, DP(P_EL), Ccur, 0);
+// dir 0: every instance's C0 into the plan's concatenated centroids;
+// dir 1: those centroids out to every instance's C_out (skipped where it is null)
+static void plan_copy_centroids(km_ctx* ctx, const batch_plan& p, int dir) {
+    hipLaunchKernelGGL(batch_centroid_copy_kernel,
+                       dim3(batch_grid(ctx, (p.P(batch_plan::P_EL, p.B) + 255) / 256)), dim3(256), 0,
+                       ctx->stream, p.dI, p.B, p.DP(batch_plan::P_EL), p.Ccur, dir);
     KM_LAUNCH_CHECK();
+}
 
-    auto iteration = [&]() {
-        hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI,
-                           B, DP(P_ASG), Ccur, labels, skeys, int(lbits), 0);
-        KM_LAUNCH_CHECK();
-        fill<int32_t>(ctx, counts, kk, 0);
-        hipLaunchKernelGGL(batch_count_kernel, dim3(g(P(P_CNT, B))), dim3(256), 0, ctx->stream, dI, B,
-                           DP(P_CNT), labels, counts);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, dI, B, counts,
-                           offsets, cb, empty, meta);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, meta, B, qpre);
-        KM_LAUNCH_CHECK();
-        KM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, skeys, skeys + rows, ids, perm,
-                                         unsigned(rows), 0, kbits, ctx->stream));
-        hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, dI, B, qpre, perm,
-                           offsets, cb, part);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(batch_finalize_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
-                           ctx->stream, dI, B, DP(P_EL), counts, cb, part, Cnext);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(batch_d64_kernel, dim3(g(P(P_D64, B))), dim3(256), 0, ctx->stream, dI, B,
-                           DP(P_D64), meta, Ccur, labels, d64);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(batch_fill_empty_kernel, dim3(g(B)), dim3(256), 0, ctx->stream, dI, B, meta,
-                           empty, d64, Cnext);
-        KM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(copy_kernel<float>, dim3(g((cfl + 255) / 256)), dim3(256), 0, ctx->stream,
-                           Cnext, Ccur, cfl);
-        KM_LAUNCH_CHECK();
-    };
+// one Lloyd iteration of every instance of a training plan
+static void plan_iteration(km_ctx* ctx, const batch_plan& p) {
+    using BP = batch_plan;
+    const int B = p.B;
+    const int cap = ctx->cu_count * 8;
+    auto g = [&](int64_t items) { return batch_grid(ctx, items); };
+    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(p.P(BP::P_ASG, B))), dim3(256), 0, ctx->stream, p.dI,
+                       B, p.DP(BP::P_ASG), p.Ccur, p.labels, p.skeys, int(p.lbits), 0);
+    KM_LAUNCH_CHECK();
+    fill<int32_t>(ctx, p.counts, p.kk, 0);
+    hipLaunchKernelGGL(batch_count_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256), 0, ctx->stream, p.dI, B,
+                       p.DP(BP::P_CNT), p.labels, p.counts);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, p.dI, B, p.counts,
+                       p.offsets, p.cb, p.empty, p.meta);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, p.meta, B, p.qpre);
+    KM_LAUNCH_CHECK();
+    rocprim::counting_iterator<int32_t> ids(0);
+    size_t tmp_bytes = p.tmp_bytes;
+    KM_HIP(rocprim::radix_sort_pairs(p.tmp, tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
+                                     unsigned(p.rows), 0, p.kbits, ctx->stream));
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre, p.perm,
+                       p.offsets, p.cb, p.part);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3(g((p.P(BP::P_EL, B) + 255) / 256)), dim3(256), 0,
+                       ctx->stream, p.dI, B, p.DP(BP::P_EL), p.counts, p.cb, p.part, p.Cnext);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_d64_kernel, dim3(g(p.P(BP::P_D64, B))), dim3(256), 0, ctx->stream, p.dI, B,
+                       p.DP(BP::P_D64), p.meta, p.Ccur, p.labels, p.d64);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_fill_empty_kernel, dim3(g(B)), dim3(256), 0, ctx->stream, p.dI, B, p.meta,
+                       p.empty, p.d64, p.Cnext);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(copy_kernel<float>, dim3(g((p.cfl + 255) / 256)), dim3(256), 0, ctx->stream,
+                       p.Cnext, p.Ccur, p.cfl);
+    KM_LAUNCH_CHECK();
+}
+
+// final pass: labels into every instance's labels_out, objectives into p.objd,
+// centroids out
+static void plan_finish(km_ctx* ctx, const batch_plan& p) {
+    using BP = batch_plan;
+    const int B = p.B;
+    auto g = [&](int64_t items) { return batch_grid(ctx, items); };
+    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(p.P(BP::P_ASG, B))), dim3(256), 0, ctx->stream, p.dI, B,
+                       p.DP(BP::P_ASG), p.Ccur, nullptr, nullptr, int(p.lbits), 1);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_objective_kernel, dim3(g(p.P(BP::P_OBJ, B))), dim3(256), 0, ctx->stream, p.dI,
+                       B, p.DP(BP::P_OBJ), p.Ccur, p.objc);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch_objective_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, p.dI,
+                       B, p.objc, p.objd);
+    KM_LAUNCH_CHECK();
+    plan_copy_centroids(ctx, p, 1);
+}
+
+static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
+                        int64_t max_iter, uint32_t flags, double* obj_out, int64_t* iters_out) {
+    if (!inst || count <= 0) throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
+    if (max_iter < 0)
+        throw status_error(KM_DOMAIN_ERROR, "Max iteration count should be >= 0");
+    if (!obj_out || !iters_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    batch_plan p;
+    plan_build(p, inst, count, true);
+    ctx->stats = km_stats{};
+    plan_upload(ctx, ctx->bws, p);
+    KM_HIP(hipStreamSynchronize(ctx->stream));  // the host tables may be released from here on
+    plan_copy_centroids(ctx, p, 0);
 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -955,7 +1019,7 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         hipGraphExec_t exec = nullptr;
         KM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            iteration();
+            plan_iteration(ctx, p);
         } catch (...) {
             hipGraph_t broken = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &broken);
@@ -974,30 +1038,59 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         (void)hipGraphDestroy(graph);
         KM_HIP(le);
     } else {
-        for (int64_t it = 0; it < max_iter; ++it) iteration();
+        for (int64_t it = 0; it < max_iter; ++it) plan_iteration(ctx, p);
     }
     // final assignment (into the callers' label buffers), objective, centroids out
-    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(P(P_ASG, B))), dim3(256), 0, ctx->stream, dI, B,
-                       DP(P_ASG), Ccur, labels, skeys, int(lbits), 1);
-    KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(batch_objective_kernel, dim3(g(P(P_OBJ, B))), dim3(256), 0, ctx->stream, dI, B,
-                       DP(P_OBJ), Ccur, objc);
-    KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(batch_objective_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, dI,
-                       B, objc, objd);
-    KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(batch_centroid_copy_kernel, dim3(g((P(P_EL, B) + 255) / 256)), dim3(256), 0,
-                       ctx->stream, dI, B, DP(P_EL), Ccur, 1);
-    KM_LAUNCH_CHECK();
+    plan_finish(ctx, p);
     if (ctx->timing) KM_HIP(hipEventRecord(e1, ctx->stream));
-    KM_HIP(hipMemcpyAsync(obj_out, objd, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipMemcpyAsync(obj_out, p.objd, sizeof(double) * p.B, hipMemcpyDeviceToHost, ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < B; ++i) iters_out[i] = max_iter;
+    for (int i = 0; i < p.B; ++i) iters_out[i] = max_iter;
     if (ctx->timing) {
         float ms = 0.f;
         KM_HIP(hipEventElapsedTime(&ms, e0, e1));
         ctx->stats.iterations_ms = ms;
     }
+}
+
+// One step of a continuous batch (km_batch_step_f32): `iterations` Lloyd
+// iterations of every instance of `it` (C0 in, C_out out; they may alias), then
+// a final pass over every instance of `fin` (labels, objective, centroids out
+// where C_out is set).  The final pass runs after the iterations, so it sees
+// centroids the iterated set of the same step wrote.
+static void batch_step(km_ctx* ctx, const km_batch_instance_f32* it, int64_t n_it, int64_t iterations,
+                       const km_batch_instance_f32* fin, int64_t n_fin, uint32_t flags,
+                       double* obj_out) {
+    if (n_it < 0 || n_fin < 0 || (n_it == 0 && n_fin == 0))
+        throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
+    if (n_it > 0 && iterations < 0)
+        throw status_error(KM_DOMAIN_ERROR, "Max iteration count should be >= 0");
+    if (n_fin > 0 && !obj_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    if (flags != 0) throw status_error(KM_INVALID_ARGUMENT, "no flags are defined for a batched step");
+    batch_plan pi, pf;
+    if (n_it > 0) plan_build(pi, it, n_it, true);
+    if (n_fin > 0) plan_build(pf, fin, n_fin, false);
+    ctx->stats = km_stats{};
+    if (n_it > 0 && iterations > 0) {
+        plan_upload(ctx, ctx->bws, pi);
+        plan_copy_centroids(ctx, pi, 0);
+        for (int64_t t = 0; t < iterations; ++t) plan_iteration(ctx, pi);
+        plan_copy_centroids(ctx, pi, 1);
+    } else if (n_it > 0) {
+        // zero iterations: centroids are carried over unchanged
+        plan_upload(ctx, ctx->bws, pi);
+        plan_copy_centroids(ctx, pi, 0);
+        plan_copy_centroids(ctx, pi, 1);
+    }
+    if (n_fin > 0) {
+        plan_upload(ctx, ctx->bws_fin, pf);
+        plan_copy_centroids(ctx, pf, 0);
+        plan_finish(ctx, pf);
+        KM_HIP(hipMemcpyAsync(obj_out, pf.objd, sizeof(double) * pf.B, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    // the plans' host tables are read by the stream until here
+    KM_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 template <typename Fn>
@@ -1071,11 +1164,10 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
                         &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
-                        &ctx->seg_counts, &ctx->b_inst, &ctx->b_pre, &ctx->b_labels, &ctx->b_keys,
-                        &ctx->b_perm, &ctx->b_d64, &ctx->b_ccur, &ctx->b_cnext, &ctx->b_counts,
-                        &ctx->b_offsets, &ctx->b_cb, &ctx->b_empty, &ctx->b_meta, &ctx->b_qpre,
-                        &ctx->b_part, &ctx->b_objc, &ctx->b_obj, &ctx->b_sort_tmp};
+                        &ctx->seg_counts};
     for (auto* b : bufs) b->release();
+    ctx->bws.release();
+    ctx->bws_fin.release();
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
@@ -1149,6 +1241,14 @@ km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances
                              int64_t* iters_out) {
     return km::guarded(ctx, [&] {
         km::batch_train(ctx, instances, count, max_iter, flags, obj_out, iters_out);
+    });
+}
+
+km_status km_batch_step_f32(km_ctx* ctx, const km_batch_instance_f32* iterated, int64_t n_iterated,
+                            int64_t iterations, const km_batch_instance_f32* finished,
+                            int64_t n_finished, uint32_t flags, double* obj_out) {
+    return km::guarded(ctx, [&] {
+        km::batch_step(ctx, iterated, n_iterated, iterations, finished, n_finished, flags, obj_out);
     });
 }
 

@@ -19,6 +19,7 @@ KM_DEVICE_BAD_ALLOC = 3
 KM_HOST_BAD_ALLOC = 4
 KM_UNIMPLEMENTED = 5
 KM_INTERNAL_ERROR = 6
+KM_PENDING = 7
 
 KM_FLAG_EXACT_OBJECTIVE = 0x1
 KM_FLAG_BF16_REFINE = 0x2
@@ -33,7 +34,9 @@ PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64", "km_batch_train_f32",
+    "km_lloyd_infer_f64", "km_batch_train_f32", "km_batch_step_f32", "km_server_create",
+    "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",
+    "km_server_submit_infer_f32", "km_server_wait", "km_server_get_stats",
 ]
 
 
@@ -42,6 +45,29 @@ class KmBatchInstance(ctypes.Structure):
     _fields_ = [("X", ctypes.c_void_p), ("C0", ctypes.c_void_p), ("C_out", ctypes.c_void_p),
                 ("labels_out", ctypes.c_void_p), ("n", ctypes.c_int64), ("d", ctypes.c_int64),
                 ("k", ctypes.c_int64)]
+
+
+class KmServerConfig(ctypes.Structure):
+    """km_server_config (include/onedal_amd/kmeans_hip.h)."""
+    _fields_ = [("device", ctypes.c_int32), ("max_batch_jobs", ctypes.c_int32),
+                ("hbm_budget_bytes", ctypes.c_int64), ("large_job_work", ctypes.c_int64)]
+
+
+class KmJobResult(ctypes.Structure):
+    """km_job_result."""
+    _fields_ = [("iterations", ctypes.c_int64), ("objective", ctypes.c_double),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("lane", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class KmServerStats(ctypes.Structure):
+    """km_server_stats."""
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "submitted", "completed", "failed", "queued", "steps", "step_jobs", "max_step_jobs",
+        "large_jobs", "bytes_admitted", "peak_bytes_admitted")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class KmStats(ctypes.Structure):
@@ -124,6 +150,24 @@ def load_library():
     L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
                                      ctypes.POINTER(D), ctypes.POINTER(I64)]
     L.km_batch_train_f32.restype = I
+    L.km_batch_step_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64,
+                                    ctypes.POINTER(KmBatchInstance), I64, U32, ctypes.POINTER(D)]
+    L.km_batch_step_f32.restype = I
+    L.km_server_create.argtypes = [ctypes.POINTER(KmServerConfig), ctypes.POINTER(P)]
+    L.km_server_create.restype = I
+    L.km_server_destroy.argtypes = [P]
+    L.km_server_destroy.restype = I
+    L.km_server_last_error.argtypes = []
+    L.km_server_last_error.restype = ctypes.c_char_p
+    L.km_server_submit_fit_f32.argtypes = [P, P, I64, I64, P, I64, I64, U32, P, P,
+                                           ctypes.POINTER(I64)]
+    L.km_server_submit_fit_f32.restype = I
+    L.km_server_submit_infer_f32.argtypes = [P, P, I64, I64, P, I64, U32, P, ctypes.POINTER(I64)]
+    L.km_server_submit_infer_f32.restype = I
+    L.km_server_wait.argtypes = [P, I64, I64, ctypes.POINTER(KmJobResult)]
+    L.km_server_wait.restype = I
+    L.km_server_get_stats.argtypes = [P, ctypes.POINTER(KmServerStats)]
+    L.km_server_get_stats.restype = I
     _lib = L
     return L
 
@@ -283,6 +327,25 @@ class Engine:
         self._check(st)
         return cs, ls, list(obj), list(its)
 
+    def batch_step_device(self, iterated, iterations, finished):
+        """km_batch_step_f32.  iterated: (X, C0, C_out) tensors; finished: (X, C,
+        C_out or None, labels_out) tensors.  Returns the finished set's objectives."""
+        ni, nf = len(iterated), len(finished)
+        ia = (KmBatchInstance * max(ni, 1))()
+        fa = (KmBatchInstance * max(nf, 1))()
+        for i, (x, c0, cout) in enumerate(iterated):
+            ia[i] = KmBatchInstance(x.data_ptr(), c0.data_ptr(), cout.data_ptr(), None,
+                                    x.shape[0], x.shape[1], c0.shape[0])
+        for i, (x, c, cout, lab) in enumerate(finished):
+            fa[i] = KmBatchInstance(x.data_ptr(), c.data_ptr(),
+                                    cout.data_ptr() if cout is not None else None, lab.data_ptr(),
+                                    x.shape[0], x.shape[1], c.shape[0])
+        obj = (ctypes.c_double * max(nf, 1))()
+        st = self.lib.km_batch_step_f32(self.ctx, ia if ni else None, ni, int(iterations),
+                                        fa if nf else None, nf, 0, obj)
+        self._check(st)
+        return list(obj)[:nf]
+
     def infer_device(self, x, c, objective=False, bf16=False):
         import torch
 
@@ -297,3 +360,112 @@ class Engine:
             ctypes.byref(obj))
         self._check(st)
         return labels, (obj.value if objective else None)
+
+
+def _raise_server(st):
+    msg = load_library().km_server_last_error().decode()
+    raise _ERRORS.get(st, EngineError)(st, msg)
+
+
+class Server:
+    """A km_server: fit and infer jobs admitted by an HBM budget and run with
+    continuous batching on one device.
+
+    numpy inputs are host jobs (copied in at admission, results copied back);
+    contiguous float32 CUDA tensors are device jobs used in place.  The buffers of
+    a job are held here until its wait() returns."""
+
+    def __init__(self, budget_bytes, device=0, max_batch_jobs=0, large_job_work=0):
+        self.lib = load_library()
+        cfg = KmServerConfig(int(device), int(max_batch_jobs), int(budget_bytes), int(large_job_work))
+        h = ctypes.c_void_p()
+        st = self.lib.km_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != KM_OK:
+            _raise_server(st)
+        self.handle = h
+        self._held = {}
+
+    def close(self):
+        if self.handle:
+            self.lib.km_server_destroy(self.handle)
+            self.handle = None
+            self._held.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _prepare(x, c, k_rows, want_centroids):
+        """(flags, X ptr, C ptr, C_out ptr, labels ptr, keep-alive, outputs)."""
+        if _is_torch_cuda(x):
+            import torch
+
+            assert x.dtype == torch.float32 and x.is_contiguous() and c.is_contiguous()
+            n, d = x.shape
+            labels = torch.empty(n, dtype=torch.int32, device=x.device)
+            cout = torch.empty((k_rows, d), dtype=torch.float32, device=x.device) if want_centroids else None
+            keep = (x, c, labels, cout)
+            return (KM_FLAG_DEVICE_POINTERS, x.data_ptr(), c.data_ptr(),
+                    cout.data_ptr() if cout is not None else None, labels.data_ptr(), keep, (cout, labels))
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        c = np.ascontiguousarray(c, dtype=np.float32)
+        n, d = x.shape
+        labels = np.empty(n, np.int32)
+        cout = np.empty((k_rows, d), np.float32) if want_centroids else None
+        keep = (x, c, labels, cout)
+        return (0, x.ctypes.data, c.ctypes.data, cout.ctypes.data if cout is not None else None,
+                labels.ctypes.data, keep, (cout, labels))
+
+    def submit_fit(self, x, c0, max_iter, bf16=False):
+        k = c0.shape[0]
+        flags, px, pc, pco, pl, keep, outs = self._prepare(x, c0, k, True)
+        flags |= KM_FLAG_BF16_REFINE if bf16 else 0
+        jid = ctypes.c_int64()
+        st = self.lib.km_server_submit_fit_f32(self.handle, px, x.shape[0], x.shape[1], pc, k,
+                                               int(max_iter), flags, pco, pl, ctypes.byref(jid))
+        if st != KM_OK:
+            _raise_server(st)
+        self._held[jid.value] = (keep, outs)
+        return jid.value
+
+    def submit_infer(self, x, c, bf16=False):
+        k = c.shape[0]
+        flags, px, pc, _, pl, keep, outs = self._prepare(x, c, k, False)
+        flags |= KM_FLAG_BF16_REFINE if bf16 else 0
+        jid = ctypes.c_int64()
+        st = self.lib.km_server_submit_infer_f32(self.handle, px, x.shape[0], x.shape[1], pc, k, flags,
+                                                 pl, ctypes.byref(jid))
+        if st != KM_OK:
+            _raise_server(st)
+        self._held[jid.value] = (keep, outs)
+        return jid.value
+
+    def wait(self, job, timeout_ms=-1):
+        """Result dict (centroids, labels, objective, iterations, queue_ms, latency_ms,
+        lane), or None if the job is still running after timeout_ms."""
+        r = KmJobResult()
+        st = self.lib.km_server_wait(self.handle, int(job), int(timeout_ms), ctypes.byref(r))
+        if st == KM_PENDING:
+            return None
+        keep, outs = self._held.pop(job, (None, (None, None)))
+        if st != KM_OK:
+            _raise_server(st)
+        return {"centroids": outs[0], "labels": outs[1], "objective": r.objective,
+                "iterations": r.iterations, "queue_ms": r.queue_ms, "latency_ms": r.latency_ms,
+                "lane": r.lane}
+
+    def stats(self):
+        s = KmServerStats()
+        st = self.lib.km_server_get_stats(self.handle, ctypes.byref(s))
+        if st != KM_OK:
+            _raise_server(st)
+        return s.as_dict()
