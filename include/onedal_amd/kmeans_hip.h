@@ -61,6 +61,17 @@ typedef enum {
 /* context options (km_context_set_option) */
 #define KM_OPT_ASSIGN_PATH 1 /* 0 auto, 1 generic tile kernel, 2 FP32 matrix core */
 #define KM_OPT_TIMING 2      /* 1: time the assignment kernels with HIP events */
+#define KM_OPT_BATCH_TILE 3  /* batched assignment tile: 0 the context's table (tuned by
+                                km_batch_autotune, else chosen by k), v + 1 force variant v
+                                (0 <= v < KM_BATCH_TILE_VARIANTS) */
+
+/* Tile variants of the batched exact assignment: variant v puts CT centroids on
+ * the lanes of a row group and RW rows on each lane, (CT, RW) = (64,8) (64,4)
+ * (32,8) (16,8) (8,8) (32,4) (16,4).  All variants compute the same labels. */
+#define KM_BATCH_TILE_VARIANTS 7
+/* Buckets of the tile table: 6 * d_bucket + k_bucket, d in {<=8, <=16, <=32,
+ * <=64, <=128, <=256, more}, k in {<=8, <=16, <=32, <=64, <=128, more}. */
+#define KM_BATCH_TILE_BUCKETS 42
 
 typedef struct {
     int64_t assign_launches;      /* main assignment kernel launches in the last call */
@@ -127,6 +138,15 @@ km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances
                              int64_t max_iter, uint32_t flags, double* obj_out,
                              int64_t* iters_out);
 
+/* Autotuner of the batched assignment's tiles: times every tile variant on
+ * synthetic instances of each (d, k) bucket's largest shape on this context's
+ * device and keeps the fastest per bucket for the context's later batched calls.
+ * table_out (may be NULL) receives min(table_len, KM_BATCH_TILE_BUCKETS) chosen
+ * variants.  Takes a few seconds and about 1 GiB of device memory while it runs.
+ * No counterpart in the reference, whose kernels take fixed launch shapes
+ * (kernels_fp_impl.hpp:90-125 sizes the row blocks only). */
+km_status km_batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len);
+
 /* One step of a continuous batch, for schedulers that change the set of
  * instances between iterations.  First `iterations` Lloyd iterations of every
  * instance of `iterated` (centroids read from C0, written to C_out, which may
@@ -168,6 +188,8 @@ typedef struct {
     int64_t hbm_budget_bytes; /* admission budget over the admitted jobs' footprints */
     int64_t large_job_work;   /* n*k*d from which a job takes the large-job lane;
                                  0: 2^34, negative: never */
+    int32_t autotune;         /* nonzero: run km_batch_autotune for the batched lane at creation */
+    int32_t reserved;
 } km_server_config;
 
 typedef struct {

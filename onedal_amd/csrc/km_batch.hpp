@@ -44,8 +44,6 @@ struct batch_inst {
     int32_t vec;  // d % 4 == 0 and X 16-byte aligned
 };
 
-constexpr int BATCH_RW = 8;               // assignment rows per wave
-constexpr int BATCH_RB = 4 * BATCH_RW;    // assignment rows per workgroup
 constexpr int BATCH_CNT_ROWS = 4096;      // rows per histogram workgroup
 constexpr int BATCH_D64_ROWS = 256;       // rows per d64 workgroup
 constexpr int BATCH_HIST_MAX = 4096;      // LDS histogram bins
@@ -61,39 +59,62 @@ __device__ __forceinline__ int batch_find(const int64_t* __restrict__ pre, int B
 }
 
 // ---------------------------------------------------------------------------
-// Exact-form assignment of BATCH_RB rows of one instance per workgroup (8 per
-// wave) against all its centroids: lane l carries centroid j0 + l of a
-// 64-centroid tile, its features in registers (32 at a time, the next 32 loaded
-// ahead), the rows are staged in LDS (feature chunks of 256) and read as
-// broadcasts; each (row, centroid) sum runs over the features in order with
-// sub, mul and add rounded separately.  Scan semantics of
-// select_min_distance (kernels_fp_impl.hpp:198-202): strict < in centroid order,
-// a NaN distance to centroid 0 keeps label 0.
+// Exact-form assignment, tiled: a wave holds 64 / CT row groups of CT lanes; lane
+// (group g, slot c) carries centroid j0 + c of a CT-centroid tile against RW rows,
+// the centroid's features in registers (32 at a time, the next 32 loaded ahead),
+// the rows staged in LDS (feature chunks of FT) and read as broadcasts within the
+// group.  A workgroup (four waves) takes RB = 4 * RW * 64 / CT rows of one
+// instance.  Each (row, centroid) sum runs over the features in order with sub,
+// mul and add rounded separately, so every tile shape gives the same distances and
+// labels; the shape only changes speed (the autotuner picks one per (d, k)
+// bucket).  Scan semantics of select_min_distance (kernels_fp_impl.hpp:198-202):
+// strict < in centroid order, a NaN distance to centroid 0 keeps label 0.
+// The launch covers the instances sel[0..nsel) with pre the prefix of their
+// workgroup counts.
 // labels: concatenated labels (row0) or, with to_user, each instance's output;
 // skeys (when labels are concatenated): the row's sort key (index << lbits) |
 // label, so one stable radix sort over the batch groups every instance's rows
 // by label in row order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __restrict__ I, int B,
+constexpr int BATCH_TILES = 7;
+// (CT, RW) of each tile variant
+__host__ __device__ constexpr int batch_tile_ct(int v) {
+    return v == 0 ? 64 : v == 1 ? 64 : v == 2 ? 32 : v == 3 ? 16 : v == 4 ? 8 : v == 5 ? 32 : 16;
+}
+__host__ __device__ constexpr int batch_tile_rw(int v) {
+    return v == 0 ? 8 : v == 1 ? 4 : v == 2 ? 8 : v == 3 ? 8 : v == 4 ? 8 : v == 5 ? 4 : 4;
+}
+// rows per workgroup
+__host__ __device__ constexpr int batch_tile_rows(int v) { return 4 * batch_tile_rw(v) * (64 / batch_tile_ct(v)); }
+
+template <int CT, int RW>
+__global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __restrict__ I,
+                                                           const int32_t* __restrict__ sel, int nsel,
                                                            const int64_t* __restrict__ pre,
                                                            const float* __restrict__ C,
                                                            int32_t* __restrict__ labels,
                                                            uint32_t* __restrict__ skeys, int lbits,
                                                            int to_user) {
-    constexpr int RW = BATCH_RW, RB = BATCH_RB;
-    constexpr int FT = 256;  // staged features per row chunk
-    constexpr int CQ = 8;    // 4-vectors per register chunk
+    constexpr int G = 64 / CT;    // row groups per wave
+    constexpr int RWW = RW * G;   // rows per wave
+    constexpr int RB = 4 * RWW;   // rows per workgroup
+    constexpr int FT = 8192 / RB < 256 ? 8192 / RB : 256;  // staged features per row chunk
+    constexpr int CQ = 8;         // 4-vectors per register chunk
+    static_assert(FT % 4 == 0 && FT >= 4, "feature chunk");
     typedef float F4 __attribute__((ext_vector_type(4)));
     __shared__ __align__(16) float Xs[RB][FT];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int64_t total = pre[B];
+    const int cl = lane % CT;
+    const int rg = lane / CT;
+    const int rbase = wave * RWW + rg * RW;  // first staged row of this lane
+    const int64_t total = pre[nsel];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const int ii = batch_find(pre, B, b);
-        const batch_inst in = I[ii];
+        const int si = batch_find(pre, nsel, b);
+        const batch_inst in = I[sel[si]];
         const int d = in.d, k = in.k;
-        const int64_t r0 = (b - pre[ii]) * RB;
+        const int64_t r0 = (b - pre[si]) * RB;
         const int nr = int(min<int64_t>(RB, in.n - r0));
         const float* Ci = C + in.c0;
         const bool vec = in.vec != 0;
@@ -106,8 +127,8 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             bi[r] = 0x7fffffff;
             nan0[r] = false;
         }
-        for (int j0 = 0; j0 < k; j0 += 64) {
-            const int j = j0 + lane;
+        for (int j0 = 0; j0 < k; j0 += CT) {
+            const int j = j0 + cl;
             const float* c = Ci + int64_t(j < k ? j : 0) * d;
             float acc[RW];
 #pragma unroll
@@ -145,7 +166,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
                             if (u0 + q < nv) {
 #pragma unroll
                                 for (int r = 0; r < RW; ++r) {
-                                    const F4 x4 = *reinterpret_cast<const F4*>(&Xs[wave * RW + r][4 * (u0 + q)]);
+                                    const F4 x4 = *reinterpret_cast<const F4*>(&Xs[rbase + r][4 * (u0 + q)]);
 #pragma unroll
                                     for (int t = 0; t < 4; ++t) {
                                         const float diff = x4[t] - cur[q][t];
@@ -163,7 +184,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
                         const float cf = c[f0 + f];
 #pragma unroll
                         for (int r = 0; r < RW; ++r) {
-                            const float diff = Xs[wave * RW + r][f] - cf;
+                            const float diff = Xs[rbase + r][f] - cf;
                             const float sq = diff * diff;
                             acc[r] = acc[r] + sq;
                         }
@@ -186,7 +207,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             float bv = best[r];
             int bj = bi[r];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
+            for (int off = CT / 2; off > 0; off >>= 1) {
                 const float ob = __shfl_xor(bv, off);
                 const int oi = __shfl_xor(bj, off);
                 if (ob < bv || (ob == bv && oi < bj)) {
@@ -194,9 +215,9 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
                     bj = oi;
                 }
             }
-            const bool n0 = __shfl(nan0[r] ? 1 : 0, 0) != 0;
-            const int rr = wave * RW + r;
-            if (lane == 0 && rr < nr) {
+            const bool n0 = __shfl(nan0[r] ? 1 : 0, rg * CT) != 0;
+            const int rr = rbase + r;
+            if (cl == 0 && rr < nr) {
                 const int lab = (n0 || bj == 0x7fffffff) ? 0 : bj;
                 if (to_user) {
                     in.labels_out[r0 + rr] = lab;
@@ -206,6 +227,18 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
                 }
             }
         }
+    }
+}
+
+// synthetic finite floats in [-10, 10) from a hash of the index (autotuner inputs)
+__global__ void batch_tune_fill_kernel(float* __restrict__ p, int64_t count, uint32_t seed) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        uint32_t h = uint32_t(i) * 0x9E3779B1u ^ uint32_t(i >> 32) ^ seed;
+        h ^= h >> 15;
+        h *= 0x85EBCA77u;
+        h ^= h >> 13;
+        p[i] = float(h >> 8) * (20.0f / 16777216.0f) - 10.0f;
     }
 }
 

@@ -84,10 +84,10 @@ struct dbuf {
 // per-row, per-cluster and per-chunk arrays, sort scratch
 struct batch_ws {
     dbuf inst, pre, labels, keys, perm, d64, ccur, cnext, counts, offsets, cb, empty, meta, qpre,
-        part, objc, obj, sort_tmp;
+        part, objc, obj, sort_tmp, asel, apre;
     void release() {
-        dbuf* all[] = {&inst, &pre,     &labels, &keys, &perm, &d64,  &ccur, &cnext, &counts,
-                       &offsets, &cb, &empty,  &meta, &qpre, &part, &objc, &obj,  &sort_tmp};
+        dbuf* all[] = {&inst,  &pre,  &labels, &keys, &perm, &d64, &ccur, &cnext, &counts, &offsets, &cb,
+                       &empty, &meta, &qpre,   &part, &objc, &obj, &sort_tmp, &asel, &apre};
         for (dbuf* b : all) b->release();
     }
 };
@@ -102,6 +102,11 @@ struct km_ctx {
     km_stats stats{};
     int assign_path_opt = 0;
     bool timing = false;
+    // batched assignment tile variant per (d, k) bucket (-1: default by k), and a
+    // forced variant (0: none, v + 1: variant v)
+    int8_t tile_table[KM_BATCH_TILE_BUCKETS];
+    int tile_force = 0;
+    km_ctx() { std::memset(tile_table, -1, sizeof(tile_table)); }
 
     // workspace
     km::dbuf x_bf16, enorm, xmax, c_bf16, cnb, params, coll_rows, coll_T, coll_count, coll_ids;
@@ -112,6 +117,7 @@ struct km_ctx {
         c_out_dev;
     // batched training and steps: the iterated set, and the finished set of a step
     km::batch_ws bws, bws_fin;
+    km::dbuf tune_x, tune_c, tune_l;  // autotuner inputs (released after tuning)
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -782,16 +788,28 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
 struct batch_plan {
     // prefix tables: assignment batches, histogram chunks, centroid elements,
     // d64 batches, objective chunks
-    enum { P_ASG = 0, P_CNT, P_EL, P_D64, P_OBJ, P_NUM };
+    enum { P_CNT = 0, P_EL, P_D64, P_OBJ, P_NUM };
+    // one assignment launch per tile variant in use: its instances
+    // (asel[sel_off ..]) and the prefix of their workgroup counts (apre[pre_off ..])
+    struct asg_launch {
+        int v;
+        int nsel;
+        int64_t sel_off, pre_off, batches;
+    };
     int B = 0;
     bool train = false;
     int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
     unsigned kbits = 1, lbits = 0;
     std::vector<batch_inst> hi;
     std::vector<int64_t> pre;
+    std::vector<int32_t> asel;
+    std::vector<int64_t> apre;
+    std::vector<asg_launch> launches;
     // device views (plan_upload)
     batch_inst* dI = nullptr;
     int64_t* dpre = nullptr;
+    int32_t* d_asel = nullptr;
+    int64_t* d_apre = nullptr;
     int32_t* labels = nullptr;
     uint32_t* skeys = nullptr;
     int32_t* perm = nullptr;
@@ -817,7 +835,25 @@ struct batch_plan {
 // Instance table and prefix tables of `count` instances.  train: the instances
 // are iterated (training preconditions, C_out required); otherwise they only get
 // a final pass (inference preconditions; labels_out required, C_out optional).
-static void plan_build(batch_plan& p, const km_batch_instance_f32* inst, int64_t count, bool train) {
+// (d, k) bucket of the tile table: d in {<=8, <=16, <=32, <=64, <=128, <=256, more},
+// k in {<=8, <=16, <=32, <=64, <=128, more}; index 6 * d_bucket + k_bucket
+static int tile_bucket(int64_t d, int64_t k) {
+    const int db = d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : d <= 64 ? 3 : d <= 128 ? 4 : d <= 256 ? 5 : 6;
+    const int kb = k <= 8 ? 0 : k <= 16 ? 1 : k <= 32 ? 2 : k <= 64 ? 3 : k <= 128 ? 4 : 5;
+    return db * 6 + kb;
+}
+
+// untuned choice: the narrowest centroid tile that still holds k
+static int default_tile(int64_t k) { return k <= 8 ? 4 : k <= 16 ? 3 : k <= 32 ? 2 : 0; }
+
+static int tile_for(const km_ctx* ctx, int64_t d, int64_t k) {
+    if (ctx->tile_force > 0) return ctx->tile_force - 1;
+    const int t = ctx->tile_table[tile_bucket(d, k)];
+    return t >= 0 ? t : default_tile(k);
+}
+
+static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
+                       bool train) {
     if (!inst || count <= 0) throw status_error(KM_DOMAIN_ERROR, "Input batch is empty");
     if (count > std::numeric_limits<int32_t>::max())
         throw status_error(KM_DOMAIN_ERROR, "Instance count is greater than max int32 value");
@@ -864,7 +900,6 @@ static void plan_build(batch_plan& p, const km_batch_instance_f32* inst, int64_t
         pp += (q.n / REDUCE_CHUNK + q.k + 1) * q.d;
         oo += (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
         maxk = std::max<int>(maxk, int(q.k));
-        P(batch_plan::P_ASG, i + 1) = P(batch_plan::P_ASG, i) + (q.n + BATCH_RB - 1) / BATCH_RB;
         P(batch_plan::P_CNT, i + 1) = P(batch_plan::P_CNT, i) + (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
         P(batch_plan::P_EL, i + 1) = P(batch_plan::P_EL, i) + q.k * q.d;
         P(batch_plan::P_D64, i + 1) = P(batch_plan::P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
@@ -888,6 +923,29 @@ static void plan_build(batch_plan& p, const km_batch_instance_f32* inst, int64_t
                                              "(instances x max cluster count above 2^32)");
     p.lbits = lbits;
     p.kbits = std::max(1u, lbits + ibits);
+    // assignment launches: instances grouped by tile variant, in instance order
+    std::vector<int> var(B);
+    for (int i = 0; i < B; ++i) var[i] = tile_for(ctx, inst[i].d, inst[i].k);
+    p.asel.clear();
+    p.apre.clear();
+    p.launches.clear();
+    for (int v = 0; v < BATCH_TILES; ++v) {
+        batch_plan::asg_launch L{v, 0, int64_t(p.asel.size()), int64_t(p.apre.size()), 0};
+        const int64_t rb = batch_tile_rows(v);
+        p.apre.push_back(0);
+        for (int i = 0; i < B; ++i) {
+            if (var[i] != v) continue;
+            p.asel.push_back(i);
+            L.batches += (inst[i].n + rb - 1) / rb;
+            p.apre.push_back(L.batches);
+            L.nsel++;
+        }
+        if (L.nsel == 0) {
+            p.apre.pop_back();
+            continue;
+        }
+        p.launches.push_back(L);
+    }
 }
 
 static int batch_grid(const km_ctx* ctx, int64_t items) {
@@ -904,6 +962,8 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
     p.Ccur = ws.ccur.get<float>(p.cfl);
     p.objc = ws.objc.get<double>(p.oo);
     p.objd = ws.obj.get<double>(B);
+    p.d_asel = ws.asel.get<int32_t>(p.asel.size());
+    p.d_apre = ws.apre.get<int64_t>(p.apre.size());
     if (p.train) {
         p.labels = ws.labels.get<int32_t>(p.rows);
         p.skeys = ws.keys.get<uint32_t>(2 * p.rows);  // keys, then their sorted copy
@@ -926,6 +986,34 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
     KM_HIP(hipMemcpyAsync(p.dI, p.hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
     KM_HIP(hipMemcpyAsync(p.dpre, p.pre.data(), sizeof(int64_t) * p.pre.size(), hipMemcpyHostToDevice,
                           ctx->stream));
+    KM_HIP(hipMemcpyAsync(p.d_asel, p.asel.data(), sizeof(int32_t) * p.asel.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    KM_HIP(hipMemcpyAsync(p.d_apre, p.apre.data(), sizeof(int64_t) * p.apre.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+}
+
+// exact assignment of every instance of the plan, one launch per tile variant in use
+static void plan_assign(km_ctx* ctx, const batch_plan& p, int32_t* labels, uint32_t* skeys, int to_user) {
+    for (const batch_plan::asg_launch& L : p.launches) {
+        const dim3 grid(batch_grid(ctx, L.batches)), block(256);
+        const int32_t* sel = p.d_asel + L.sel_off;
+        const int64_t* pre = p.d_apre + L.pre_off;
+        const int lb = int(p.lbits);
+#define KM_BATCH_ASSIGN(CT, RW)                                                                      \
+    hipLaunchKernelGGL((batch_assign_kernel<CT, RW>), grid, block, 0, ctx->stream, p.dI, sel, L.nsel, pre, \
+                       p.Ccur, labels, skeys, lb, to_user)
+        switch (L.v) {
+            case 0: KM_BATCH_ASSIGN(64, 8); break;
+            case 1: KM_BATCH_ASSIGN(64, 4); break;
+            case 2: KM_BATCH_ASSIGN(32, 8); break;
+            case 3: KM_BATCH_ASSIGN(16, 8); break;
+            case 4: KM_BATCH_ASSIGN(8, 8); break;
+            case 5: KM_BATCH_ASSIGN(32, 4); break;
+            default: KM_BATCH_ASSIGN(16, 4); break;
+        }
+#undef KM_BATCH_ASSIGN
+        KM_LAUNCH_CHECK();
+    }
 }
 
 // dir 0: every instance's C0 into the plan's concatenated centroids;
@@ -943,9 +1031,7 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p) {
     const int B = p.B;
     const int cap = ctx->cu_count * 8;
     auto g = [&](int64_t items) { return batch_grid(ctx, items); };
-    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(p.P(BP::P_ASG, B))), dim3(256), 0, ctx->stream, p.dI,
-                       B, p.DP(BP::P_ASG), p.Ccur, p.labels, p.skeys, int(p.lbits), 0);
-    KM_LAUNCH_CHECK();
+    plan_assign(ctx, p, p.labels, p.skeys, 0);
     fill<int32_t>(ctx, p.counts, p.kk, 0);
     hipLaunchKernelGGL(batch_count_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256), 0, ctx->stream, p.dI, B,
                        p.DP(BP::P_CNT), p.labels, p.counts);
@@ -982,9 +1068,7 @@ static void plan_finish(km_ctx* ctx, const batch_plan& p) {
     using BP = batch_plan;
     const int B = p.B;
     auto g = [&](int64_t items) { return batch_grid(ctx, items); };
-    hipLaunchKernelGGL(batch_assign_kernel, dim3(g(p.P(BP::P_ASG, B))), dim3(256), 0, ctx->stream, p.dI, B,
-                       p.DP(BP::P_ASG), p.Ccur, nullptr, nullptr, int(p.lbits), 1);
-    KM_LAUNCH_CHECK();
+    plan_assign(ctx, p, nullptr, nullptr, 1);
     hipLaunchKernelGGL(batch_objective_kernel, dim3(g(p.P(BP::P_OBJ, B))), dim3(256), 0, ctx->stream, p.dI,
                        B, p.DP(BP::P_OBJ), p.Ccur, p.objc);
     KM_LAUNCH_CHECK();
@@ -1001,7 +1085,7 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         throw status_error(KM_DOMAIN_ERROR, "Max iteration count should be >= 0");
     if (!obj_out || !iters_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
     batch_plan p;
-    plan_build(p, inst, count, true);
+    plan_build(p, ctx, inst, count, true);
     ctx->stats = km_stats{};
     plan_upload(ctx, ctx->bws, p);
     KM_HIP(hipStreamSynchronize(ctx->stream));  // the host tables may be released from here on
@@ -1068,8 +1152,8 @@ static void batch_step(km_ctx* ctx, const km_batch_instance_f32* it, int64_t n_i
     if (n_fin > 0 && !obj_out) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
     if (flags != 0) throw status_error(KM_INVALID_ARGUMENT, "no flags are defined for a batched step");
     batch_plan pi, pf;
-    if (n_it > 0) plan_build(pi, it, n_it, true);
-    if (n_fin > 0) plan_build(pf, fin, n_fin, false);
+    if (n_it > 0) plan_build(pi, ctx, it, n_it, true);
+    if (n_fin > 0) plan_build(pf, ctx, fin, n_fin, false);
     ctx->stats = km_stats{};
     if (n_it > 0 && iterations > 0) {
         plan_upload(ctx, ctx->bws, pi);
@@ -1091,6 +1175,78 @@ static void batch_step(km_ctx* ctx, const km_batch_instance_f32* it, int64_t n_i
     }
     // the plans' host tables are read by the stream until here
     KM_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// Tile autotuner (km_batch_autotune): for every (d, k) bucket, synthetic
+// instances of the bucket's largest shape are assigned with every tile variant;
+// the fastest (HIP-event time, best of three after a warm-up launch) becomes the
+// context's variant for the bucket.  Every variant computes the same labels.
+static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
+    static const int dreps[7] = {8, 16, 32, 64, 128, 256, 512};
+    static const int kreps[6] = {8, 16, 32, 64, 128, 256};
+    constexpr int NI = 16;  // instances per timed launch
+    struct force_guard {
+        km_ctx* c;
+        int f;
+        ~force_guard() { c->tile_force = f; }
+    } guard{ctx, ctx->tile_force};
+    int8_t table[KM_BATCH_TILE_BUCKETS];
+    ctx->ev_used = 0;
+    hipEvent_t e0 = ctx->event(), e1 = ctx->event();
+    for (int db = 0; db < 7; ++db) {
+        for (int kb = 0; kb < 6; ++kb) {
+            const int d = dreps[db], k = kreps[kb];
+            // about 1e11 distance operations per launch, X at most 1 GiB
+            int64_t rows = int64_t(1e11 / (3.0 * k * d));
+            rows = std::min<int64_t>(std::max<int64_t>(rows, int64_t(1) << 18), int64_t(1) << 22);
+            rows = std::min<int64_t>(rows, (int64_t(1) << 28) / d);
+            const int64_t n = rows / NI;
+            float* X = ctx->tune_x.get<float>(NI * n * d);
+            float* C = ctx->tune_c.get<float>(int64_t(NI) * k * d);
+            int32_t* lab = ctx->tune_l.get<int32_t>(NI * n);
+            hipLaunchKernelGGL(batch_tune_fill_kernel, dim3(batch_grid(ctx, (NI * n * d + 255) / 256)),
+                               dim3(256), 0, ctx->stream, X, NI * n * d, 0x1234u + uint32_t(db * 6 + kb));
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(batch_tune_fill_kernel, dim3(batch_grid(ctx, (int64_t(NI) * k * d + 255) / 256)),
+                               dim3(256), 0, ctx->stream, C, int64_t(NI) * k * d, 0x9876u + uint32_t(db * 6 + kb));
+            KM_LAUNCH_CHECK();
+            std::vector<km_batch_instance_f32> q(NI);
+            for (int i = 0; i < NI; ++i)
+                q[i] = km_batch_instance_f32{X + i * n * d, C + int64_t(i) * k * d, nullptr, lab + i * n, n, d, k};
+            float best_ms = std::numeric_limits<float>::infinity();
+            int best_v = default_tile(k);
+            for (int v = 0; v < BATCH_TILES; ++v) {
+                ctx->tile_force = v + 1;
+                batch_plan p;
+                plan_build(p, ctx, q.data(), NI, false);
+                plan_upload(ctx, ctx->bws_fin, p);
+                plan_copy_centroids(ctx, p, 0);
+                plan_assign(ctx, p, nullptr, nullptr, 1);  // warm-up
+                float ms_v = std::numeric_limits<float>::infinity();
+                for (int rep = 0; rep < 3; ++rep) {
+                    KM_HIP(hipEventRecord(e0, ctx->stream));
+                    plan_assign(ctx, p, nullptr, nullptr, 1);
+                    KM_HIP(hipEventRecord(e1, ctx->stream));
+                    KM_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    KM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    ms_v = std::min(ms_v, ms);
+                }
+                KM_HIP(hipStreamSynchronize(ctx->stream));  // the plan's host tables go out of scope
+                if (ms_v < best_ms) {
+                    best_ms = ms_v;
+                    best_v = v;
+                }
+            }
+            table[db * 6 + kb] = int8_t(best_v);
+        }
+    }
+    std::memcpy(ctx->tile_table, table, sizeof(table));
+    ctx->tune_x.release();
+    ctx->tune_c.release();
+    ctx->tune_l.release();
+    if (table_out)
+        for (int64_t i = 0; i < std::min<int64_t>(table_len, KM_BATCH_TILE_BUCKETS); ++i) table_out[i] = table[i];
 }
 
 template <typename Fn>
@@ -1164,7 +1320,7 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
                         &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
-                        &ctx->seg_counts};
+                        &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l};
     for (auto* b : bufs) b->release();
     ctx->bws.release();
     ctx->bws_fin.release();
@@ -1186,6 +1342,10 @@ km_status km_context_set_option(km_ctx* ctx, int option, int64_t value) {
             return KM_OK;
         case KM_OPT_TIMING:
             ctx->timing = value != 0;
+            return KM_OK;
+        case KM_OPT_BATCH_TILE:
+            if (value < 0 || value > km::BATCH_TILES) return KM_INVALID_ARGUMENT;
+            ctx->tile_force = int(value);
             return KM_OK;
         default:
             return KM_INVALID_ARGUMENT;
@@ -1241,6 +1401,13 @@ km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances
                              int64_t* iters_out) {
     return km::guarded(ctx, [&] {
         km::batch_train(ctx, instances, count, max_iter, flags, obj_out, iters_out);
+    });
+}
+
+km_status km_batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
+    return km::guarded(ctx, [&] {
+        if (table_len < 0) throw km::status_error(KM_INVALID_ARGUMENT, "table length is negative");
+        km::batch_autotune(ctx, table_out, table_len);
     });
 }
 

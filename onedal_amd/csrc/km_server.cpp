@@ -515,6 +515,11 @@ km_status km_server_create(const km_server_config* cfg, km_server** out) {
         teardown(s);
         return fail(st, "no usable HIP device " + std::to_string(cfg->device));
     }
+    if (cfg->autotune && (st = km_batch_autotune(s->ctx_batch, nullptr, 0)) != KM_OK) {
+        const std::string msg = km_last_error(s->ctx_batch);
+        teardown(s);
+        return fail(st, "tile autotuning failed: " + msg);
+    }
     if (hipSetDevice(cfg->device) != hipSuccess ||
         hipStreamCreateWithFlags(&s->stage, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->out_batch, hipStreamNonBlocking) != hipSuccess ||

@@ -28,13 +28,17 @@ KM_FLAG_GRAPH = 0x8
 
 KM_OPT_ASSIGN_PATH = 1
 KM_OPT_TIMING = 2
+KM_OPT_BATCH_TILE = 3
+KM_BATCH_TILE_VARIANTS = 7
+KM_BATCH_TILE_BUCKETS = 42
 
 PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64", "km_batch_train_f32", "km_batch_step_f32", "km_server_create",
+    "km_lloyd_infer_f64", "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
+    "km_server_create",
     "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",
     "km_server_submit_infer_f32", "km_server_wait", "km_server_get_stats",
 ]
@@ -50,7 +54,8 @@ class KmBatchInstance(ctypes.Structure):
 class KmServerConfig(ctypes.Structure):
     """km_server_config (include/onedal_amd/kmeans_hip.h)."""
     _fields_ = [("device", ctypes.c_int32), ("max_batch_jobs", ctypes.c_int32),
-                ("hbm_budget_bytes", ctypes.c_int64), ("large_job_work", ctypes.c_int64)]
+                ("hbm_budget_bytes", ctypes.c_int64), ("large_job_work", ctypes.c_int64),
+                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class KmJobResult(ctypes.Structure):
@@ -150,6 +155,8 @@ def load_library():
     L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
                                      ctypes.POINTER(D), ctypes.POINTER(I64)]
     L.km_batch_train_f32.restype = I
+    L.km_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32), I64]
+    L.km_batch_autotune.restype = I
     L.km_batch_step_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64,
                                     ctypes.POINTER(KmBatchInstance), I64, U32, ctypes.POINTER(D)]
     L.km_batch_step_f32.restype = I
@@ -327,6 +334,13 @@ class Engine:
         self._check(st)
         return cs, ls, list(obj), list(its)
 
+    def batch_autotune(self):
+        """Tune the batched assignment's tile per (d, k) bucket; returns the chosen
+        variant of each of the KM_BATCH_TILE_BUCKETS buckets."""
+        table = (ctypes.c_int32 * KM_BATCH_TILE_BUCKETS)()
+        self._check(self.lib.km_batch_autotune(self.ctx, table, KM_BATCH_TILE_BUCKETS))
+        return list(table)
+
     def batch_step_device(self, iterated, iterations, finished):
         """km_batch_step_f32.  iterated: (X, C0, C_out) tensors; finished: (X, C,
         C_out or None, labels_out) tensors.  Returns the finished set's objectives."""
@@ -375,9 +389,10 @@ class Server:
     contiguous float32 CUDA tensors are device jobs used in place.  The buffers of
     a job are held here until its wait() returns."""
 
-    def __init__(self, budget_bytes, device=0, max_batch_jobs=0, large_job_work=0):
+    def __init__(self, budget_bytes, device=0, max_batch_jobs=0, large_job_work=0, autotune=False):
         self.lib = load_library()
-        cfg = KmServerConfig(int(device), int(max_batch_jobs), int(budget_bytes), int(large_job_work))
+        cfg = KmServerConfig(int(device), int(max_batch_jobs), int(budget_bytes), int(large_job_work),
+                             1 if autotune else 0, 0)
         h = ctypes.c_void_p()
         st = self.lib.km_server_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != KM_OK:

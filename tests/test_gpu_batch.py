@@ -106,3 +106,39 @@ def test_batch_badargs(engine):
     # usable afterwards
     c, l, o, it = engine.train_batch_device([x], [c0], 1)
     assert it == [1]
+
+
+def test_batch_every_tile_variant_identical(engine, oracle):
+    """Every tile shape of the batched assignment gives the canonical result."""
+    from onedal_amd.engine import KM_BATCH_TILE_VARIANTS, KM_OPT_BATCH_TILE
+
+    inst = make_batch(31, 20)
+    want = [oracle.canonical_lloyd(x, c0, 2, 0.0)[:3] for x, c0 in inst]
+    try:
+        for v in range(KM_BATCH_TILE_VARIANTS):
+            engine.set_option(KM_OPT_BATCH_TILE, v + 1)
+            c_out, l_out, obj, its = run_batch(engine, inst, 2, graph=(v % 2 == 1))
+            for i, (wc, wl, wo) in enumerate(want):
+                assert np.array_equal(l_out[i], wl), (v, i)
+                assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), (v, i)
+                assert obj[i] == wo, (v, i)
+    finally:
+        engine.set_option(KM_OPT_BATCH_TILE, 0)
+
+
+def test_batch_autotune_table_and_results(oracle):
+    """The autotuner fills every bucket with a valid variant, and batched training
+    on the tuned context still gives the canonical result."""
+    from onedal_amd.engine import KM_BATCH_TILE_BUCKETS, KM_BATCH_TILE_VARIANTS, Engine
+
+    with Engine(0) as eng:
+        table = eng.batch_autotune()
+        assert len(table) == KM_BATCH_TILE_BUCKETS
+        assert all(0 <= v < KM_BATCH_TILE_VARIANTS for v in table), table
+        inst = make_batch(37, 16)
+        c_out, l_out, obj, its = run_batch(eng, inst, 3, graph=True)
+        for i, (x, c0) in enumerate(inst):
+            wc, wl, wo = oracle.canonical_lloyd(x, c0, 3, 0.0)[:3]
+            assert np.array_equal(l_out[i], wl), i
+            assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), i
+            assert obj[i] == wo, i
