@@ -127,6 +127,28 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             bi[r] = 0x7fffffff;
             nan0[r] = false;
         }
+        // rows [r0, r0 + RB), features [f0, f0 + ft) into the LDS tile
+        auto stage = [&](int f0, int ft) {
+            __syncthreads();
+            if (vec) {
+                const int nv = ft / 4;
+                for (int e = tid; e < RB * nv; e += 256) {
+                    const int r = e / nv, q = e - r * nv;
+                    F4 v = F4{0.f, 0.f, 0.f, 0.f};
+                    if (r < nr) v = *reinterpret_cast<const F4*>(in.X + (r0 + r) * d + f0 + 4 * q);
+                    *reinterpret_cast<F4*>(&Xs[r][4 * q]) = v;
+                }
+            } else {
+                for (int e = tid; e < RB * ft; e += 256) {
+                    const int r = e / ft, q = e - r * ft;
+                    Xs[r][q] = r < nr ? in.X[(r0 + r) * d + f0 + q] : 0.f;
+                }
+            }
+            __syncthreads();
+        };
+        // rows that fit one chunk are staged once for every centroid tile
+        const bool one_chunk = d <= FT;
+        if (one_chunk) stage(0, d);
         for (int j0 = 0; j0 < k; j0 += CT) {
             const int j = j0 + cl;
             const float* c = Ci + int64_t(j < k ? j : 0) * d;
@@ -135,23 +157,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             for (int r = 0; r < RW; ++r) acc[r] = 0.f;
             for (int f0 = 0; f0 < d; f0 += FT) {
                 const int ft = min(FT, d - f0);
-                __syncthreads();
-                if (vec) {
-                    const int nv = ft / 4;
-                    for (int e = tid; e < RB * nv; e += 256) {
-                        const int r = e / nv, q = e - r * nv;
-                        F4 v = F4{0.f, 0.f, 0.f, 0.f};
-                        if (r < nr)
-                            v = *reinterpret_cast<const F4*>(in.X + (r0 + r) * d + f0 + 4 * q);
-                        *reinterpret_cast<F4*>(&Xs[r][4 * q]) = v;
-                    }
-                } else {
-                    for (int e = tid; e < RB * ft; e += 256) {
-                        const int r = e / ft, q = e - r * ft;
-                        Xs[r][q] = r < nr ? in.X[(r0 + r) * d + f0 + q] : 0.f;
-                    }
-                }
-                __syncthreads();
+                if (!one_chunk) stage(f0, ft);
                 if (vec) {
                     const int nv = ft / 4;
                     const F4* c4 = reinterpret_cast<const F4*>(c + f0);
@@ -485,36 +491,66 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const batch_inst* _
     }
 }
 
-// exact per-row d64 = sum_f ((double)x_f - (double)c_f)^2 (features in order)
-__device__ __forceinline__ double batch_row_d64(const float* __restrict__ x,
-                                                const float* __restrict__ c, int d) {
+// Row staging of the FP64 distance kernels: 256 rows of one instance, 32
+// features at a time (row stride 33 floats: a thread reading its own row hits its
+// own bank), so the X reads are coalesced whatever d is.
+constexpr int BATCH_D64_FT = 32;
+typedef float batch_d64_tile[256][BATCH_D64_FT + 1];
+
+// FP64 squared distance of row r0 + threadIdx.x (valid when threadIdx.x < nr) to
+// its labelled centroid: sum over the features in order of (double(x) - double(c))^2,
+// each sub, mul and add rounded separately (the single-instance d64 and objective
+// order); every thread of the workgroup must call it.  lab_base[r] is the label of row r.
+__device__ __forceinline__ double batch_block_d64(const batch_inst& in, int64_t r0, int nr,
+                                                  const int32_t* __restrict__ lab_base,
+                                                  const float* __restrict__ Ci, batch_d64_tile& Xs) {
+    const int t = threadIdx.x;
+    const int d = in.d;
+    int lab = 0;
+    if (t < nr) {
+        lab = lab_base[r0 + t];
+        lab = (lab >= 0 && lab < in.k) ? lab : 0;
+    }
+    const float* c = Ci + int64_t(lab) * d;
     double acc = 0.0;
-    for (int f = 0; f < d; ++f) {
-        const double diff = double(x[f]) - double(c[f]);
-        const double sq = diff * diff;
-        acc = acc + sq;
+    for (int f0 = 0; f0 < d; f0 += BATCH_D64_FT) {
+        const int ft = min(BATCH_D64_FT, d - f0);
+        __syncthreads();
+        for (int e = t; e < nr * ft; e += 256) {
+            const int r = e / ft, q = e - r * ft;
+            Xs[r][q] = in.X[(r0 + r) * d + f0 + q];
+        }
+        __syncthreads();
+        if (t < nr) {
+            for (int q = 0; q < ft; ++q) {
+                const double diff = double(Xs[t][q]) - double(c[f0 + q]);
+                const double sq = diff * diff;
+                acc = acc + sq;
+            }
+        }
     }
     return acc;
 }
 
 // d64 of every row of the instances that have empty clusters (meta[2i] > 0);
-// BATCH_D64_ROWS rows per workgroup, one row per thread
+// BATCH_D64_ROWS rows per workgroup
 __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __restrict__ I, int B,
                                                         const int64_t* __restrict__ pre,
                                                         const int32_t* __restrict__ meta,
                                                         const float* __restrict__ C,
                                                         const int32_t* __restrict__ labels,
                                                         double* __restrict__ d64) {
+    static_assert(BATCH_D64_ROWS == 256, "one row per thread");
+    __shared__ batch_d64_tile Xs;
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
         const int ii = batch_find(pre, B, b);
-        if (meta[2 * ii] == 0) continue;
+        if (meta[2 * ii] == 0) continue;  // uniform over the workgroup
         const batch_inst in = I[ii];
-        const int64_t r = (b - pre[ii]) * BATCH_D64_ROWS + threadIdx.x;
-        if (r >= in.n) continue;
-        const int lab = labels[in.row0 + r];
-        const float* c = C + in.c0 + int64_t(lab >= 0 && lab < in.k ? lab : 0) * in.d;
-        d64[in.row0 + r] = batch_row_d64(in.X + r * in.d, c, in.d);
+        const int64_t r0 = (b - pre[ii]) * BATCH_D64_ROWS;
+        const int nr = int(min<int64_t>(BATCH_D64_ROWS, in.n - r0));
+        const double v = batch_block_d64(in, r0, nr, labels + in.row0, C + in.c0, Xs);
+        if (int(threadIdx.x) < nr) d64[in.row0 + r0 + threadIdx.x] = v;
     }
 }
 
@@ -584,6 +620,7 @@ __global__ __launch_bounds__(256) void batch_objective_kernel(const batch_inst* 
                                                               const float* __restrict__ C,
                                                               double* __restrict__ chunk_sums) {
     __shared__ double part[256];
+    __shared__ batch_d64_tile Xs;
     const int t = threadIdx.x;
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -593,11 +630,10 @@ __global__ __launch_bounds__(256) void batch_objective_kernel(const batch_inst* 
         const int64_t r0 = lq * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, in.n - r0));
         double p = 0.0;
-        for (int r = t; r < rows; r += 256) {
-            const int64_t row = r0 + r;
-            const int lab = in.labels_out[row];
-            const float* c = C + in.c0 + int64_t(lab >= 0 && lab < in.k ? lab : 0) * in.d;
-            p = p + batch_row_d64(in.X + row * in.d, c, in.d);
+        for (int s0 = 0; s0 < rows; s0 += 256) {
+            const int nr = min(256, rows - s0);
+            const double v = batch_block_d64(in, r0 + s0, nr, in.labels_out, C + in.c0, Xs);
+            if (t < nr) p = p + v;
         }
         __syncthreads();
         part[t] = p;
