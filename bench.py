@@ -210,7 +210,8 @@ def run_batched(torch, args, wl):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "instances": count, "rows": rows, "iterations": iters,
-                   "path": "batched exact-form assignment, HIP graph per iteration",
+                   "path": "batched assignment (exact-form VALU tiles or certified FP32 matrix "
+                           "cores per tuned (d, k) bucket), HIP graph per iteration",
                    "tiles": "autotuned per (d, k) bucket" if tune else "default by k"},
         "autotune": tune,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2),

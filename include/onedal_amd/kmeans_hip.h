@@ -65,10 +65,14 @@ typedef enum {
                                 km_batch_autotune, else chosen by k), v + 1 force variant v
                                 (0 <= v < KM_BATCH_TILE_VARIANTS) */
 
-/* Tile variants of the batched exact assignment: variant v puts CT centroids on
- * the lanes of a row group and RW rows on each lane, (CT, RW) = (64,8) (64,4)
- * (32,8) (16,8) (8,8) (32,4) (16,4).  All variants compute the same labels. */
-#define KM_BATCH_TILE_VARIANTS 7
+/* Tile variants of the batched assignment.  Variants 0..6 compute the exact
+ * form on the vector units with CT centroids on the lanes of a row group and RW
+ * rows on each lane, (CT, RW) = (64,8) (64,4) (32,8) (16,8) (8,8) (32,4) (16,4).
+ * Variant 7 computes FP32 matrix-core distances with a proven error band and
+ * re-scores in the exact form every row the band cannot settle; it applies to
+ * instances with d <= 128 and k <= 128 (others take the default tile).  All
+ * variants compute the same labels. */
+#define KM_BATCH_TILE_VARIANTS 8
 /* Buckets of the tile table: 6 * d_bucket + k_bucket, d in {<=8, <=16, <=32,
  * <=64, <=128, <=256, more}, k in {<=8, <=16, <=32, <=64, <=128, more}. */
 #define KM_BATCH_TILE_BUCKETS 42

@@ -17,6 +17,7 @@
 // prefix table.
 #pragma once
 
+#include "km_assign.hpp"
 #include "km_common.hpp"
 
 namespace km {
@@ -76,8 +77,11 @@ __device__ __forceinline__ int batch_find(const int64_t* __restrict__ pre, int B
 // label, so one stable radix sort over the batch groups every instance's rows
 // by label in row order.
 // ---------------------------------------------------------------------------
-constexpr int BATCH_TILES = 7;
-// (CT, RW) of each tile variant
+// Tile variants 0..6 are exact-form VALU tiles; variant 7 is the certified FP32
+// matrix-core kernel below (instances with d <= 128 and k <= 128).
+constexpr int BATCH_TILES = 8;
+constexpr int BATCH_TILE_MFMA = 7;
+// (CT, RW) of each VALU tile variant
 __host__ __device__ constexpr int batch_tile_ct(int v) {
     return v == 0 ? 64 : v == 1 ? 64 : v == 2 ? 32 : v == 3 ? 16 : v == 4 ? 8 : v == 5 ? 32 : 16;
 }
@@ -234,6 +238,285 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Certified FP32 matrix-core assignment (tile variant 7) for instances with
+// d <= DP (DP in {32, 64, 128}) and k <= 128.  A work item is BATCH_MFMA_ROWS
+// rows of one instance.  The item loads the instance's centroids into LDS once
+// (zero-padded to DP features and to whole blocks of 32), with ||c_j||^2 (double
+// sum rounded to float) and max ||c_j|| rounded up.  Then each 128-row tile puts
+// a row in the registers of a lane pair (feature halves), runs
+// v_mfma_f32_32x32x2f32 against every block of 32 centroids and keeps four top-2
+// slots per row, as the single-instance FP32 kernel does (km_assign.hpp).  The
+// settle uses that kernel's error band: v_j = ||c_j||^2 - 2 x.c_j is within the
+// band of the exact-form distance minus ||x||^2, so every centroid that can be
+// the exact-form argmin has v_j <= T.  Rows with one such centroid are settled.
+// Rows with a few are re-scored exactly over those candidates (ties to the lowest
+// index).  Every other row, and every row of an item whose centroids are not all
+// finite, gets the exact scan of select_min_distance (kernels_fp_impl.hpp:198-202).
+// So labels equal the exact-form kernel's.
+// ---------------------------------------------------------------------------
+constexpr int BATCH_MFMA_ROWS = 1024;  // rows per work item
+constexpr int BATCH_MFMA_MAXK = 128;
+__host__ __device__ constexpr size_t batch_mfma_lds_bytes(int dp) {
+    return size_t(BATCH_MFMA_MAXK) * (dp + 4) * 4 + BATCH_MFMA_MAXK * 4 + 4 * 8;
+}
+
+// error-band constants of the FP32 matrix-core distance for d features (the
+// host's make_band<float>)
+__device__ __forceinline__ band_consts batch_band_f32(int d) {
+    band_consts b;
+    const double u = fp_traits<float>::u, ua = fp_traits<float>::u_acc;
+    b.a_xc = 2.0 * gamma_m(double(d), ua) + 2.01 * u;
+    b.a_cc = 2.01 * u + gamma_m(double(d), 2.220446049250313e-16);
+    const double g = gamma_m(double(d + 2), u);
+    b.g_canon = g / (1.0 - g);
+    return b;
+}
+
+template <int DP>
+__global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_inst* __restrict__ I,
+                                                                   const int32_t* __restrict__ sel, int nsel,
+                                                                   const int64_t* __restrict__ pre,
+                                                                   const float* __restrict__ C,
+                                                                   int32_t* __restrict__ labels,
+                                                                   uint32_t* __restrict__ skeys, int lbits,
+                                                                   int to_user) {
+    constexpr int HALF = DP / 2;
+    constexpr int CST = DP + 4;  // LDS row stride in floats
+    extern __shared__ __align__(16) float smem[];
+    float* Cs = smem;                                                   // BATCH_MFMA_MAXK x CST
+    float* Cn = smem + BATCH_MFMA_MAXK * CST;                           // BATCH_MFMA_MAXK
+    double* wmax = reinterpret_cast<double*>(Cn + BATCH_MFMA_MAXK);     // one per wave
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int r32 = lane & 31;
+    const int h = lane >> 5;
+    const float finf = __builtin_inff();
+    const int64_t total = pre[nsel];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int si = batch_find(pre, nsel, b);
+        const batch_inst in = I[sel[si]];
+        const int d = in.d, k = in.k;
+        if (d > DP || k > BATCH_MFMA_MAXK) continue;  // never planned so; keeps LDS in bounds
+        const int64_t rb = (b - pre[si]) * BATCH_MFMA_ROWS;
+        const int64_t re = min<int64_t>(in.n, rb + BATCH_MFMA_ROWS);
+        const float* Ci = C + in.c0;
+        const int mblocks = (k + 31) / 32;
+        const int kp = mblocks * 32;
+        __syncthreads();  // the previous item's LDS reads are done
+        for (int e = tid; e < kp * DP; e += 256) {
+            const int j = e / DP, f = e - j * DP;
+            Cs[j * CST + f] = (j < k && f < d) ? Ci[int64_t(j) * d + f] : 0.f;
+        }
+        __syncthreads();
+        // ||c_j||^2 and the largest of them: one wave per centroid; a centroid
+        // that is not finite sends every row of the item to the exact scan
+        double wm = 0.0;
+        bool wbad = false;
+        for (int j = wave; j < kp; j += 4) {
+            double sq = 0.0;
+            for (int f = lane; f < DP; f += 64) {
+                const double v = double(Cs[j * CST + f]);
+                sq = fma(v, v, sq);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+            if (lane == 0) Cn[j] = j < k ? float(sq) : finf;
+            if (j < k) {
+                if (!(sq < 1.0e300)) wbad = true;  // NaN or overflow
+                wm = fmax(wm, sq);
+            }
+        }
+        if (lane == 0) wmax[wave] = wbad ? -1.0 : wm;
+        __syncthreads();
+        const bool cbad = wmax[0] < 0.0 || wmax[1] < 0.0 || wmax[2] < 0.0 || wmax[3] < 0.0;
+        const double cmax =
+            sqrt(fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]))) * (1.0 + 1e-12);
+        const band_consts bc = batch_band_f32(d);
+        for (int64_t t0 = rb; t0 < re; t0 += 128) {
+            const int64_t row = t0 + wave * 32 + r32;
+            const bool ok = row < re;
+            const float* xrow = in.X + (ok ? row : rb) * int64_t(d);
+            float xr[HALF];
+            if (in.vec) {
+#pragma unroll
+                for (int s4 = 0; s4 < HALF; s4 += 4) {
+                    const int f = h * HALF + s4;
+                    floatx4 v = {0.f, 0.f, 0.f, 0.f};
+                    if (ok && f < d) v = *reinterpret_cast<const floatx4*>(xrow + f);
+                    xr[s4] = v[0];
+                    xr[s4 + 1] = v[1];
+                    xr[s4 + 2] = v[2];
+                    xr[s4 + 3] = v[3];
+                }
+            } else {
+#pragma unroll
+                for (int s1 = 0; s1 < HALF; ++s1) {
+                    const int f = h * HALF + s1;
+                    xr[s1] = (ok && f < d) ? xrow[f] : 0.f;
+                }
+            }
+            // ||x|| rounded up (double sum over both halves)
+            double xs = 0.0;
+#pragma unroll
+            for (int s1 = 0; s1 < HALF; ++s1) {
+                const double v = double(xr[s1]);
+                xs = fma(v, v, xs);
+            }
+            xs += __shfl_xor(xs, 32);
+            const double Xup = sqrt(xs) * (1.0 + 1e-12);
+
+            top2<float> st[2];
+            st[0].init();
+            st[1].init();
+            for (int mb = 0; mb < mblocks; ++mb) {
+                floatx16 acc;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+                const float* arow = &Cs[(mb * 32 + r32) * CST + h * HALF];
+#pragma unroll
+                for (int s4 = 0; s4 < HALF; s4 += 4) {
+                    const floatx4 a4 = *reinterpret_cast<const floatx4*>(arow + s4);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q], xr[s4 + q], acc, 0, 0, 0);
+                }
+                const float* cnb = Cn + mb * 32;
+                top2<float>& slot = st[mb & 1];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const floatx4 c4 = *reinterpret_cast<const floatx4*>(cnb + 8 * q + 4 * h);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        slot.push(__builtin_fmaf(-2.f, acc[4 * q + t], c4[t]), mb * 32 + 8 * q + 4 * h + t);
+                }
+            }
+            // lane r32 (h = 0) holds the row's four slots: its two and lane r32 + 32's
+            top2<float> sl[4];
+            sl[0] = st[0];
+            sl[1] = st[1];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                sl[2 + q].m1 = __shfl_xor(st[q].m1, 32);
+                sl[2 + q].m2 = __shfl_xor(st[q].m2, 32);
+                sl[2 + q].i1 = __shfl_xor(st[q].i1, 32);
+                sl[2 + q].i2 = __shfl_xor(st[q].i2, 32);
+            }
+            int lab = 0;
+            bool need_scan = false;
+            if (h == 0 && ok) {
+                float best = finf;
+                int bi = -1;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (sl[q].m1 < best || (sl[q].m1 == best && sl[q].i1 >= 0 && sl[q].i1 < bi)) {
+                        best = sl[q].m1;
+                        bi = sl[q].i1;
+                    }
+                }
+                const double T = bi >= 0 ? band_threshold(bc, Xup, cmax, double(best)) : 0.0;
+                bool cert = !cbad && bi >= 0 && bi < k && T < double(finf);
+                int cnt = 0;
+                int cand[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (double(sl[q].m2) <= T) cert = false;
+                    if (sl[q].i1 >= 0 && double(sl[q].m1) <= T) {
+                        if (sl[q].i1 >= k) cert = false;
+                        cand[cnt++] = sl[q].i1;
+                    }
+                }
+                if (cert && cnt == 1) {
+                    lab = bi;
+                } else if (cert) {
+                    // exact form over the candidates, ties to the lowest index
+                    float bd = finf;
+                    lab = k;
+                    for (int c = 0; c < cnt; ++c) {
+                        const int j = cand[c];
+                        const float D = canon_dist<float>(xrow, Cs + j * CST, d);
+                        if (D < bd || (D == bd && j < lab)) {
+                            bd = D;
+                            lab = j;
+                        }
+                    }
+                    if (lab >= k) lab = bi;
+                } else {
+                    need_scan = true;
+                }
+            }
+            // Exact scans, one row at a time by the whole wave: lane l takes
+            // centroids l, l + 64, ...; strict < in centroid order per lane, then
+            // the smallest value with ties to the lowest index; a NaN distance to
+            // centroid 0 keeps label 0 (the exact-form tile kernel's rule).
+            uint64_t scan_rows = __ballot(need_scan);
+            while (scan_rows) {
+                const int src = __ffsll((unsigned long long)scan_rows) - 1;  // a lane with h = 0
+                scan_rows &= scan_rows - 1;
+                const float* fx = in.X + (t0 + wave * 32 + src) * int64_t(d);
+                float bv = finf;
+                int bj = 0x7fffffff;
+                bool nan0 = false;
+                for (int j = lane; j < k; j += 64) {
+                    const float D = canon_dist<float>(fx, Cs + j * CST, d);
+                    if (j == 0 && D != D) nan0 = true;
+                    if (D < bv) {
+                        bv = D;
+                        bj = j;
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float ob = __shfl_xor(bv, off);
+                    const int oj = __shfl_xor(bj, off);
+                    if (ob < bv || (ob == bv && oj < bj)) {
+                        bv = ob;
+                        bj = oj;
+                    }
+                }
+                const bool n0 = __shfl(nan0 ? 1 : 0, 0) != 0;
+                if (lane == src) lab = (n0 || bj == 0x7fffffff) ? 0 : bj;
+            }
+            if (h == 0 && ok) {
+                if (to_user) {
+                    in.labels_out[row] = lab;
+                } else {
+                    labels[in.row0 + row] = lab;
+                    skeys[in.row0 + row] = (uint32_t(in.index) << lbits) | uint32_t(lab);
+                }
+            }
+        }
+    }
+}
+
+// Autotuner inputs: instance i's rows are its centroids (row % k) plus uniform
+// noise in [-1.75, 1.75) per feature, so that rows lie near one centroid as in
+// clustered data.
+__global__ void batch_tune_points_kernel(float* __restrict__ X, const float* __restrict__ C, int ni,
+                                         int64_t n, int d, int k, uint32_t seed) {
+    const int64_t total = int64_t(ni) * n * d;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t r = e / d;
+        const int f = int(e - r * d);
+        const int64_t inst = r / n;
+        const int64_t j = (r - inst * n) % k;
+        uint32_t h = uint32_t(e) * 0x9E3779B1u ^ uint32_t(e >> 32) ^ seed;
+        h ^= h >> 15;
+        h *= 0x85EBCA77u;
+        h ^= h >> 13;
+        X[e] = C[(inst * k + j) * d + f] + (float(h >> 8) * (3.5f / 16777216.0f) - 1.75f);
+    }
+}
+
+// Launch kinds of a plan: the VALU tiles 0..6, then the matrix-core kernel for
+// DP = 32, 64, 128 (7, 8, 9); rows per work item of each.
+constexpr int BATCH_LAUNCH_KINDS = 10;
+__host__ __device__ constexpr int batch_kind_rows(int kind) {
+    return kind < BATCH_TILE_MFMA ? batch_tile_rows(kind) : BATCH_MFMA_ROWS;
 }
 
 // synthetic finite floats in [-10, 10) from a hash of the index (autotuner inputs)

@@ -846,10 +846,19 @@ static int tile_bucket(int64_t d, int64_t k) {
 // untuned choice: the narrowest centroid tile that still holds k
 static int default_tile(int64_t k) { return k <= 8 ? 4 : k <= 16 ? 3 : k <= 32 ? 2 : 0; }
 
+static bool mfma_tile_ok(int64_t d, int64_t k) { return d <= 128 && k <= BATCH_MFMA_MAXK; }
+
+// tile variant of an instance (the matrix-core variant only where it applies)
 static int tile_for(const km_ctx* ctx, int64_t d, int64_t k) {
-    if (ctx->tile_force > 0) return ctx->tile_force - 1;
-    const int t = ctx->tile_table[tile_bucket(d, k)];
-    return t >= 0 ? t : default_tile(k);
+    int t = ctx->tile_force > 0 ? ctx->tile_force - 1 : ctx->tile_table[tile_bucket(d, k)];
+    if (t < 0 || (t == BATCH_TILE_MFMA && !mfma_tile_ok(d, k))) t = default_tile(k);
+    return t;
+}
+
+// launch kind of a tile variant: the matrix-core variant splits by padded width
+static int launch_kind(int v, int64_t d) {
+    if (v != BATCH_TILE_MFMA) return v;
+    return BATCH_TILE_MFMA + (d <= 32 ? 0 : d <= 64 ? 1 : 2);
 }
 
 static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
@@ -925,13 +934,13 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
     p.kbits = std::max(1u, lbits + ibits);
     // assignment launches: instances grouped by tile variant, in instance order
     std::vector<int> var(B);
-    for (int i = 0; i < B; ++i) var[i] = tile_for(ctx, inst[i].d, inst[i].k);
+    for (int i = 0; i < B; ++i) var[i] = launch_kind(tile_for(ctx, inst[i].d, inst[i].k), inst[i].d);
     p.asel.clear();
     p.apre.clear();
     p.launches.clear();
-    for (int v = 0; v < BATCH_TILES; ++v) {
+    for (int v = 0; v < BATCH_LAUNCH_KINDS; ++v) {
         batch_plan::asg_launch L{v, 0, int64_t(p.asel.size()), int64_t(p.apre.size()), 0};
-        const int64_t rb = batch_tile_rows(v);
+        const int64_t rb = batch_kind_rows(v);
         p.apre.push_back(0);
         for (int i = 0; i < B; ++i) {
             if (var[i] != v) continue;
@@ -990,6 +999,18 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
                           ctx->stream));
     KM_HIP(hipMemcpyAsync(p.d_apre, p.apre.data(), sizeof(int64_t) * p.apre.size(), hipMemcpyHostToDevice,
                           ctx->stream));
+    // the matrix-core kernels take more than 64 KB of LDS (set outside any capture)
+    for (const batch_plan::asg_launch& L : p.launches) {
+        if (L.v < BATCH_TILE_MFMA) continue;
+        KM_HIP(hipFuncSetAttribute((const void*)batch_assign_mfma_kernel<32>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(batch_mfma_lds_bytes(32))));
+        KM_HIP(hipFuncSetAttribute((const void*)batch_assign_mfma_kernel<64>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(batch_mfma_lds_bytes(64))));
+        KM_HIP(hipFuncSetAttribute((const void*)batch_assign_mfma_kernel<128>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(batch_mfma_lds_bytes(128))));
+        break;
+    }
 }
 
 // exact assignment of every instance of the plan, one launch per tile variant in use
@@ -1002,6 +1023,9 @@ static void plan_assign(km_ctx* ctx, const batch_plan& p, int32_t* labels, uint3
 #define KM_BATCH_ASSIGN(CT, RW)                                                                      \
     hipLaunchKernelGGL((batch_assign_kernel<CT, RW>), grid, block, 0, ctx->stream, p.dI, sel, L.nsel, pre, \
                        p.Ccur, labels, skeys, lb, to_user)
+#define KM_BATCH_MFMA(DP)                                                                          \
+    hipLaunchKernelGGL((batch_assign_mfma_kernel<DP>), grid, block, batch_mfma_lds_bytes(DP), ctx->stream, \
+                       p.dI, sel, L.nsel, pre, p.Ccur, labels, skeys, lb, to_user)
         switch (L.v) {
             case 0: KM_BATCH_ASSIGN(64, 8); break;
             case 1: KM_BATCH_ASSIGN(64, 4); break;
@@ -1009,9 +1033,13 @@ static void plan_assign(km_ctx* ctx, const batch_plan& p, int32_t* labels, uint3
             case 3: KM_BATCH_ASSIGN(16, 8); break;
             case 4: KM_BATCH_ASSIGN(8, 8); break;
             case 5: KM_BATCH_ASSIGN(32, 4); break;
-            default: KM_BATCH_ASSIGN(16, 4); break;
+            case 6: KM_BATCH_ASSIGN(16, 4); break;
+            case 7: KM_BATCH_MFMA(32); break;
+            case 8: KM_BATCH_MFMA(64); break;
+            default: KM_BATCH_MFMA(128); break;
         }
 #undef KM_BATCH_ASSIGN
+#undef KM_BATCH_MFMA
         KM_LAUNCH_CHECK();
     }
 }
@@ -1178,7 +1206,8 @@ static void batch_step(km_ctx* ctx, const km_batch_instance_f32* it, int64_t n_i
 }
 
 // Tile autotuner (km_batch_autotune): for every (d, k) bucket, synthetic
-// instances of the bucket's largest shape are assigned with every tile variant;
+// clustered instances of the bucket's largest shape (rows near one of k random
+// centroids) are assigned with every tile variant;
 // the fastest (HIP-event time, best of three after a warm-up launch) becomes the
 // context's variant for the bucket.  Every variant computes the same labels.
 static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
@@ -1204,11 +1233,11 @@ static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
             float* X = ctx->tune_x.get<float>(NI * n * d);
             float* C = ctx->tune_c.get<float>(int64_t(NI) * k * d);
             int32_t* lab = ctx->tune_l.get<int32_t>(NI * n);
-            hipLaunchKernelGGL(batch_tune_fill_kernel, dim3(batch_grid(ctx, (NI * n * d + 255) / 256)),
-                               dim3(256), 0, ctx->stream, X, NI * n * d, 0x1234u + uint32_t(db * 6 + kb));
-            KM_LAUNCH_CHECK();
             hipLaunchKernelGGL(batch_tune_fill_kernel, dim3(batch_grid(ctx, (int64_t(NI) * k * d + 255) / 256)),
                                dim3(256), 0, ctx->stream, C, int64_t(NI) * k * d, 0x9876u + uint32_t(db * 6 + kb));
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(batch_tune_points_kernel, dim3(batch_grid(ctx, (NI * n * d + 255) / 256)),
+                               dim3(256), 0, ctx->stream, X, C, NI, n, d, k, 0x1234u + uint32_t(db * 6 + kb));
             KM_LAUNCH_CHECK();
             std::vector<km_batch_instance_f32> q(NI);
             for (int i = 0; i < NI; ++i)
@@ -1216,6 +1245,7 @@ static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
             float best_ms = std::numeric_limits<float>::infinity();
             int best_v = default_tile(k);
             for (int v = 0; v < BATCH_TILES; ++v) {
+                if (v == BATCH_TILE_MFMA && !mfma_tile_ok(d, k)) continue;
                 ctx->tile_force = v + 1;
                 batch_plan p;
                 plan_build(p, ctx, q.data(), NI, false);

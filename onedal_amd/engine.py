@@ -29,7 +29,7 @@ KM_FLAG_GRAPH = 0x8
 KM_OPT_ASSIGN_PATH = 1
 KM_OPT_TIMING = 2
 KM_OPT_BATCH_TILE = 3
-KM_BATCH_TILE_VARIANTS = 7
+KM_BATCH_TILE_VARIANTS = 8
 KM_BATCH_TILE_BUCKETS = 42
 
 PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2

@@ -142,3 +142,29 @@ def test_batch_autotune_table_and_results(oracle):
             assert np.array_equal(l_out[i], wl), i
             assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), i
             assert obj[i] == wo, i
+
+
+def test_batch_matrix_core_tile_exact_on_ties(engine, oracle):
+    """The certified matrix-core variant on integer data with duplicate centroids
+    (exact ties everywhere: candidate re-score and full re-scan paths) equals the
+    canonical result; instances it does not apply to (k > 128) take a VALU tile."""
+    from onedal_amd.engine import KM_OPT_BATCH_TILE
+
+    rng = np.random.default_rng(41)
+    inst = []
+    for i, (n, d, k) in enumerate([(3000, 3, 5), (5000, 16, 40), (4000, 64, 128), (2500, 100, 77),
+                                   (1500, 128, 128), (2000, 7, 129), (1000, 32, 1)]):
+        x = rng.integers(-3, 4, size=(n, d)).astype(np.float32)
+        idx = rng.choice(n, size=max(1, k // 2), replace=False)
+        c0 = x[np.resize(idx, k)].copy()  # duplicated centroids
+        inst.append((x, c0))
+    try:
+        engine.set_option(KM_OPT_BATCH_TILE, 8)  # variant 7
+        c_out, l_out, obj, its = run_batch(engine, inst, 3, graph=False)
+    finally:
+        engine.set_option(KM_OPT_BATCH_TILE, 0)
+    for i, (x, c0) in enumerate(inst):
+        wc, wl, wo = oracle.canonical_lloyd(x, c0, 3, 0.0)[:3]
+        assert np.array_equal(l_out[i], wl), i
+        assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), i
+        assert obj[i] == wo, i
