@@ -699,7 +699,21 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
             const int g = lane / d, f0 = lane % d;
             const bool active = g < G;
             double acc = 0.0;
-            for (int64_t m = m_begin; m < m_end; m += G) {
+            int64_t m = m_begin;
+            // eight steps of G members: ids and values loaded first, added in order
+            for (; m + 8 * G <= m_end; m += 8 * G) {
+                if (active) {
+                    int64_t r[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) r[u] = int64_t(pm[m + u * G + g]) - in.row0;
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = double(in.X[r[u] * d + f0]);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = acc + v[u];
+                }
+            }
+            for (; m < m_end; m += G) {
                 const int64_t mm = m + g;
                 if (active && mm < m_end) {
                     const int64_t row = int64_t(pm[mm]) - in.row0;
@@ -717,16 +731,16 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
                 const int f = fc + lane;
                 double acc = 0.0;
                 int64_t m = m_begin;
-                for (; m + 8 <= m_end; m += 8) {
-                    int64_t r[8];
+                for (; m + 16 <= m_end; m += 16) {
+                    int64_t r[16];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) r[u] = int64_t(pm[m + u]) - in.row0;
+                    for (int u = 0; u < 16; ++u) r[u] = int64_t(pm[m + u]) - in.row0;
                     if (f < d) {
-                        double v[8];
+                        double v[16];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) v[u] = double(in.X[r[u] * d + f]);
+                        for (int u = 0; u < 16; ++u) v[u] = double(in.X[r[u] * d + f]);
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+                        for (int u = 0; u < 16; ++u) acc = acc + v[u];
                     }
                 }
                 for (; m < m_end; ++m) {
