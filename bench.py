@@ -377,7 +377,10 @@ def run_serving(torch, args, wl):
                    "large_fraction": wl["large_frac"], "fit_iterations": iters,
                    "budget_bytes": wl["budget"], "large_job": {"n": nl, "d": dl, "k": kl}},
         "latency": {"p50_ms_at_best": best["p50_ms"], "p99_ms_at_best": best["p99_ms"],
-                    "rate_at_best": best["offered_rate"]},
+                    "rate_at_best": best["offered_rate"],
+                    # the highest offered rate of the sweep still served with p99 <= 100 ms
+                    "max_rate_p99_le_100ms": max([z["offered_rate"] for z in sweep if z["p99_ms"] <= 100.0],
+                                                 default=None)},
         # peak of the lane mix: the flops each lane ran, each priced at its precision's
         # dense peak (batched lane FP32, large-job lane BF16)
         "roofline": {"bound": "mfma", "achieved": best["tflops"],

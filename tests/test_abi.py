@@ -70,3 +70,35 @@ def test_cpp_api_header_compiles_and_validates():
                     "-L", lib_dir, "-lkmeans_hip", "-Wl,-rpath," + lib_dir, "-o", exe], check=True)
     res = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert res.returncode == 0, res.stdout
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct the binding passes across the ABI has the header's size and
+    field offsets (gcc on the header itself prints them)."""
+    from onedal_amd import engine as E
+
+    structs = {"km_stats": E.KmStats, "km_batch_instance_f32": E.KmBatchInstance,
+               "km_server_config": E.KmServerConfig, "km_job_result": E.KmJobResult,
+               "km_server_stats": E.KmServerStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "onedal_amd/kmeans_hip.h"',
+             'int main(void) {']
+    for cname, cls in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for field, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, field, cname, field))
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            name, field, value = line.split()
+            got[(name, field)] = int(value)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
