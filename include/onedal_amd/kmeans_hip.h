@@ -217,7 +217,8 @@ typedef struct {
 } km_server_stats;
 
 km_status km_server_create(const km_server_config* cfg, km_server** out);
-/* Runs every submitted job to completion, then frees the server. */
+/* Runs every submitted job to completion, then frees the server.  No other
+ * call on the server may overlap it or follow it. */
 km_status km_server_destroy(km_server* server);
 /* Message of this thread's last failed km_server_* call (or failed job's wait). */
 const char* km_server_last_error(void);
