@@ -118,6 +118,29 @@ km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, 
 km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
                              int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
 
+/* Initial centroids (kmeans_init).  Methods: */
+#define KM_INIT_DENSE 0               /* the first k rows */
+#define KM_INIT_RANDOM_DENSE 1        /* k distinct rows, uniformly at random */
+#define KM_INIT_PLUS_PLUS_DENSE 2     /* k-means++: first row uniform, then D^2 sampling */
+#define KM_INIT_PARALLEL_PLUS_DENSE 3 /* not on the GPU (KM_UNIMPLEMENTED, as the reference) */
+
+/* C_out (k x d) = initial centroids of X (n x d) by `method`, drawn with the
+ * counter-based generator keyed by `seed` (plus_plus_dense: D^2 = FP64
+ * exact-order distances to the nearest chosen centroid, sampled in a fixed
+ * summation order; the result does not depend on timing).  Preconditions and
+ * messages of kmeans_init (detail/compute_ops.hpp:40-56): data non-empty,
+ * 0 < k < n.  Replaces kmeans_init's GPU compute kernels
+ * (backend/gpu/compute_kernels_impl.hpp:36-68 dense,
+ * compute_kernel_distr_random_dense_dpc.cpp:33-66 random_dense); the reference
+ * runs plus_plus_dense on the host only (train_kernel_lloyd_dense_dpc.cpp:44-74
+ * calls DAAL's k-means++ when train has no initial centroids).  Random methods
+ * match the reference in distribution, not draw for draw (its engine is DAAL's
+ * mt19937). */
+km_status km_init_centroids_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, int64_t k,
+                                int32_t method, uint64_t seed, uint32_t flags, float* C_out);
+km_status km_init_centroids_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, int64_t k,
+                                int32_t method, uint64_t seed, uint32_t flags, double* C_out);
+
 /* One instance of a batched Lloyd training; all pointers are device memory. */
 typedef struct {
     const float* X;      /* n x d row-major points */

@@ -27,6 +27,7 @@
 #include <type_traits>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/onedal_amd/kmeans_hip.h"
@@ -35,6 +36,8 @@
 #include "km_assign_bf16.hpp"
 #include "km_batch.hpp"
 #include "km_common.hpp"
+#include "km_init.hpp"
+#include "km_rng.hpp"
 #include "km_refine.hpp"
 #include "km_update.hpp"
 
@@ -118,6 +121,7 @@ struct km_ctx {
     // batched training and steps: the iterated set, and the finished set of a step
     km::batch_ws bws, bws_fin;
     km::dbuf tune_x, tune_c, tune_l;  // autotuner inputs (released after tuning)
+    km::dbuf init_d2, init_chunks, init_chosen, init_ids;  // initialisation
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -1279,6 +1283,88 @@ static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
         for (int64_t i = 0; i < std::min<int64_t>(table_len, KM_BATCH_TILE_BUCKETS); ++i) table_out[i] = table[i];
 }
 
+// ---------------------------------------------------------------------------
+// Initial centroids (km_init_centroids_*): kmeans_init's dense, random_dense and
+// plus_plus_dense methods.  Preconditions of kmeans_init/detail/compute_ops.hpp:
+// 40-56 (checked in that order).
+// ---------------------------------------------------------------------------
+template <typename F>
+static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_t k, int method,
+                           uint64_t seed, uint32_t flags, F* C_out) {
+    if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
+    if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
+    if (k >= n) throw status_error(KM_DOMAIN_ERROR, "Cluster count exceeds data row count");
+    if (C_out == nullptr) throw status_error(KM_INVALID_ARGUMENT, "Output buffers are required");
+    if (method == KM_INIT_PARALLEL_PLUS_DENSE)
+        throw status_error(KM_UNIMPLEMENTED, "K-Means init++ parallel dense method is not implemented for GPU");
+    if (method < KM_INIT_DENSE || method > KM_INIT_PLUS_PLUS_DENSE)
+        throw status_error(KM_INVALID_ARGUMENT, "unknown initialisation method");
+    if (d > std::numeric_limits<int32_t>::max())
+        throw status_error(KM_DOMAIN_ERROR, "Column count is greater than max int32 value");
+    const bool dev = (flags & KM_FLAG_DEVICE_POINTERS) != 0;
+    const F* Xd = X;
+    F* Cd = C_out;
+    if (!dev) {
+        F* xb = ctx->x_dev.get<F>(n * d);
+        KM_HIP(hipMemcpyAsync(xb, X, sizeof(F) * n * d, hipMemcpyHostToDevice, ctx->stream));
+        Xd = xb;
+        Cd = ctx->c_out_dev.get<F>(k * d);
+    }
+    const int di = int(d);
+    if (method == KM_INIT_DENSE) {
+        // the first k rows (compute_kernels_impl.hpp:53-68)
+        KM_HIP(hipMemcpyAsync(Cd, Xd, sizeof(F) * k * d, hipMemcpyDeviceToDevice, ctx->stream));
+    } else if (method == KM_INIT_RANDOM_DENSE) {
+        // k distinct rows by a partial Fisher-Yates shuffle of [0, n) on the host,
+        // gathered on the device (compute_kernel_distr_random_dense_dpc.cpp:33-66)
+        std::vector<int64_t> ids(k);
+        std::unordered_map<int64_t, int64_t> moved;
+        for (int64_t i = 0; i < k; ++i) {
+            const int64_t j = i + int64_t(rng_index(seed, RNG_STREAM_RANDOM, uint64_t(i), uint64_t(n - i)));
+            auto at = [&](int64_t p) {
+                auto it = moved.find(p);
+                return it == moved.end() ? p : it->second;
+            };
+            const int64_t vi = at(i), vj = at(j);
+            ids[i] = vj;
+            moved[j] = vi;
+        }
+        int64_t* dids = ctx->init_ids.get<int64_t>(k);
+        KM_HIP(hipMemcpyAsync(dids, ids.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(init_gather_rows_kernel<F>, dim3(grid_for(ctx, k, 1, 4)), dim3(64), 0, ctx->stream, Xd,
+                           di, dids, k, Cd);
+        KM_LAUNCH_CHECK();
+        KM_HIP(hipStreamSynchronize(ctx->stream));  // ids is read by the copy above
+    } else {
+        // k-means++ (D^2 sampling), every step on the device without host syncs
+        double* D2 = ctx->init_d2.get<double>(n);
+        const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        double* chunks = ctx->init_chunks.get<double>(nchunks);
+        int64_t* chosen = ctx->init_chosen.get<int64_t>(1);
+        const int64_t first = int64_t(rng_index(seed, RNG_STREAM_PLUS_PLUS, 0, uint64_t(n)));
+        hipLaunchKernelGGL(init_copy_row_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, Xd, di, nullptr, first,
+                           Cd, int64_t(0));
+        KM_LAUNCH_CHECK();
+        const int d2_grid = grid_for(ctx, n, 256, 8);
+        const int ch_grid = grid_for(ctx, nchunks, 1, 8);
+        for (int64_t t = 1; t < k; ++t) {
+            hipLaunchKernelGGL(init_d2_kernel<F>, dim3(d2_grid), dim3(256), 0, ctx->stream, Xd, n, di,
+                               Cd + (t - 1) * d, D2, t == 1 ? 1 : 0);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(init_chunk_sums_kernel, dim3(ch_grid), dim3(256), 0, ctx->stream, D2, n, chunks);
+            KM_LAUNCH_CHECK();
+            const double u = rng_uniform01(seed, RNG_STREAM_PLUS_PLUS, uint64_t(t));
+            hipLaunchKernelGGL(init_select_kernel, dim3(1), dim3(64), 0, ctx->stream, D2, n, chunks, u, chosen);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(init_copy_row_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, Xd, di, chosen,
+                               int64_t(0), Cd, t);
+            KM_LAUNCH_CHECK();
+        }
+    }
+    if (!dev) KM_HIP(hipMemcpyAsync(C_out, Cd, sizeof(F) * k * d, hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 template <typename Fn>
 static km_status guarded(km_ctx* ctx, Fn fn) {
     if (!ctx) return KM_INVALID_ARGUMENT;
@@ -1350,7 +1436,8 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
                         &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
-                        &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l};
+                        &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l,
+                        &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids};
     for (auto* b : bufs) b->release();
     ctx->bws.release();
     ctx->bws_fin.release();
@@ -1432,6 +1519,16 @@ km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances
     return km::guarded(ctx, [&] {
         km::batch_train(ctx, instances, count, max_iter, flags, obj_out, iters_out);
     });
+}
+
+km_status km_init_centroids_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, int64_t k,
+                                int32_t method, uint64_t seed, uint32_t flags, float* C_out) {
+    return km::guarded(ctx, [&] { km::init_centroids<float>(ctx, X, n, d, k, method, seed, flags, C_out); });
+}
+
+km_status km_init_centroids_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, int64_t k,
+                                int32_t method, uint64_t seed, uint32_t flags, double* C_out) {
+    return km::guarded(ctx, [&] { km::init_centroids<double>(ctx, X, n, d, k, method, seed, flags, C_out); });
 }
 
 km_status km_batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {

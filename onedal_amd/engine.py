@@ -26,6 +26,8 @@ KM_FLAG_BF16_REFINE = 0x2
 KM_FLAG_DEVICE_POINTERS = 0x4
 KM_FLAG_GRAPH = 0x8
 
+KM_INIT_DENSE, KM_INIT_RANDOM_DENSE, KM_INIT_PLUS_PLUS_DENSE, KM_INIT_PARALLEL_PLUS_DENSE = 0, 1, 2, 3
+
 KM_OPT_ASSIGN_PATH = 1
 KM_OPT_TIMING = 2
 KM_OPT_BATCH_TILE = 3
@@ -37,7 +39,8 @@ PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64", "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
+    "km_lloyd_infer_f64", "km_init_centroids_f32", "km_init_centroids_f64",
+    "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
     "km_server_create",
     "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",
     "km_server_submit_infer_f32", "km_server_wait", "km_server_get_stats",
@@ -155,6 +158,10 @@ def load_library():
     L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
                                      ctypes.POINTER(D), ctypes.POINTER(I64)]
     L.km_batch_train_f32.restype = I
+    for sfx in ("f32", "f64"):
+        f = getattr(L, "km_init_centroids_" + sfx)
+        f.argtypes = [P, P, I64, I64, I64, ctypes.c_int32, ctypes.c_uint64, U32, P]
+        f.restype = I
     L.km_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32), I64]
     L.km_batch_autotune.restype = I
     L.km_batch_step_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64,
@@ -333,6 +340,29 @@ class Engine:
         st = self.lib.km_batch_train_f32(self.ctx, arr, count, int(max_iter), flags, obj, its)
         self._check(st)
         return cs, ls, list(obj), list(its)
+
+    def init_centroids(self, x, k, method, seed=777):
+        """Initial centroids (KM_INIT_* method) of numpy rows or a CUDA tensor
+        (then the result is a CUDA tensor)."""
+        if _is_torch_cuda(x):
+            import torch
+
+            assert x.is_contiguous()
+            c = torch.empty((k, x.shape[1]), dtype=x.dtype, device=x.device)
+            sfx = _sfx(str(x.dtype).replace("torch.", ""))
+            st = getattr(self.lib, "km_init_centroids_" + sfx)(
+                self.ctx, x.data_ptr(), x.shape[0], x.shape[1], int(k), int(method),
+                ctypes.c_uint64(seed & (2**64 - 1)), KM_FLAG_DEVICE_POINTERS, c.data_ptr())
+            self._check(st)
+            return c
+        x = np.ascontiguousarray(x)
+        sfx = _sfx(x.dtype)
+        c = np.empty((k, x.shape[1]), x.dtype)
+        st = getattr(self.lib, "km_init_centroids_" + sfx)(
+            self.ctx, x.ctypes.data, x.shape[0], x.shape[1], int(k), int(method),
+            ctypes.c_uint64(seed & (2**64 - 1)), 0, c.ctypes.data)
+        self._check(st)
+        return c
 
     def batch_autotune(self):
         """Tune the batched assignment's tile per (d, k) bucket; returns the chosen

@@ -66,6 +66,7 @@
 #include <limits>
 #include <numeric>
 #include <thread>
+#include <map>
 #include <vector>
 
 namespace {
@@ -538,9 +539,139 @@ void fp64_assign(const F* x, int64_t n, int64_t d, const F* c, int64_t k, int32_
     });
 }
 
+// ---------------------------------------------------------------------------
+// 3. initial centroids: the engine's kmeans_init methods, restated (the reference
+//    draws from DAAL's mt19937, backend/primitives/rng/host_engine.hpp:160-185,
+//    so against the reference only the distribution can agree; the property the
+//    reference's own test checks is that the centroids are distinct data rows,
+//    kmeans_init/test/fixture.hpp:61-90)
+// ---------------------------------------------------------------------------
+// SplitMix64 finaliser (Steele, Lea, Flood, OOPSLA 2014), counter based:
+// draw i of stream s under seed q = mix(mix(q ^ s * C1) ^ i * C2)
+uint64_t init_mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+uint64_t init_draw(uint64_t seed, uint64_t stream, uint64_t i) {
+    return init_mix(init_mix(seed ^ (stream * 0xD1B54A32D192ED03ull)) ^ (i * 0x9E3779B97F4A7C15ull));
+}
+double init_unif(uint64_t seed, uint64_t stream, uint64_t i) {
+    return double(init_draw(seed, stream, i) >> 11) / 9007199254740992.0;  // top 53 bits
+}
+uint64_t init_index(uint64_t seed, uint64_t stream, uint64_t i, uint64_t m) {  // in [0, m)
+    return uint64_t((static_cast<unsigned __int128>(init_draw(seed, stream, i)) * m) >> 64);
+}
+
+// method 0: first k rows; 1: partial Fisher-Yates over [0, n) (stream 2, draw i
+// picks position i + index(n - i)); 2: k-means++ (stream 1): first row index(n),
+// then for t = 1..k-1 D^2 sampling with u = unif(t) over
+//   D2_r = min over chosen c of sum_f ((double)x_f - (double)c_f)^2 (in order),
+//   chunk sums of 4096 rows (thread t of 256 adds rows t, t+256, ... in order,
+//   then the tree 128, 64, ..., 1), chunks added in order, and the first row
+//   whose running sum (chunks, then that chunk's rows) exceeds u * total
+//   (rows with D2 = 0 are skipped; rounding fallback: the chunk's last row with
+//   D2 > 0; total = 0: row min(n - 1, floor(u * n))).
+template <typename F>
+void init_centroids(const F* x, int64_t n, int64_t d, int64_t k, int method, uint64_t seed, F* c) {
+    auto copy_row = [&](int64_t src, int64_t dst) {
+        std::copy(x + src * d, x + (src + 1) * d, c + dst * d);
+    };
+    if (method == 0) {
+        for (int64_t i = 0; i < k; ++i) copy_row(i, i);
+        return;
+    }
+    if (method == 1) {
+        std::map<int64_t, int64_t> moved;
+        auto at = [&](int64_t p) {
+            auto it = moved.find(p);
+            return it == moved.end() ? p : it->second;
+        };
+        for (int64_t i = 0; i < k; ++i) {
+            const int64_t j = i + int64_t(init_index(seed, 2, uint64_t(i), uint64_t(n - i)));
+            const int64_t vi = at(i), vj = at(j);
+            copy_row(vj, i);
+            moved[j] = vi;
+        }
+        return;
+    }
+    copy_row(int64_t(init_index(seed, 1, 0, uint64_t(n))), 0);
+    std::vector<double> D2(size_t(n), 0.0);
+    const int64_t nchunks = (n + 4095) / 4096;
+    std::vector<double> chunk(static_cast<size_t>(nchunks));
+    for (int64_t t = 1; t < k; ++t) {
+        const F* ct = c + (t - 1) * d;
+        for (int64_t r = 0; r < n; ++r) {
+            double acc = 0.0;
+            for (int64_t f = 0; f < d; ++f) {
+                const double diff = double(x[r * d + f]) - double(ct[f]);
+                const double sq = diff * diff;
+                acc = acc + sq;
+            }
+            D2[size_t(r)] = t == 1 ? acc : std::fmin(D2[size_t(r)], acc);
+        }
+        for (int64_t q = 0; q < nchunks; ++q) {
+            const int64_t r0 = q * 4096, rows = std::min<int64_t>(4096, n - r0);
+            double part[256];
+            for (int th = 0; th < 256; ++th) {
+                double p = 0.0;
+                for (int64_t r = th; r < rows; r += 256) p = p + D2[size_t(r0 + r)];
+                part[th] = p;
+            }
+            for (int st = 128; st > 0; st >>= 1)
+                for (int th = 0; th < st; ++th) part[th] = part[th] + part[th + st];
+            chunk[size_t(q)] = part[0];
+        }
+        double total = 0.0;
+        for (int64_t q = 0; q < nchunks; ++q) total = total + chunk[size_t(q)];
+        const double u = init_unif(seed, 1, uint64_t(t));
+        int64_t idx;
+        if (!(total > 0.0)) {
+            const double pos = u * double(n);
+            idx = pos >= double(n - 1) ? n - 1 : int64_t(pos);
+        } else {
+            const double target = u * total;
+            double cum = 0.0;
+            int64_t q = 0;
+            for (; q + 1 < nchunks; ++q) {
+                if (cum + chunk[size_t(q)] > target) break;
+                cum = cum + chunk[size_t(q)];
+            }
+            const int64_t r0 = q * 4096, r1 = std::min<int64_t>(n, r0 + 4096);
+            idx = -1;
+            int64_t last_pos = -1;
+            double sacc = cum;
+            for (int64_t r = r0; r < r1; ++r) {
+                const double w = D2[size_t(r)];
+                if (w > 0.0) last_pos = r;
+                sacc = sacc + w;
+                if (w > 0.0 && sacc > target) {
+                    idx = r;
+                    break;
+                }
+            }
+            if (idx < 0) idx = last_pos >= 0 ? last_pos : r0;
+        }
+        copy_row(idx, t);
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t oracle_init_mix(uint64_t z) { return init_mix(z); }
+
+void oracle_init_centroids_f32(const float* x, int64_t n, int64_t d, int64_t k, int method,
+                               uint64_t seed, float* c) {
+    init_centroids<float>(x, n, d, k, method, seed, c);
+}
+
+void oracle_init_centroids_f64(const double* x, int64_t n, int64_t d, int64_t k, int method,
+                               uint64_t seed, double* c) {
+    init_centroids<double>(x, n, d, k, method, seed, c);
+}
 
 int oracle_daal_lloyd_f32(const float* x, int64_t n, int64_t d, const float* c0, int64_t k,
                           int64_t max_iter, double thr, int stop_rule, int64_t block_size,

@@ -48,6 +48,12 @@ def _declare(L):
     D = ctypes.c_double
     I = ctypes.c_int
     for sfx in ("f32", "f64"):
+        f = getattr(L, "oracle_init_centroids_" + sfx)
+        f.argtypes = [P, I64, I64, I64, I, ctypes.c_uint64, P]
+        f.restype = None
+    L.oracle_init_mix.argtypes = [ctypes.c_uint64]
+    L.oracle_init_mix.restype = ctypes.c_uint64
+    for sfx in ("f32", "f64"):
         f = getattr(L, "oracle_daal_lloyd_" + sfx)
         f.argtypes = [P, I64, I64, P, I64, I64, D, I, I64, I, P, P, P, P]
         f.restype = I
@@ -66,6 +72,17 @@ def _declare(L):
         f = getattr(L, "oracle_fp64_assign_" + sfx)
         f.argtypes = [P, I64, I64, P, I64, P, P, P, I]
         f.restype = None
+
+
+def init_centroids(x, k, method, seed):
+    """The engine's kmeans_init methods restated (0 dense, 1 random_dense,
+    2 plus_plus_dense)."""
+    x = np.ascontiguousarray(x)
+    n, d = x.shape
+    c = np.empty((k, d), x.dtype)
+    f = getattr(lib(), "oracle_init_centroids_" + _sfx(x))
+    f(_ptr(x), n, d, k, int(method), ctypes.c_uint64(seed & (2**64 - 1)), _ptr(c))
+    return c
 
 
 def _sfx(x):
