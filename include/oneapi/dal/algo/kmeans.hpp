@@ -400,12 +400,24 @@ train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
     const table& xt = in.get_data();
     const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
     const std::int64_t k = desc.get_cluster_count();
-    if (!in.get_initial_centroids().has_data())
-        throw unimplemented(
-            "Initial centroids are required: the default k-means++ initialisation is not part "
-            "of this engine");
     const auto x = row_accessor<const Float>(xt).pull();
-    const auto c0 = row_accessor<const Float>(in.get_initial_centroids()).pull();
+    array<Float> c0;
+    if (in.get_initial_centroids().has_data()) {
+        c0 = row_accessor<const Float>(in.get_initial_centroids()).pull();
+    } else {
+        // The reference draws them with DAAL's k-means++ on the host, seed 777
+        // (train_kernel_lloyd_dense_dpc.cpp:44-74, detail/train_init_centroids.hpp:47-74);
+        // here k-means++ runs on the device (km_init_centroids_*, same seed).
+        c0 = array<Float>(k * d);
+        km_status ist;
+        if constexpr (std::is_same_v<Float, float>)
+            ist = km_init_centroids_f32(q.get_context(), x.get_data(), n, d, k, KM_INIT_PLUS_PLUS_DENSE,
+                                        777u, 0u, c0.get_mutable_data());
+        else
+            ist = km_init_centroids_f64(q.get_context(), x.get_data(), n, d, k, KM_INIT_PLUS_PLUS_DENSE,
+                                        777u, 0u, c0.get_mutable_data());
+        hip::check(ist, q.get_context());
+    }
     array<Float> c(k * d);
     array<std::int32_t> labels(n);
     double obj = 0.0;

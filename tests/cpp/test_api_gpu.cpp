@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "oneapi/dal/algo/kmeans.hpp"
+#include "oneapi/dal/algo/kmeans_init.hpp"
 
 namespace dal = oneapi::dal;
 namespace km = oneapi::dal::kmeans;
@@ -94,9 +95,65 @@ static void smoke_infer(dal::hip::queue& q) {
     for (int i = 0; i < 8; ++i) CHECK(r[i] == wr[i]);
 }
 
+// kmeans_init through dal::compute: the reference's test data and check
+// (kmeans_init/test/batch.cpp:40-56, fixture.hpp:61-90: k distinct data rows),
+// dense = first rows, and k-means train without initial centroids equal to train
+// from dal::compute<plus_plus_dense> with the same seed.
+template <typename F, typename M>
+static void init_fixture(dal::hip::queue& q, bool first_rows) {
+    namespace ki = oneapi::dal::kmeans_init;
+    const F data[] = {1.0, 1.0, 2.0, 2.0, 1.0, 2.0, 2.0, 1.0, -1.0, -1.0, -1.0, -2.0, -2.0, -1.0, -2.0, -2.0};
+    const auto x = dal::homogen_table::wrap(data, 8, 2);
+    for (int seed = 0; seed < 16; ++seed) {
+        const auto r = dal::compute(q, ki::descriptor<F, M>(4).set_seed(seed), x);
+        const auto c = dal::row_accessor<const F>(r.get_centroids()).pull();
+        CHECK(r.get_centroids().get_row_count() == 4 && r.get_centroids().get_column_count() == 2);
+        bool used[8] = {};
+        for (int i = 0; i < 4; ++i) {
+            int hit = -1;
+            for (int j = 0; j < 8 && hit < 0; ++j)
+                if (!used[j] && data[2 * j] == c[2 * i] && data[2 * j + 1] == c[2 * i + 1]) hit = j;
+            CHECK(hit >= 0);
+            if (hit >= 0) used[hit] = true;
+            if (first_rows) CHECK(hit == i);
+        }
+    }
+}
+
+static void init_checks(dal::hip::queue& q) {
+    namespace ki = oneapi::dal::kmeans_init;
+    init_fixture<float, ki::method::dense>(q, true);
+    init_fixture<double, ki::method::dense>(q, true);
+    init_fixture<float, ki::method::random_dense>(q, false);
+    init_fixture<double, ki::method::plus_plus_dense>(q, false);
+    init_fixture<float, ki::method::plus_plus_dense>(q, false);
+    float data[16] = {0};
+    bool threw = false;
+    try {
+        dal::compute(q, ki::descriptor<float, ki::method::parallel_plus_dense>(3),
+                     dal::homogen_table::wrap(data, 8, 2));
+    } catch (const dal::unimplemented&) {
+        threw = true;
+    }
+    CHECK(threw);
+    // train without initial centroids == train from compute<plus_plus_dense>(seed 777)
+    std::vector<float> x(3000 * 4);
+    for (size_t i = 0; i < x.size(); ++i) x[i] = float((i * 2654435761u) % 1000) / 100.0f + float(i % 4 == 0 ? (i / 400) % 5 : 0);
+    const auto xt = dal::homogen_table::wrap(x.data(), 3000, 4);
+    const auto desc = km::descriptor<float>(6).set_max_iteration_count(8);
+    const auto r1 = dal::train(q, desc, xt);
+    const auto c0 = dal::compute(q, ki::descriptor<float, ki::method::plus_plus_dense>(6), xt).get_centroids();
+    const auto r2 = dal::train(q, desc, xt, c0);
+    const auto a = dal::row_accessor<const float>(r1.get_model().get_centroids()).pull();
+    const auto b = dal::row_accessor<const float>(r2.get_model().get_centroids()).pull();
+    for (int i = 0; i < 24; ++i) CHECK(a[i] == b[i]);
+    CHECK(r1.get_objective_function_value() == r2.get_objective_function_value());
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     dal::hip::queue q(0);
+    init_checks(q);
     gold<float>(argv[1], q);
     gold<double>(argv[1], q);
     empty_clusters<float>(q);

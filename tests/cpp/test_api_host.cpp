@@ -8,6 +8,7 @@
 #include <string>
 
 #include "oneapi/dal/algo/kmeans.hpp"
+#include "oneapi/dal/algo/kmeans_init.hpp"
 
 namespace dal = oneapi::dal;
 namespace km = oneapi::dal::kmeans;
@@ -113,6 +114,25 @@ int main() {
     auto m = km::model<>().set_centroids(dal::homogen_table::wrap(c3, 3, 2));
     const auto m_copy = m;
     CHECK(m_copy.get_cluster_count() == 3);
+
+    // kmeans_init descriptor (kmeans_init/common.cpp:20-68): defaults and setters
+    namespace ki = oneapi::dal::kmeans_init;
+    ki::descriptor<float, ki::method::plus_plus_dense> kd;
+    CHECK(kd.get_cluster_count() == 2 && kd.get_seed() == 777 && kd.get_local_trials_count() == -1);
+    EXPECT_THROW(dal::domain_error, kd.set_cluster_count(0), cluster_count_leq_zero());
+    EXPECT_THROW(dal::domain_error, kd.set_local_trials_count(0), cluster_count_leq_zero());
+    EXPECT_THROW(dal::domain_error, ki::descriptor<double>(-3), cluster_count_leq_zero());
+    kd.set_cluster_count(4).set_seed(-5).set_local_trials_count(1);
+    CHECK(kd.get_cluster_count() == 4 && kd.get_seed() == -5 && kd.get_local_trials_count() == 1);
+    // preconditions run before any device call (detail/compute_ops.hpp:40-56)
+    EXPECT_THROW(dal::domain_error, dal::kmeans_init::detail::check_compute_preconditions(
+                     ki::descriptor<float>(3), ki::compute_input<>(dal::table{})), input_data_is_empty());
+    float few[6] = {1, 2, 3, 4, 5, 6};
+    EXPECT_THROW(dal::domain_error, dal::kmeans_init::detail::check_compute_preconditions(
+                     ki::descriptor<float>(3), ki::compute_input<>(dal::homogen_table::wrap(few, 3, 2))),
+                 cluster_count_exceeds_data_row_count());
+    dal::kmeans_init::detail::check_compute_preconditions(
+        ki::descriptor<float>(2), ki::compute_input<>(dal::homogen_table::wrap(few, 3, 2)));
 
     if (failures) {
         std::printf("%d failure(s)\n", failures);
