@@ -62,6 +62,12 @@ SERVING = {"serving_c5": dict(rates=(50, 100, 200, 400, 1000, 2000), window_s=2.
                               pool_small=96, pool_large=2, large=(5_000_000, 128, 512), n_lo=1e3,
                               n_hi=1e6, dims=(8, 16, 32, 64, 128), k_lo=4, k_hi=128)}
 
+# Initial centroids: k-means++ (plus_plus_dense) on the FP32 Lloyd workload's data
+# (n=1e7, d=64, k=256, 256 Gaussians); one step = one full initialisation; the unit
+# is point-steps/s = n * (k - 1) / time (every step reads all rows once).
+INIT = {"init_pp_c2": dict(n=10_000_000, d=64, k=256, method=2, center_range=20.0, sigma=(0.5, 2.0),
+                           cpu_rows=200_000)}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -229,6 +235,68 @@ def _pct(v, q):
         return None
     v = sorted(v)
     return v[min(len(v) - 1, int(math.ceil(q / 100.0 * len(v))) - 1)]
+
+
+def run_init(torch, args, wl):
+    from onedal_amd.engine import Engine
+
+    n, d, k = wl["n"], wl["d"], wl["k"]
+    dev = torch.device("cuda:0")
+    log("generating %s: n=%d d=%d k=%d" % (args.workload, n, d, k))
+    x, _ = make_data(torch, n, d, k, wl["center_range"], wl["sigma"], False, 0, dev)
+    torch.cuda.synchronize()
+    eng = Engine(0)
+    for w in range(args.warmup):
+        eng.init_centroids(x, k, wl["method"], 777)
+        torch.cuda.synchronize()
+        log("warmup %d done" % w)
+    times = []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        c = eng.init_centroids(x, k, wl["method"], 777 + s)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log("step %d: %.3f s" % (s, times[-1]))
+    elapsed = sum(times)
+    steps_per_s = n * (k - 1) * args.steps / elapsed
+    # every k-means++ step reads X once (n*d*4 B) and reads and writes D2 (16 B per row)
+    bytes_per_step = n * d * 4 + 16 * n
+    gbs = bytes_per_step * (k - 1) * args.steps / elapsed / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        from oracle import oracle as O
+
+        rows = wl["cpu_rows"]
+        xs = x[:rows].cpu().numpy()
+        t0 = time.perf_counter()
+        O.init_centroids(xs, k, wl["method"], 777)
+        dt = time.perf_counter() - t0
+        cpu = {"value": rows * (k - 1) / dt, "unit": "point-steps/s", "cores": 1, "kind": "port",
+               "sample": "first %d rows, full k-means++ (k=%d), oracle restatement, one thread, %.1f s"
+                         % (rows, k, dt)}
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 data, f64 distances",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "d": d, "k": k, "method": "plus_plus_dense"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                     "frac": round(gbs / PEAK["hbm_gbs"], 4), "traffic": None,
+                     "kernel": "whole k-means++ step (D2 update + chunk sums + pick): n*d*4 + 16n B "
+                               "per step over the whole call time"},
+        "cpu_baseline": cpu,
+        "centroid_checksum": float(c.double().sum().item()),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_serving(torch, args, wl):
@@ -400,7 +468,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="precision_c3",
-                    choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING))
+                    choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + sorted(INIT))
     ap.add_argument("--rates", default=None,
                     help="serving workloads: comma-separated offered rates in jobs/s")
     ap.add_argument("--instances", type=int, default=None,
@@ -427,6 +495,8 @@ def main():
 
     if args.workload in BATCHED:
         return run_batched(torch, args, BATCHED[args.workload])
+    if args.workload in INIT:
+        return run_init(torch, args, INIT[args.workload])
     if args.workload in SERVING:
         return run_serving(torch, args, SERVING[args.workload])
     wl = WORKLOADS[args.workload]

@@ -1354,7 +1354,7 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
             hipLaunchKernelGGL(init_chunk_sums_kernel, dim3(ch_grid), dim3(256), 0, ctx->stream, D2, n, chunks);
             KM_LAUNCH_CHECK();
             const double u = rng_uniform01(seed, RNG_STREAM_PLUS_PLUS, uint64_t(t));
-            hipLaunchKernelGGL(init_select_kernel, dim3(1), dim3(64), 0, ctx->stream, D2, n, chunks, u, chosen);
+            hipLaunchKernelGGL(init_select_kernel, dim3(1), dim3(256), 0, ctx->stream, D2, n, chunks, u, chosen);
             KM_LAUNCH_CHECK();
             hipLaunchKernelGGL(init_copy_row_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, Xd, di, chosen,
                                int64_t(0), Cd, t);

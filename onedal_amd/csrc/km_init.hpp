@@ -18,6 +18,9 @@
 //          D2 > 0; if total is 0, row min(n - 1, floor(u * n)).
 #pragma once
 
+#include <cstdint>
+#include <type_traits>
+
 #include "km_common.hpp"
 
 namespace km {
@@ -33,15 +36,32 @@ __global__ __launch_bounds__(256) void init_d2_kernel(const F* __restrict__ X, i
                                                       int first) {
     __shared__ F Xs[256][INIT_FT + 1];
     const int t = threadIdx.x;
+    // float rows with d % 4 == 0 and a 16-byte aligned base load as 4-vectors
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const bool vec = std::is_same<F, float>::value && d % 4 == 0 &&
+                     reinterpret_cast<uintptr_t>(X) % 16 == 0;
     for (int64_t r0 = int64_t(blockIdx.x) * 256; r0 < n; r0 += int64_t(gridDim.x) * 256) {
         const int nr = int(min<int64_t>(256, n - r0));
         double acc = 0.0;
         for (int f0 = 0; f0 < d; f0 += INIT_FT) {
             const int ft = min(INIT_FT, d - f0);
             __syncthreads();
-            for (int e = t; e < nr * ft; e += 256) {
-                const int r = e / ft, q = e - r * ft;
-                Xs[r][q] = X[(r0 + r) * d + f0 + q];
+            if (vec) {
+                const int nv = ft / 4;
+                for (int e = t; e < nr * nv; e += 256) {
+                    const int r = e / nv, q = e - r * nv;
+                    const F4 v = *reinterpret_cast<const F4*>(reinterpret_cast<const float*>(X) + (r0 + r) * d +
+                                                              f0 + 4 * q);
+                    Xs[r][4 * q] = F(v[0]);
+                    Xs[r][4 * q + 1] = F(v[1]);
+                    Xs[r][4 * q + 2] = F(v[2]);
+                    Xs[r][4 * q + 3] = F(v[3]);
+                }
+            } else {
+                for (int e = t; e < nr * ft; e += 256) {
+                    const int r = e / ft, q = e - r * ft;
+                    Xs[r][q] = X[(r0 + r) * d + f0 + q];
+                }
             }
             __syncthreads();
             if (t < nr) {
@@ -81,43 +101,108 @@ __global__ __launch_bounds__(256) void init_chunk_sums_kernel(const double* __re
     }
 }
 
-// the sampled row (one thread): see the file comment
-__global__ void init_select_kernel(const double* __restrict__ D2, int64_t n,
-                                   const double* __restrict__ chunk_sums, double u,
-                                   int64_t* __restrict__ chosen) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The sampled row (one workgroup of 256): see the file comment.  The sequential
+// walks are done by thread 0 over LDS tiles of 4096 values that the whole
+// workgroup loads, so the order of every sum is the sequential one.
+__global__ __launch_bounds__(256) void init_select_kernel(const double* __restrict__ D2, int64_t n,
+                                                         const double* __restrict__ chunk_sums, double u,
+                                                         int64_t* __restrict__ chosen) {
+    constexpr int TILE = 4096;
+    __shared__ double tile[TILE];
+    __shared__ double sh_total, sh_cum;
+    __shared__ int64_t sh_q;
+    const int t = threadIdx.x;
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+    // total = chunk sums added in order
     double total = 0.0;
-    for (int64_t q = 0; q < nchunks; ++q) total = total + chunk_sums[q];
-    int64_t idx;
-    if (!(total > 0.0)) {
-        const double pos = u * double(n);
-        idx = pos >= double(n - 1) ? n - 1 : int64_t(pos);
-    } else {
-        const double target = u * total;
-        double cum = 0.0;
-        int64_t q = 0;
-        for (; q + 1 < nchunks; ++q) {
-            if (cum + chunk_sums[q] > target) break;
-            cum = cum + chunk_sums[q];
+    for (int64_t b = 0; b < nchunks; b += TILE) {
+        const int m = int(min<int64_t>(TILE, nchunks - b));
+        __syncthreads();
+        for (int i = t; i < m; i += 256) tile[i] = chunk_sums[b + i];
+        __syncthreads();
+        if (t == 0) {
+            // sixteen values in registers, then their adds in order
+            int i = 0;
+            for (; i + 16 <= m; i += 16) {
+                double v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) v[j] = tile[i + j];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) total = total + v[j];
+            }
+            for (; i < m; ++i) total = total + tile[i];
         }
-        const int64_t r0 = q * OBJ_CHUNK;
-        const int64_t r1 = min<int64_t>(n, r0 + OBJ_CHUNK);
-        idx = -1;
-        int64_t last_pos = -1;
+    }
+    if (t == 0) sh_total = total;
+    __syncthreads();
+    total = sh_total;
+    if (!(total > 0.0)) {
+        if (t == 0) {
+            const double pos = u * double(n);
+            chosen[0] = pos >= double(n - 1) ? n - 1 : int64_t(pos);
+        }
+        return;
+    }
+    const double target = u * total;
+    // chunk q: the first whose running sum exceeds target (the last chunk otherwise)
+    double cum = 0.0;
+    int64_t qsel = -1;
+    for (int64_t b = 0; b < nchunks; b += TILE) {
+        const int m = int(min<int64_t>(TILE, nchunks - b));
+        __syncthreads();
+        for (int i = t; i < m; i += 256) tile[i] = chunk_sums[b + i];
+        __syncthreads();
+        if (t == 0) {
+            for (int i0 = 0; i0 < m && qsel < 0; i0 += 16) {
+                double v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) v[j] = i0 + j < m ? tile[i0 + j] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int64_t q = b + i0 + j;
+                    if (qsel < 0 && i0 + j < m) {
+                        if (q == nchunks - 1 || cum + v[j] > target)
+                            qsel = q;
+                        else
+                            cum = cum + v[j];
+                    }
+                }
+            }
+        }
+    }
+    if (t == 0) {
+        sh_q = qsel;
+        sh_cum = cum;
+    }
+    __syncthreads();
+    qsel = sh_q;
+    cum = sh_cum;
+    // the row inside chunk qsel
+    const int64_t r0 = qsel * OBJ_CHUNK;
+    const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
+    static_assert(OBJ_CHUNK <= TILE, "one chunk fits the tile");
+    for (int i = t; i < rows; i += 256) tile[i] = D2[r0 + i];
+    __syncthreads();
+    if (t == 0) {
+        int64_t idx = -1, last_pos = -1;
         double s = cum;
-        for (int64_t r = r0; r < r1; ++r) {
-            const double w = D2[r];
-            if (w > 0.0) last_pos = r;
-            s = s + w;
-            if (w > 0.0 && s > target) {
-                idx = r;
-                break;
+        for (int i0 = 0; i0 < rows && idx < 0; i0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = i0 + j < rows ? tile[i0 + j] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (idx < 0 && i0 + j < rows) {
+                    const double w = v[j];
+                    if (w > 0.0) last_pos = r0 + i0 + j;
+                    s = s + w;
+                    if (w > 0.0 && s > target) idx = r0 + i0 + j;
+                }
             }
         }
         if (idx < 0) idx = last_pos >= 0 ? last_pos : r0;
+        chosen[0] = idx;
     }
-    chosen[0] = idx;
 }
 
 // C[row_out] = X[*chosen] (or X[fixed] when chosen is null)
