@@ -1305,8 +1305,10 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
     const F* Xd = X;
     F* Cd = C_out;
     if (!dev) {
-        F* xb = ctx->x_dev.get<F>(n * d);
-        KM_HIP(hipMemcpyAsync(xb, X, sizeof(F) * n * d, hipMemcpyHostToDevice, ctx->stream));
+        // dense reads only the first k rows
+        const int64_t up_rows = method == KM_INIT_DENSE ? k : n;
+        F* xb = ctx->x_dev.get<F>(up_rows * d);
+        KM_HIP(hipMemcpyAsync(xb, X, sizeof(F) * up_rows * d, hipMemcpyHostToDevice, ctx->stream));
         Xd = xb;
         Cd = ctx->c_out_dev.get<F>(k * d);
     }
