@@ -14,8 +14,11 @@
 //
 // Where the reference takes a sycl::queue, this surface takes a
 // oneapi::dal::hip::queue (one HIP device + stream); the overloads without a
-// queue use device 0.  Tables are host tables (homogen_table::wrap); device
-// pointers go through the C ABI directly (KM_FLAG_DEVICE_POINTERS).
+// queue use device 0.  Tables hold host data (homogen_table::wrap(pointer, ...))
+// or device data (homogen_table::wrap(queue, device_pointer, ...), the
+// reference's USM tables).  With device data train and infer keep every buffer
+// on the device (KM_FLAG_DEVICE_POINTERS) and return device tables, read on the
+// host with row_accessor<const T>(table).pull(queue).
 // Extension (no reference counterpart): hip::queue::set_bf16_refine(true)
 // selects BF16 matrix-core distances with exact float re-score for float
 // descriptors; results are identical to the default path.
@@ -391,6 +394,68 @@ void check_infer_preconditions(const Desc& desc, const infer_input<>& in) {
         throw invalid_argument(input_model_centroids_cc_neq_input_data_cc());
 }
 
+// typed entry points of the C ABI
+template <typename F>
+km_status abi_train(km_ctx* ctx, const F* X, std::int64_t n, std::int64_t d, const F* C0, std::int64_t k,
+                    std::int64_t max_iter, double thr, std::uint32_t flags, F* C, std::int32_t* L, double* obj,
+                    std::int64_t* iters) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_lloyd_train_f32(ctx, X, n, d, C0, k, max_iter, thr, flags, C, L, obj, iters);
+    else
+        return km_lloyd_train_f64(ctx, X, n, d, C0, k, max_iter, thr, flags, C, L, obj, iters);
+}
+
+template <typename F>
+km_status abi_infer(km_ctx* ctx, const F* X, std::int64_t n, std::int64_t d, const F* C, std::int64_t k,
+                    std::uint32_t flags, std::int32_t* L, double* obj) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_lloyd_infer_f32(ctx, X, n, d, C, k, flags, L, obj);
+    else
+        return km_lloyd_infer_f64(ctx, X, n, d, C, k, flags, L, obj);
+}
+
+// Default initial centroids when train gets none.  The reference draws them with
+// DAAL's k-means++ on the host, seed 777 (train_kernel_lloyd_dense_dpc.cpp:44-74,
+// detail/train_init_centroids.hpp:47-74); here k-means++ runs on the device with
+// the same seed.
+template <typename F>
+km_status abi_init_plus_plus(km_ctx* ctx, const F* X, std::int64_t n, std::int64_t d, std::int64_t k,
+                             std::uint32_t flags, F* C) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_init_centroids_f32(ctx, X, n, d, k, KM_INIT_PLUS_PLUS_DENSE, 777u, flags, C);
+    else
+        return km_init_centroids_f64(ctx, X, n, d, k, KM_INIT_PLUS_PLUS_DENSE, 777u, flags, C);
+}
+
+// Device memory of `count` T on the queue's device, freed with the last owner
+// (the owner keeps the queue's context alive).
+template <typename T>
+std::shared_ptr<T> device_array(const hip::queue& q, std::int64_t count) {
+    void* p = nullptr;
+    hip::check(km_device_alloc(q.get_context(), std::int64_t(sizeof(T)) * count, &p), q.get_context());
+    const hip::queue keep = q;
+    return std::shared_ptr<T>(static_cast<T*>(p), [keep](T* ptr) { km_device_free(keep.get_context(), ptr); });
+}
+
+// A table's rows as Float in device memory: device tables of that element type
+// are used in place, other tables are uploaded.
+template <typename Float>
+std::shared_ptr<const Float> to_device(const hip::queue& q, const table& t) {
+    if (t.is_device()) {
+        if (t.get_device() != q.get_device())
+            throw invalid_argument("The queue's device does not hold the table");
+        if (t.get_data_type() != oneapi::dal::detail::data_type_of<Float>())
+            throw invalid_argument("A device table's element type must be the descriptor's float type");
+        return std::shared_ptr<const Float>(static_cast<const Float*>(t.get_raw_data()), [](const Float*) {});
+    }
+    const auto h = row_accessor<const Float>(t).pull();
+    auto dptr = device_array<Float>(q, h.get_count());
+    hip::check(km_memcpy(q.get_context(), dptr.get(), h.get_data(), std::int64_t(sizeof(Float)) * h.get_count(),
+                         KM_COPY_HOST_TO_DEVICE),
+               q.get_context());
+    return dptr;
+}
+
 template <typename Float, typename Method>
 train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
                         const train_input<>& in) {
@@ -400,42 +465,49 @@ train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
     const table& xt = in.get_data();
     const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
     const std::int64_t k = desc.get_cluster_count();
-    const auto x = row_accessor<const Float>(xt).pull();
-    array<Float> c0;
-    if (in.get_initial_centroids().has_data()) {
-        c0 = row_accessor<const Float>(in.get_initial_centroids()).pull();
-    } else {
-        // The reference draws them with DAAL's k-means++ on the host, seed 777
-        // (train_kernel_lloyd_dense_dpc.cpp:44-74, detail/train_init_centroids.hpp:47-74);
-        // here k-means++ runs on the device (km_init_centroids_*, same seed).
-        c0 = array<Float>(k * d);
-        km_status ist;
-        if constexpr (std::is_same_v<Float, float>)
-            ist = km_init_centroids_f32(q.get_context(), x.get_data(), n, d, k, KM_INIT_PLUS_PLUS_DENSE,
-                                        777u, 0u, c0.get_mutable_data());
-        else
-            ist = km_init_centroids_f64(q.get_context(), x.get_data(), n, d, k, KM_INIT_PLUS_PLUS_DENSE,
-                                        777u, 0u, c0.get_mutable_data());
-        hip::check(ist, q.get_context());
-    }
-    array<Float> c(k * d);
-    array<std::int32_t> labels(n);
+    km_ctx* ctx = q.get_context();
+    const bool has_c0 = in.get_initial_centroids().has_data();
     double obj = 0.0;
     std::int64_t iters = 0;
-    const std::uint32_t flags = q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u;
-    km_status st;
-    if constexpr (std::is_same_v<Float, float>)
-        st = km_lloyd_train_f32(q.get_context(), x.get_data(), n, d, c0.get_data(), k,
-                                desc.get_max_iteration_count(), desc.get_accuracy_threshold(),
-                                flags, c.get_mutable_data(), labels.get_mutable_data(), &obj, &iters);
-    else
-        st = km_lloyd_train_f64(q.get_context(), x.get_data(), n, d, c0.get_data(), k,
-                                desc.get_max_iteration_count(), desc.get_accuracy_threshold(),
-                                flags, c.get_mutable_data(), labels.get_mutable_data(), &obj, &iters);
-    hip::check(st, q.get_context());
     train_result<> r;
-    r.set_model(model<>().set_centroids(homogen_table::wrap(c, k, d)));
-    r.set_responses(homogen_table::wrap(labels, n, 1));
+    if (xt.is_device()) {
+        // device data: every buffer stays on the device, results are device tables
+        const std::uint32_t flags = KM_FLAG_DEVICE_POINTERS | (q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u);
+        const auto x = to_device<Float>(q, xt);
+        std::shared_ptr<const Float> c0;
+        if (has_c0) {
+            c0 = to_device<Float>(q, in.get_initial_centroids());
+        } else {
+            auto ci = device_array<Float>(q, k * d);
+            hip::check(abi_init_plus_plus<Float>(ctx, x.get(), n, d, k, KM_FLAG_DEVICE_POINTERS, ci.get()), ctx);
+            c0 = ci;
+        }
+        auto c = device_array<Float>(q, k * d);
+        auto labels = device_array<std::int32_t>(q, n);
+        hip::check(abi_train<Float>(ctx, x.get(), n, d, c0.get(), k, desc.get_max_iteration_count(),
+                                    desc.get_accuracy_threshold(), flags, c.get(), labels.get(), &obj, &iters),
+                   ctx);
+        r.set_model(model<>().set_centroids(homogen_table::wrap_device<Float>(c, k, d, q.get_device())));
+        r.set_responses(homogen_table::wrap_device<std::int32_t>(labels, n, 1, q.get_device()));
+    } else {
+        const std::uint32_t flags = q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u;
+        const auto x = row_accessor<const Float>(xt).pull();
+        array<Float> c0;
+        if (has_c0) {
+            c0 = row_accessor<const Float>(in.get_initial_centroids()).pull(q);
+        } else {
+            c0 = array<Float>(k * d);
+            hip::check(abi_init_plus_plus<Float>(ctx, x.get_data(), n, d, k, 0u, c0.get_mutable_data()), ctx);
+        }
+        array<Float> c(k * d);
+        array<std::int32_t> labels(n);
+        hip::check(abi_train<Float>(ctx, x.get_data(), n, d, c0.get_data(), k, desc.get_max_iteration_count(),
+                                    desc.get_accuracy_threshold(), flags, c.get_mutable_data(),
+                                    labels.get_mutable_data(), &obj, &iters),
+                   ctx);
+        r.set_model(model<>().set_centroids(homogen_table::wrap(c, k, d)));
+        r.set_responses(homogen_table::wrap(labels, n, 1));
+    }
     r.set_iteration_count(iters);
     r.set_objective_function_value(obj);
     return r;
@@ -450,24 +522,27 @@ infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
     const table& xt = in.get_data();
     const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
     const std::int64_t k = desc.get_cluster_count();
-    const auto x = row_accessor<const Float>(xt).pull();
-    const auto c = row_accessor<const Float>(in.get_model().get_centroids()).pull();
+    km_ctx* ctx = q.get_context();
     const bool want_obj = desc.get_result_options().test(result_options::compute_exact_objective_function);
-    array<std::int32_t> labels(n);
+    std::uint32_t flags = (want_obj ? KM_FLAG_EXACT_OBJECTIVE : 0u) | (q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u);
     double obj = 0.0;
-    const std::uint32_t flags = (want_obj ? KM_FLAG_EXACT_OBJECTIVE : 0u) |
-                                (q.get_bf16_refine() ? KM_FLAG_BF16_REFINE : 0u);
-    km_status st;
-    if constexpr (std::is_same_v<Float, float>)
-        st = km_lloyd_infer_f32(q.get_context(), x.get_data(), n, d, c.get_data(), k, flags,
-                                labels.get_mutable_data(), &obj);
-    else
-        st = km_lloyd_infer_f64(q.get_context(), x.get_data(), n, d, c.get_data(), k, flags,
-                                labels.get_mutable_data(), &obj);
-    hip::check(st, q.get_context());
     infer_result<> r;
     r.set_result_options(desc.get_result_options());
-    r.set_responses(homogen_table::wrap(labels, n, 1));
+    if (xt.is_device()) {
+        flags |= KM_FLAG_DEVICE_POINTERS;
+        const auto x = to_device<Float>(q, xt);
+        const auto c = to_device<Float>(q, in.get_model().get_centroids());
+        auto labels = device_array<std::int32_t>(q, n);
+        hip::check(abi_infer<Float>(ctx, x.get(), n, d, c.get(), k, flags, labels.get(), &obj), ctx);
+        r.set_responses(homogen_table::wrap_device<std::int32_t>(labels, n, 1, q.get_device()));
+    } else {
+        const auto x = row_accessor<const Float>(xt).pull();
+        const auto c = row_accessor<const Float>(in.get_model().get_centroids()).pull(q);
+        array<std::int32_t> labels(n);
+        hip::check(abi_infer<Float>(ctx, x.get_data(), n, d, c.get_data(), k, flags, labels.get_mutable_data(), &obj),
+                   ctx);
+        r.set_responses(homogen_table::wrap(labels, n, 1));
+    }
     if (want_obj) r.set_objective_function_value(obj);
     return r;
 }

@@ -1,17 +1,21 @@
 // oneapi/dal/table.hpp -- minimal homogeneous table of the drop-in C++ surface.
 //
 // Covers what the k-means path exchanges (reference cpp/oneapi/dal/table/
-// homogen.hpp:48-114, row_accessor.hpp:70-135): row-major float / double /
-// int32 data held on the host, shared on copy, and row_accessor<const T>::pull
-// with element-type conversion.
+// homogen.hpp:48-160, row_accessor.hpp:70-135): row-major float / double /
+// int32 data, shared on copy, held on the host or on a device (the reference's
+// USM tables: homogen_table::wrap(queue, pointer, ...)), and
+// row_accessor<const T>::pull with element-type conversion (pull(queue, ...)
+// copies device rows to the host).
 #pragma once
 
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <type_traits>
 #include <vector>
 
+#include "../../onedal_amd/kmeans_hip.h"
 #include "exceptions.hpp"
 
 namespace oneapi::dal {
@@ -55,11 +59,15 @@ public:
     std::int64_t get_column_count() const { return cols_; }
     data_type get_data_type() const { return dtype_; }
     const void* get_raw_data() const { return data_.get(); }
+    // device tables: the HIP device holding the data (-1: host memory)
+    bool is_device() const { return device_ >= 0; }
+    int get_device() const { return device_; }
 
 protected:
     std::shared_ptr<const void> data_;
     std::int64_t rows_ = 0, cols_ = 0;
     data_type dtype_ = data_type::float32;
+    int device_ = -1;
 };
 
 class homogen_table : public table {
@@ -74,6 +82,28 @@ public:
         t.rows_ = rows;
         t.cols_ = cols;
         t.dtype_ = detail::data_type_of<T>();
+        return t;
+    }
+
+    // Non-owning view of row-major data in device memory of the queue's device
+    // (the reference's homogen_table::wrap(queue, usm_pointer, ...)).
+    template <typename Queue, typename T>
+    static homogen_table wrap(const Queue& q, const T* device_data, std::int64_t rows, std::int64_t cols) {
+        homogen_table t = wrap(device_data, rows, cols);
+        t.device_ = q.get_device();
+        return t;
+    }
+
+    // Device data whose lifetime the table shares (results of device calls).
+    template <typename T>
+    static homogen_table wrap_device(std::shared_ptr<const T> data, std::int64_t rows, std::int64_t cols,
+                                     int device) {
+        homogen_table t;
+        t.data_ = std::shared_ptr<const void>(data, data.get());
+        t.rows_ = rows;
+        t.cols_ = cols;
+        t.dtype_ = detail::data_type_of<T>();
+        t.device_ = device;
         return t;
     }
 
@@ -103,16 +133,46 @@ class row_accessor {
 public:
     explicit row_accessor(const table& t) : t_(t) {}
 
-    // Rows [r.start_idx, r.end_idx) as a host array of V, converted if needed.
+    // Rows [r.start_idx, r.end_idx) of a host table as a host array of V,
+    // converted if needed.  Device tables need the queue overload.
     array<V> pull(const range& r = {}) const {
+        if (t_.is_device())
+            throw invalid_argument("The table holds device data: pull it with a queue");
+        std::int64_t start, count;
+        const char* base = rows_of(r, start, count);
+        return convert_all(base, count);
+    }
+
+    // The same rows copied from a device table through the queue's context
+    // (host tables are read directly).
+    template <typename Queue>
+    array<V> pull(const Queue& q, const range& r = {}) const {
+        if (!t_.is_device()) return pull(r);
+        if (q.get_device() != t_.get_device())
+            throw invalid_argument("The queue's device does not hold the table");
+        std::int64_t start, count;
+        const char* base = rows_of(r, start, count);
+        std::vector<char> host(std::size_t(count) * detail::size_of(t_.get_data_type()));
+        const km_status st = km_memcpy(q.get_context(), host.data(), base, std::int64_t(host.size()),
+                                       KM_COPY_DEVICE_TO_HOST);
+        if (st != KM_OK) throw internal_error(std::string("device copy failed: ") + km_last_error(q.get_context()));
+        return convert_all(host.data(), count);
+    }
+
+private:
+    // first row and element count of r; pointer to its first element
+    const char* rows_of(const range& r, std::int64_t& start, std::int64_t& count) const {
         const std::int64_t end = r.end_idx < 0 ? t_.get_row_count() : r.end_idx;
         if (r.start_idx < 0 || end > t_.get_row_count() || r.start_idx > end)
             throw out_of_range("Row range is out of table bounds");
         const std::int64_t cols = t_.get_column_count();
-        const std::int64_t count = (end - r.start_idx) * cols;
+        start = r.start_idx;
+        count = (end - r.start_idx) * cols;
+        return static_cast<const char*>(t_.get_raw_data()) + r.start_idx * cols * detail::size_of(t_.get_data_type());
+    }
+
+    array<V> convert_all(const char* base, std::int64_t count) const {
         array<V> out(count);
-        const char* base = static_cast<const char*>(t_.get_raw_data()) +
-                           r.start_idx * cols * detail::size_of(t_.get_data_type());
         switch (t_.get_data_type()) {
             case data_type::float32: convert(reinterpret_cast<const float*>(base), out, count); break;
             case data_type::float64: convert(reinterpret_cast<const double*>(base), out, count); break;
@@ -121,7 +181,6 @@ public:
         return out;
     }
 
-private:
     template <typename S>
     static void convert(const S* src, array<V>& out, std::int64_t count) {
         V* dst = out.get_mutable_data();

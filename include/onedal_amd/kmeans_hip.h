@@ -97,6 +97,17 @@ const char* km_last_error(const km_ctx* ctx);
 km_status km_context_set_option(km_ctx* ctx, int option, int64_t value);
 km_status km_get_stats(const km_ctx* ctx, km_stats* out);
 
+/* Device memory on the context's device, for callers that keep data there (the
+ * C++ surface's device tables).  km_memcpy copies synchronously on the context's
+ * stream.  No counterpart in the reference, whose device tables hold SYCL USM
+ * (sycl::malloc_device, queue.memcpy; table/homogen.hpp:120-160). */
+#define KM_COPY_HOST_TO_DEVICE 1
+#define KM_COPY_DEVICE_TO_HOST 2
+#define KM_COPY_DEVICE_TO_DEVICE 3
+km_status km_device_alloc(km_ctx* ctx, int64_t bytes, void** out);
+km_status km_device_free(km_ctx* ctx, void* ptr);
+km_status km_memcpy(km_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t kind);
+
 /* Lloyd training.  X: n x d row-major; C0: k x d row-major initial centroids.
  * Outputs: C_out k x d, labels_out n (int32), *obj_out objective (sum of squared
  * distances to the final centroids), *iters_out iterations run.

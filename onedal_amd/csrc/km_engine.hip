@@ -1523,6 +1523,43 @@ km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances
     });
 }
 
+km_status km_device_alloc(km_ctx* ctx, int64_t bytes, void** out) {
+    return km::guarded(ctx, [&] {
+        if (!out || bytes < 0) throw km::status_error(KM_INVALID_ARGUMENT, "allocation size and output are required");
+        *out = nullptr;
+        hipError_t e = hipMalloc(out, size_t(std::max<int64_t>(bytes, 1)));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            throw km::status_error(KM_DEVICE_BAD_ALLOC,
+                                   "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+    });
+}
+
+km_status km_device_free(km_ctx* ctx, void* ptr) {
+    return km::guarded(ctx, [&] {
+        if (ptr) KM_HIP(hipFree(ptr));
+    });
+}
+
+km_status km_memcpy(km_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t kind) {
+    return km::guarded(ctx, [&] {
+        if (bytes < 0 || (bytes > 0 && (!dst || !src)))
+            throw km::status_error(KM_INVALID_ARGUMENT, "copy needs a size and both pointers");
+        hipMemcpyKind k;
+        switch (kind) {
+            case KM_COPY_HOST_TO_DEVICE: k = hipMemcpyHostToDevice; break;
+            case KM_COPY_DEVICE_TO_HOST: k = hipMemcpyDeviceToHost; break;
+            case KM_COPY_DEVICE_TO_DEVICE: k = hipMemcpyDeviceToDevice; break;
+            default: throw km::status_error(KM_INVALID_ARGUMENT, "unknown copy kind");
+        }
+        if (bytes == 0) return;
+        KM_HIP(hipMemcpyAsync(dst, src, size_t(bytes), k, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 km_status km_init_centroids_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, int64_t k,
                                 int32_t method, uint64_t seed, uint32_t flags, float* C_out) {
     return km::guarded(ctx, [&] { km::init_centroids<float>(ctx, X, n, d, k, method, seed, flags, C_out); });

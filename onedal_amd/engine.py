@@ -39,7 +39,8 @@ PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64", "km_init_centroids_f32", "km_init_centroids_f64",
+    "km_lloyd_infer_f64", "km_device_alloc", "km_device_free", "km_memcpy",
+    "km_init_centroids_f32", "km_init_centroids_f64",
     "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
     "km_server_create",
     "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",

@@ -150,9 +150,73 @@ static void init_checks(dal::hip::queue& q) {
     CHECK(r1.get_objective_function_value() == r2.get_objective_function_value());
 }
 
+// Device tables (the reference's USM tables): data wrapped over device memory
+// trains and infers without host copies, results are device tables, and they
+// equal the host-table results bit for bit.
+template <typename F>
+static void device_tables(dal::hip::queue& q) {
+    const int64_t n = 5000, d = 12, k = 9;
+    std::vector<F> x(n * d);
+    for (int64_t i = 0; i < n * d; ++i) x[i] = F(((i * 2654435761ull) % 2000) / 100.0) + F((i / d) % 7);
+    std::vector<F> c0(x.begin(), x.begin() + k * d);
+    void* dx = nullptr;
+    void* dc = nullptr;
+    CHECK(km_device_alloc(q.get_context(), int64_t(sizeof(F)) * n * d, &dx) == KM_OK);
+    CHECK(km_device_alloc(q.get_context(), int64_t(sizeof(F)) * k * d, &dc) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dx, x.data(), int64_t(sizeof(F)) * n * d, KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dc, c0.data(), int64_t(sizeof(F)) * k * d, KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    {
+        const auto xd = dal::homogen_table::wrap(q, static_cast<const F*>(dx), n, d);
+        const auto cd = dal::homogen_table::wrap(q, static_cast<const F*>(dc), k, d);
+        CHECK(xd.is_device() && xd.get_device() == q.get_device());
+        const auto desc = km::descriptor<F>(k).set_max_iteration_count(6);
+        const auto hr = dal::train(q, desc, dal::homogen_table::wrap(x.data(), n, d),
+                                   dal::homogen_table::wrap(c0.data(), k, d));
+        const auto dr = dal::train(q, desc, xd, cd);
+        CHECK(dr.get_model().get_centroids().is_device() && dr.get_responses().is_device());
+        const auto hc = dal::row_accessor<const F>(hr.get_model().get_centroids()).pull();
+        const auto dcen = dal::row_accessor<const F>(dr.get_model().get_centroids()).pull(q);
+        for (int64_t i = 0; i < k * d; ++i) CHECK(hc[i] == dcen[i]);
+        const auto hl = dal::row_accessor<const int>(hr.get_responses()).pull();
+        const auto dl = dal::row_accessor<const int>(dr.get_responses()).pull(q, {100, 200});
+        for (int64_t i = 0; i < 100; ++i) CHECK(hl[100 + i] == dl[i]);
+        CHECK(hr.get_objective_function_value() == dr.get_objective_function_value());
+        // a device table cannot be pulled without a queue
+        bool threw = false;
+        try {
+            (void)dal::row_accessor<const F>(xd).pull();
+        } catch (const dal::invalid_argument&) {
+            threw = true;
+        }
+        CHECK(threw);
+        // no initial centroids: device k-means++ equals the host-table path
+        const auto hr2 = dal::train(q, desc, dal::homogen_table::wrap(x.data(), n, d));
+        const auto dr2 = dal::train(q, desc, xd);
+        CHECK(hr2.get_objective_function_value() == dr2.get_objective_function_value());
+        // inference on device data with the device model, and with a host model
+        const auto idesc = km::descriptor<F>(k).set_result_options(km::result_options::compute_exact_objective_function);
+        const auto hi = dal::infer(q, idesc, hr.get_model(), dal::homogen_table::wrap(x.data(), n, d));
+        const auto di = dal::infer(q, idesc, dr.get_model(), xd);
+        const auto mi = dal::infer(q, idesc, hr.get_model(), xd);
+        CHECK(di.get_responses().is_device());
+        const auto hil = dal::row_accessor<const int>(hi.get_responses()).pull();
+        const auto dil = dal::row_accessor<const int>(di.get_responses()).pull(q);
+        const auto mil = dal::row_accessor<const int>(mi.get_responses()).pull(q);
+        for (int64_t i = 0; i < n; ++i) CHECK(hil[i] == dil[i] && hil[i] == mil[i]);
+        CHECK(hi.get_objective_function_value() == di.get_objective_function_value());
+        // host data with the device model
+        const auto xi = dal::infer(q, idesc, dr.get_model(), dal::homogen_table::wrap(x.data(), n, d));
+        CHECK(xi.get_objective_function_value() == hi.get_objective_function_value());
+    }
+    CHECK(km_device_free(q.get_context(), dx) == KM_OK);
+    CHECK(km_device_free(q.get_context(), dc) == KM_OK);
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     dal::hip::queue q(0);
+    device_tables<float>(q);
+    device_tables<double>(q);
     init_checks(q);
     gold<float>(argv[1], q);
     gold<double>(argv[1], q);
