@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
 
 // ---------------------------------------------------------------------------
 // Certified FP32 matrix-core assignment (tile variant 7) for instances with
-// d <= DP (DP in {32, 64, 128}) and k <= 128.  A work item is BATCH_MFMA_ROWS
+// d <= DP (DP in {16, 32, 64, 128}) and k <= 128.  A work item is BATCH_MFMA_ROWS
 // rows of one instance.  The item loads the instance's centroids into LDS once
 // (zero-padded to DP features and to whole blocks of 32), with ||c_j||^2 (double
 // sum rounded to float) and max ||c_j|| rounded up.  Then each 128-row tile puts
@@ -513,8 +513,8 @@ __global__ void batch_tune_points_kernel(float* __restrict__ X, const float* __r
 }
 
 // Launch kinds of a plan: the VALU tiles 0..6, then the matrix-core kernel for
-// DP = 32, 64, 128 (7, 8, 9); rows per work item of each.
-constexpr int BATCH_LAUNCH_KINDS = 10;
+// DP = 32, 64, 128, 16 (7, 8, 9, 10); rows per work item of each.
+constexpr int BATCH_LAUNCH_KINDS = 11;
 __host__ __device__ constexpr int batch_kind_rows(int kind) {
     return kind < BATCH_TILE_MFMA ? batch_tile_rows(kind) : BATCH_MFMA_ROWS;
 }

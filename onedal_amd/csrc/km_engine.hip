@@ -862,7 +862,7 @@ static int tile_for(const km_ctx* ctx, int64_t d, int64_t k) {
 // launch kind of a tile variant: the matrix-core variant splits by padded width
 static int launch_kind(int v, int64_t d) {
     if (v != BATCH_TILE_MFMA) return v;
-    return BATCH_TILE_MFMA + (d <= 32 ? 0 : d <= 64 ? 1 : 2);
+    return d <= 16 ? BATCH_TILE_MFMA + 3 : BATCH_TILE_MFMA + (d <= 32 ? 0 : d <= 64 ? 1 : 2);
 }
 
 static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance_f32* inst, int64_t count,
@@ -1040,7 +1040,8 @@ static void plan_assign(km_ctx* ctx, const batch_plan& p, int32_t* labels, uint3
             case 6: KM_BATCH_ASSIGN(16, 4); break;
             case 7: KM_BATCH_MFMA(32); break;
             case 8: KM_BATCH_MFMA(64); break;
-            default: KM_BATCH_MFMA(128); break;
+            case 9: KM_BATCH_MFMA(128); break;
+            default: KM_BATCH_MFMA(16); break;
         }
 #undef KM_BATCH_ASSIGN
 #undef KM_BATCH_MFMA
