@@ -484,9 +484,9 @@ def main():
                          "of known size, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the row-sharded design is described in DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path "
+              "(rows would shard with one all-reduce of the FP64 sums per iteration); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

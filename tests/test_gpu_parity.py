@@ -99,8 +99,8 @@ def test_train_bit_exact_vs_canonical(engine, oracle, dtype, path, shape):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_train_vs_daal_restatement(engine, oracle, dtype):
-    """BASELINE configs[0] shape (n=1e4, d=20, k=10, first-k init, threshold 0):
-    the engine against the reference's CPU algorithm, to rounding."""
+    """The reference's CPU workload shape (n=1e4, d=20, k=10, first-k init,
+    threshold 0): the engine against the reference's CPU algorithm, to rounding."""
     x = gaussian_mixture(10000, 20, 10, seed=0, center_range=10.0, dtype=dtype)
     c0 = x[:10].copy()
     c, lab, obj, it = engine.train(x, c0, 50, 0.0)
