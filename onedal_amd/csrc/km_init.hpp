@@ -10,12 +10,18 @@
 // restates it bit for bit:
 //   D2_r = min over chosen centroids of sum_f ((double)x_rf - (double)c_f)^2
 //          (features in order, each sub/mul/add rounded separately);
-//   total = chunk sums of OBJ_CHUNK rows (thread t of 256 sums rows t, t+256, ...
-//          in order, then a tree of the 256 partials), chunks added in order;
-//   target = u * total with u from the counter RNG; the chosen row is the first
-//          whose running sum (chunks in order, then that chunk's rows in order)
-//          exceeds target; if rounding leaves none, the chunk's last row with
-//          D2 > 0; if total is 0, row min(n - 1, floor(u * n)).
+//   chunk sums of OBJ_CHUNK rows (thread t of 256 sums rows t, t+256, ... in
+//          order, then a tree of the 256 partials); group sums of INIT_GROUP
+//          chunk sums each (in order); total = the group sums in order;
+//   target = u * total with u from the counter RNG.  The walk descends with one
+//          running sum: the first group whose running sum exceeds target (the
+//          last group otherwise), then in it the first such chunk (its last
+//          chunk otherwise), then in that chunk the first such block of
+//          INIT_SUB rows (block sums in row order; its last block otherwise),
+//          then in that block the first row with D2 > 0 whose running sum
+//          exceeds target.  Rounding fallbacks: the block's last row with
+//          D2 > 0, else the chunk's, else the chunk's first row.  If total is
+//          0: row min(n - 1, floor(u * n)).
 #pragma once
 
 #include <cstdint>
@@ -101,37 +107,39 @@ __global__ __launch_bounds__(256) void init_chunk_sums_kernel(const double* __re
     }
 }
 
-// The sampled row (one workgroup of 256): see the file comment.  The sequential
-// walks are done by thread 0 over LDS tiles of 4096 values that the whole
-// workgroup loads, so the order of every sum is the sequential one.
+constexpr int INIT_GROUP = 64;  // chunk sums per group sum
+constexpr int INIT_SUB = 16;    // rows per block of the last walk level
+
+// The sampled row (one workgroup of 256): see the file comment.  Group sums and
+// block sums are formed by all threads, each summing its own range in order;
+// thread 0 does the short sequential walks.
 __global__ __launch_bounds__(256) void init_select_kernel(const double* __restrict__ D2, int64_t n,
                                                          const double* __restrict__ chunk_sums, double u,
                                                          int64_t* __restrict__ chosen) {
-    constexpr int TILE = 4096;
-    __shared__ double tile[TILE];
+    constexpr int GT = 2048;  // group sums per LDS tile
+    static_assert(OBJ_CHUNK / INIT_SUB <= 256, "one block sum per thread");
+    __shared__ double gsum[GT];
+    __shared__ double bsum[OBJ_CHUNK / INIT_SUB];
     __shared__ double sh_total, sh_cum;
-    __shared__ int64_t sh_q;
+    __shared__ int64_t sh_sel;
     const int t = threadIdx.x;
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
-    // total = chunk sums added in order
+    const int64_t ngroups = (nchunks + INIT_GROUP - 1) / INIT_GROUP;
+    auto group_sum = [&](int64_t g) {
+        const int64_t q0 = g * INIT_GROUP, q1 = min<int64_t>(nchunks, q0 + INIT_GROUP);
+        double sg = 0.0;
+        for (int64_t q = q0; q < q1; ++q) sg = sg + chunk_sums[q];
+        return sg;
+    };
+    // total = group sums in order
     double total = 0.0;
-    for (int64_t b = 0; b < nchunks; b += TILE) {
-        const int m = int(min<int64_t>(TILE, nchunks - b));
+    for (int64_t b = 0; b < ngroups; b += GT) {
+        const int m = int(min<int64_t>(GT, ngroups - b));
         __syncthreads();
-        for (int i = t; i < m; i += 256) tile[i] = chunk_sums[b + i];
+        for (int i = t; i < m; i += 256) gsum[i] = group_sum(b + i);
         __syncthreads();
-        if (t == 0) {
-            // sixteen values in registers, then their adds in order
-            int i = 0;
-            for (; i + 16 <= m; i += 16) {
-                double v[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) v[j] = tile[i + j];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) total = total + v[j];
-            }
-            for (; i < m; ++i) total = total + tile[i];
-        }
+        if (t == 0)
+            for (int i = 0; i < m; ++i) total = total + gsum[i];
     }
     if (t == 0) sh_total = total;
     __syncthreads();
@@ -144,61 +152,80 @@ __global__ __launch_bounds__(256) void init_select_kernel(const double* __restri
         return;
     }
     const double target = u * total;
-    // chunk q: the first whose running sum exceeds target (the last chunk otherwise)
+    // the group
     double cum = 0.0;
-    int64_t qsel = -1;
-    for (int64_t b = 0; b < nchunks; b += TILE) {
-        const int m = int(min<int64_t>(TILE, nchunks - b));
+    int64_t gsel = -1;
+    for (int64_t b = 0; b < ngroups; b += GT) {
+        const int m = int(min<int64_t>(GT, ngroups - b));
         __syncthreads();
-        for (int i = t; i < m; i += 256) tile[i] = chunk_sums[b + i];
+        for (int i = t; i < m; i += 256) gsum[i] = group_sum(b + i);
         __syncthreads();
         if (t == 0) {
-            for (int i0 = 0; i0 < m && qsel < 0; i0 += 16) {
-                double v[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) v[j] = i0 + j < m ? tile[i0 + j] : 0.0;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int64_t q = b + i0 + j;
-                    if (qsel < 0 && i0 + j < m) {
-                        if (q == nchunks - 1 || cum + v[j] > target)
-                            qsel = q;
-                        else
-                            cum = cum + v[j];
-                    }
-                }
+            for (int i = 0; i < m && gsel < 0; ++i) {
+                const int64_t g = b + i;
+                if (g == ngroups - 1 || cum + gsum[i] > target)
+                    gsel = g;
+                else
+                    cum = cum + gsum[i];
             }
         }
     }
+    // the chunk inside the group
     if (t == 0) {
-        sh_q = qsel;
+        const int64_t q0 = gsel * INIT_GROUP, q1 = min<int64_t>(nchunks, q0 + INIT_GROUP);
+        int64_t qsel = q1 - 1;
+        for (int64_t q = q0; q < q1; ++q) {
+            const double c = chunk_sums[q];
+            if (q == q1 - 1 || cum + c > target) {
+                qsel = q;
+                break;
+            }
+            cum = cum + c;
+        }
+        sh_sel = qsel;
         sh_cum = cum;
     }
     __syncthreads();
-    qsel = sh_q;
+    const int64_t qsel = sh_sel;
     cum = sh_cum;
-    // the row inside chunk qsel
+    // block sums of the chunk: thread j sums rows INIT_SUB * j, ... in order
     const int64_t r0 = qsel * OBJ_CHUNK;
     const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
-    static_assert(OBJ_CHUNK <= TILE, "one chunk fits the tile");
-    for (int i = t; i < rows; i += 256) tile[i] = D2[r0 + i];
+    const int nb = (rows + INIT_SUB - 1) / INIT_SUB;
+    if (t < nb) {
+        const int e = min(rows, (t + 1) * INIT_SUB);
+        double sb = 0.0;
+        for (int r = t * INIT_SUB; r < e; ++r) sb = sb + D2[r0 + r];
+        bsum[t] = sb;
+    }
     __syncthreads();
     if (t == 0) {
-        int64_t idx = -1, last_pos = -1;
-        double s = cum;
-        for (int i0 = 0; i0 < rows && idx < 0; i0 += 16) {
-            double v[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) v[j] = i0 + j < rows ? tile[i0 + j] : 0.0;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                if (idx < 0 && i0 + j < rows) {
-                    const double w = v[j];
-                    if (w > 0.0) last_pos = r0 + i0 + j;
-                    s = s + w;
-                    if (w > 0.0 && s > target) idx = r0 + i0 + j;
-                }
+        int jsel = nb - 1;
+        for (int j = 0; j < nb; ++j) {
+            if (j == nb - 1 || cum + bsum[j] > target) {
+                jsel = j;
+                break;
             }
+            cum = cum + bsum[j];
+        }
+        int64_t idx = -1, last_pos = -1;
+        double sr = cum;
+        const int rb = jsel * INIT_SUB, re = min(rows, rb + INIT_SUB);
+        for (int r = rb; r < re; ++r) {
+            const double w = D2[r0 + r];
+            if (w > 0.0) last_pos = r0 + r;
+            sr = sr + w;
+            if (w > 0.0 && sr > target) {
+                idx = r0 + r;
+                break;
+            }
+        }
+        if (idx < 0 && last_pos < 0) {
+            for (int r = rows - 1; r >= 0; --r)
+                if (D2[r0 + r] > 0.0) {
+                    last_pos = r0 + r;
+                    break;
+                }
         }
         if (idx < 0) idx = last_pos >= 0 ? last_pos : r0;
         chosen[0] = idx;

@@ -569,10 +569,14 @@ uint64_t init_index(uint64_t seed, uint64_t stream, uint64_t i, uint64_t m) {  /
 // then for t = 1..k-1 D^2 sampling with u = unif(t) over
 //   D2_r = min over chosen c of sum_f ((double)x_f - (double)c_f)^2 (in order),
 //   chunk sums of 4096 rows (thread t of 256 adds rows t, t+256, ... in order,
-//   then the tree 128, 64, ..., 1), chunks added in order, and the first row
-//   whose running sum (chunks, then that chunk's rows) exceeds u * total
-//   (rows with D2 = 0 are skipped; rounding fallback: the chunk's last row with
-//   D2 > 0; total = 0: row min(n - 1, floor(u * n))).
+//   then the tree 128, 64, ..., 1), group sums of 64 chunk sums (in order),
+//   total = group sums in order; one running sum descends: the first group whose
+//   running sum exceeds u * total (else the last), in it the first such chunk
+//   (else its last), in that chunk the first such block of 16 rows (block sums
+//   in row order; else its last), in that block the first row with D2 > 0 whose
+//   running sum exceeds u * total.  Fallbacks: the block's last row with D2 > 0,
+//   else the chunk's, else the chunk's first row; total = 0: row
+//   min(n - 1, floor(u * n)).
 template <typename F>
 void init_centroids(const F* x, int64_t n, int64_t d, int64_t k, int method, uint64_t seed, F* c) {
     auto copy_row = [&](int64_t src, int64_t dst) {
@@ -623,8 +627,15 @@ void init_centroids(const F* x, int64_t n, int64_t d, int64_t k, int method, uin
                 for (int th = 0; th < st; ++th) part[th] = part[th] + part[th + st];
             chunk[size_t(q)] = part[0];
         }
+        const int64_t ngroups = (nchunks + 63) / 64;
+        std::vector<double> gsum(static_cast<size_t>(ngroups));
+        for (int64_t g = 0; g < ngroups; ++g) {
+            double sg = 0.0;
+            for (int64_t q = g * 64; q < std::min<int64_t>(nchunks, g * 64 + 64); ++q) sg = sg + chunk[size_t(q)];
+            gsum[size_t(g)] = sg;
+        }
         double total = 0.0;
-        for (int64_t q = 0; q < nchunks; ++q) total = total + chunk[size_t(q)];
+        for (int64_t g = 0; g < ngroups; ++g) total = total + gsum[size_t(g)];
         const double u = init_unif(seed, 1, uint64_t(t));
         int64_t idx;
         if (!(total > 0.0)) {
@@ -633,24 +644,44 @@ void init_centroids(const F* x, int64_t n, int64_t d, int64_t k, int method, uin
         } else {
             const double target = u * total;
             double cum = 0.0;
-            int64_t q = 0;
-            for (; q + 1 < nchunks; ++q) {
+            int64_t g = 0;
+            for (; g + 1 < ngroups; ++g) {
+                if (cum + gsum[size_t(g)] > target) break;
+                cum = cum + gsum[size_t(g)];
+            }
+            const int64_t q1 = std::min<int64_t>(nchunks, g * 64 + 64);
+            int64_t q = g * 64;
+            for (; q + 1 < q1; ++q) {
                 if (cum + chunk[size_t(q)] > target) break;
                 cum = cum + chunk[size_t(q)];
             }
-            const int64_t r0 = q * 4096, r1 = std::min<int64_t>(n, r0 + 4096);
+            const int64_t r0 = q * 4096, rows = std::min<int64_t>(4096, n - r0);
+            const int64_t nb = (rows + 15) / 16;
+            int64_t j = 0;
+            for (; j + 1 < nb; ++j) {
+                double sb = 0.0;
+                for (int64_t r = j * 16; r < std::min<int64_t>(rows, j * 16 + 16); ++r) sb = sb + D2[size_t(r0 + r)];
+                if (cum + sb > target) break;
+                cum = cum + sb;
+            }
             idx = -1;
             int64_t last_pos = -1;
             double sacc = cum;
-            for (int64_t r = r0; r < r1; ++r) {
-                const double w = D2[size_t(r)];
-                if (w > 0.0) last_pos = r;
+            for (int64_t r = j * 16; r < std::min<int64_t>(rows, j * 16 + 16); ++r) {
+                const double w = D2[size_t(r0 + r)];
+                if (w > 0.0) last_pos = r0 + r;
                 sacc = sacc + w;
                 if (w > 0.0 && sacc > target) {
-                    idx = r;
+                    idx = r0 + r;
                     break;
                 }
             }
+            if (idx < 0 && last_pos < 0)
+                for (int64_t r = rows - 1; r >= 0; --r)
+                    if (D2[size_t(r0 + r)] > 0.0) {
+                        last_pos = r0 + r;
+                        break;
+                    }
             if (idx < 0) idx = last_pos >= 0 ? last_pos : r0;
         }
         copy_row(idx, t);
