@@ -78,7 +78,9 @@ typedef enum {
 #define KM_BATCH_TILE_BUCKETS 42
 
 typedef struct {
-    int64_t assign_launches;      /* main assignment kernel launches in the last call */
+    int64_t assign_launches;      /* main assignment kernel launches in the last call
+                                     (batched training without KM_FLAG_GRAPH: assignment
+                                     stages, one per iteration, KM_OPT_TIMING only) */
     double assign_kernel_ms;      /* their summed duration (KM_OPT_TIMING only) */
     double iterations_ms;         /* Lloyd loop wall time on the stream (KM_OPT_TIMING only) */
     int64_t refined_candidate_rows; /* rows settled by the candidate re-score */

@@ -1058,13 +1058,17 @@ static void plan_copy_centroids(km_ctx* ctx, const batch_plan& p, int dir) {
     KM_LAUNCH_CHECK();
 }
 
-// one Lloyd iteration of every instance of a training plan
-static void plan_iteration(km_ctx* ctx, const batch_plan& p) {
+// one Lloyd iteration of every instance of a training plan; with events, the
+// assignment stage is bracketed by them (eager runs only)
+static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_start = nullptr,
+                           hipEvent_t asg_end = nullptr) {
     using BP = batch_plan;
     const int B = p.B;
     const int cap = ctx->cu_count * 8;
     auto g = [&](int64_t items) { return batch_grid(ctx, items); };
+    if (asg_start) KM_HIP(hipEventRecord(asg_start, ctx->stream));
     plan_assign(ctx, p, p.labels, p.skeys, 0);
+    if (asg_end) KM_HIP(hipEventRecord(asg_end, ctx->stream));
     fill<int32_t>(ctx, p.counts, p.kk, 0);
     hipLaunchKernelGGL(batch_count_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256), 0, ctx->stream, p.dI, B,
                        p.DP(BP::P_CNT), p.labels, p.counts);
@@ -1155,7 +1159,39 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         (void)hipGraphDestroy(graph);
         KM_HIP(le);
     } else {
-        for (int64_t it = 0; it < max_iter; ++it) plan_iteration(ctx, p);
+        // eager, timed: events around every iteration's assignment stage
+        std::vector<hipEvent_t> asg;
+        for (int64_t it = 0; it < max_iter; ++it) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (ctx->timing) {
+                a = ctx->event();
+                b = ctx->event();
+                asg.push_back(a);
+                asg.push_back(b);
+            }
+            plan_iteration(ctx, p, a, b);
+        }
+        if (ctx->timing) {
+            // the events are read after the synchronisation below
+            ctx->stats.assign_launches = int64_t(asg.size() / 2);
+        }
+        plan_finish(ctx, p);
+        if (ctx->timing) KM_HIP(hipEventRecord(e1, ctx->stream));
+        KM_HIP(hipMemcpyAsync(obj_out, p.objd, sizeof(double) * p.B, hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < p.B; ++i) iters_out[i] = max_iter;
+        if (ctx->timing) {
+            float ms = 0.f;
+            KM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ctx->stats.iterations_ms = ms;
+            double sum = 0.0;
+            for (size_t i = 0; i + 1 < asg.size(); i += 2) {
+                KM_HIP(hipEventElapsedTime(&ms, asg[i], asg[i + 1]));
+                sum += ms;
+            }
+            ctx->stats.assign_kernel_ms = sum;
+        }
+        return;
     }
     // final assignment (into the callers' label buffers), objective, centroids out
     plan_finish(ctx, p);
