@@ -131,6 +131,40 @@ km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, 
 km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
                              int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
 
+/* CSR data (method lloyd_csr).  values[nnz], col_indices[nnz], row_offsets[n + 1]
+ * as csr_table holds them (table/csr.hpp), zero- or one-based (sparse_indexing);
+ * column indices need not be sorted.  Offsets must run from the base to
+ * base + nnz without decreasing and every column index must lie in
+ * [base, base + d): otherwise KM_INVALID_ARGUMENT.  Distances are the expanded
+ * form of the reference (assign_clusters, backend/gpu/kernels_csr_impl.hpp:118-187):
+ * s_j = |c_j|^2 - 2 x.c_j over the stored entries in stored order, label = first
+ * j with the smallest s_j, closest = s_label + |x|^2; the objective is the sum of
+ * the closest distances (calc_objective_function, :193-203).  Replaces
+ * train_kernel_gpu<Float, method::lloyd_csr> (train_kernel_lloyd_csr_dpc.cpp:76-242)
+ * and the lloyd_csr inference (infer_kernel_dpc.cpp); loop, stop rule, centroid
+ * sums and empty-cluster handling are those of km_lloyd_train_f32, with candidate
+ * rows copied as in copy_candidates_from_data (empty_cluster_handling_dpc.cpp:152-189). */
+#define KM_SPARSE_ZERO_BASED 0
+#define KM_SPARSE_ONE_BASED 1
+km_status km_lloyd_train_csr_f32(km_ctx* ctx, const float* values, const int64_t* col_indices,
+                                 const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                 int32_t indexing, const float* C0, int64_t k, int64_t max_iter,
+                                 double acc_thr, uint32_t flags, float* C_out, int32_t* labels_out,
+                                 double* obj_out, int64_t* iters_out);
+km_status km_lloyd_train_csr_f64(km_ctx* ctx, const double* values, const int64_t* col_indices,
+                                 const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                 int32_t indexing, const double* C0, int64_t k, int64_t max_iter,
+                                 double acc_thr, uint32_t flags, double* C_out,
+                                 int32_t* labels_out, double* obj_out, int64_t* iters_out);
+km_status km_lloyd_infer_csr_f32(km_ctx* ctx, const float* values, const int64_t* col_indices,
+                                 const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                 int32_t indexing, const float* C, int64_t k, uint32_t flags,
+                                 int32_t* labels_out, double* obj_out);
+km_status km_lloyd_infer_csr_f64(km_ctx* ctx, const double* values, const int64_t* col_indices,
+                                 const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                 int32_t indexing, const double* C, int64_t k, uint32_t flags,
+                                 int32_t* labels_out, double* obj_out);
+
 /* Initial centroids (kmeans_init).  Methods: */
 #define KM_INIT_DENSE 0               /* the first k rows */
 #define KM_INIT_RANDOM_DENSE 1        /* k distinct rows, uniformly at random */

@@ -53,4 +53,15 @@ inline void check_infer_args(const void* X, int64_t n, int64_t d, const void* C,
     if (d > 4096) throw status_error(KM_UNIMPLEMENTED, "column count above 4096 is not supported");
 }
 
+// CSR arrays (csr_table: table/csr.hpp; the structure itself is checked on the
+// device before use)
+inline void check_csr_args(const void* values, const void* cols, const void* offs, int64_t nnz,
+                           int32_t indexing) {
+    if (indexing != KM_SPARSE_ZERO_BASED && indexing != KM_SPARSE_ONE_BASED)
+        throw status_error(KM_INVALID_ARGUMENT, "Sparse indexing must be zero-based or one-based");
+    if (nnz < 0) throw status_error(KM_INVALID_ARGUMENT, "Non-zero count is negative");
+    if (offs == nullptr || (nnz > 0 && (values == nullptr || cols == nullptr)))
+        throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
+}
+
 }  // namespace km

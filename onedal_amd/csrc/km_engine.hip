@@ -36,6 +36,7 @@
 #include "km_assign_bf16.hpp"
 #include "km_batch.hpp"
 #include "km_common.hpp"
+#include "km_csr.hpp"
 #include "km_init.hpp"
 #include "km_rng.hpp"
 #include "km_refine.hpp"
@@ -122,6 +123,10 @@ struct km_ctx {
     km::batch_ws bws, bws_fin;
     km::dbuf tune_x, tune_c, tune_l;  // autotuner inputs (released after tuning)
     km::dbuf init_d2, init_chunks, init_chosen, init_ids;  // initialisation
+    km::dbuf gt_d;  // empty-cluster candidates' d64
+    // CSR path: values, column indices, row offsets (when uploaded), row squares,
+    // transposed centroids, centroid squares, closest distances, structure flag
+    km::dbuf csr_vals, csr_cols, csr_offs, csr_xs, csr_ct, csr_cs, csr_closest, csr_flag;
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -458,14 +463,10 @@ static void sort_by_label(km_ctx* ctx, const int32_t* labels, int64_t n, int k, 
                                      bits, ctx->stream));
 }
 
-// Refill empty clusters: the m rows with the largest d64 (ties: lowest row) go,
-// in that order, to the empty clusters in ascending index order.  Returns the
-// objective correction (sum of their d64, subtracted in order by the caller).
-template <typename F>
-static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m,
-                         int32_t* empty_list, double* obj_host, bool have_obj) {
-    double* d64 = ctx->d64.get<double>(w.n);
-    double* objp = objective<F>(ctx, w, Ccur, d64);
+// The m rows with the largest d64 (ties: lowest row), in that order, on the
+// device; their d64 in cand_d.
+static int32_t* select_far_rows(km_ctx* ctx, const double* d64, int64_t n, int m,
+                                std::vector<double>& cand_d) {
     // radix select of the m-th largest key, 8 bits at a time from the top
     unsigned int* hist = ctx->hist.get<unsigned int>(256);
     unsigned long long prefix = 0, mask = 0;
@@ -474,8 +475,8 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
         fill<unsigned int>(ctx, hist, 256, 0u);
-        const int grid = grid_for(ctx, w.n, 256, 4);
-        hipLaunchKernelGGL(digit_hist_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, w.n,
+        const int grid = grid_for(ctx, n, 256, 4);
+        hipLaunchKernelGGL(digit_hist_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n,
                            prefix, mask, shift, hist);
         KM_LAUNCH_CHECK();
         KM_HIP(hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned int), hipMemcpyDeviceToHost,
@@ -496,11 +497,11 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
     }
     // prefix = key of the m-th largest; `remaining` rows equal to it are taken
     int32_t* gt = ctx->cand_gt.get<int32_t>(m);
-    int32_t* eq = ctx->cand_eq.get<int32_t>(w.n);
+    int32_t* eq = ctx->cand_eq.get<int32_t>(n);
     int32_t* cc = ctx->cand_counters.get<int32_t>(2);
     fill<int32_t>(ctx, cc, 2, 0);
-    const int grid = grid_for(ctx, w.n, 256, 4);
-    hipLaunchKernelGGL(collect_candidates_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, w.n,
+    const int grid = grid_for(ctx, n, 256, 4);
+    hipLaunchKernelGGL(collect_candidates_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n,
                        prefix, gt, eq, cc);
     KM_LAUNCH_CHECK();
     int32_t counts[2];
@@ -512,7 +513,7 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
     std::vector<int32_t> vgt(n_gt), veq(n_eq);
     if (n_gt) KM_HIP(hipMemcpyAsync(vgt.data(), gt, n_gt * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (n_eq) KM_HIP(hipMemcpyAsync(veq.data(), eq, n_eq * 4, hipMemcpyDeviceToHost, ctx->stream));
-    double* gt_d = ctx->obj_chunks.get<double>(std::max<int64_t>(n_gt, (w.n + OBJ_CHUNK - 1) / OBJ_CHUNK));
+    double* gt_d = ctx->gt_d.get<double>(std::max<int64_t>(n_gt, 1));
     std::vector<double> dgt(n_gt);
     if (n_gt) {
         hipLaunchKernelGGL(gather_f64_kernel, dim3(grid_for(ctx, n_gt, 256, 1)), dim3(256), 0,
@@ -535,7 +536,7 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
         std::memcpy(&thr_val, &pk, sizeof(double));
     }
     std::vector<int32_t> cand(m);
-    std::vector<double> cand_d(m);
+    cand_d.assign(m, 0.0);
     for (int64_t i = 0; i < n_gt; ++i) {
         cand[i] = order[i].second;
         cand_d[i] = order[i].first;
@@ -546,6 +547,21 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
     }
     int32_t* cand_dev = ctx->cand_dev.get<int32_t>(m);
     KM_HIP(hipMemcpyAsync(cand_dev, cand.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+    // cand is a stack vector: the copy must finish before it goes
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    return cand_dev;
+}
+
+// Refill empty clusters: the m rows with the largest d64 (ties: lowest row) go,
+// in that order, to the empty clusters in ascending index order; the objective
+// drops by their d64, subtracted in that order.
+template <typename F>
+static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m,
+                         int32_t* empty_list, double* obj_host, bool have_obj) {
+    double* d64 = ctx->d64.get<double>(w.n);
+    double* objp = objective<F>(ctx, w, Ccur, d64);
+    std::vector<double> cand_d;
+    int32_t* cand_dev = select_far_rows(ctx, d64, w.n, m, cand_d);
     const int fgrid = grid_for(ctx, int64_t(m) * w.d, 256, 4);
     hipLaunchKernelGGL(fill_empty_kernel<F>, dim3(fgrid), dim3(256), 0, ctx->stream, w.X, w.d,
                        empty_list, cand_dev, m, Cnext);
@@ -780,6 +796,265 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
         }
         ctx->stats.assign_kernel_ms = tot;
     }
+}
+
+// ---------------------------------------------------------------------------
+// CSR Lloyd (method lloyd_csr: train_kernel_lloyd_csr_dpc.cpp:76-242, and the
+// lloyd_csr branch of infer_kernel_dpc.cpp).  Loop, stop rule, grouping by label,
+// chunk sums, finalize and empty-cluster policy are the dense driver's; the
+// distances are the expanded form over the stored entries (km_csr.hpp).
+// ---------------------------------------------------------------------------
+template <typename F>
+struct csr_view {
+    const F* vals;
+    const int64_t* cols;
+    const int64_t* offs;
+    int64_t base, n, d, nnz;
+};
+
+// device view of the caller's CSR arrays (uploaded unless they are device
+// pointers), after the structure check
+template <typename F>
+static csr_view<F> csr_prepare(km_ctx* ctx, const F* vals, const int64_t* cols,
+                               const int64_t* offs, int64_t n, int64_t d, int64_t nnz,
+                               int32_t indexing, bool dev) {
+    csr_view<F> v{vals, cols, offs, indexing == KM_SPARSE_ONE_BASED ? 1 : 0, n, d, nnz};
+    if (!dev) {
+        F* vb = ctx->csr_vals.get<F>(std::max<int64_t>(nnz, 1));
+        int64_t* cb = ctx->csr_cols.get<int64_t>(std::max<int64_t>(nnz, 1));
+        int64_t* ob = ctx->csr_offs.get<int64_t>(n + 1);
+        if (nnz > 0) {
+            KM_HIP(hipMemcpyAsync(vb, vals, sizeof(F) * nnz, hipMemcpyHostToDevice, ctx->stream));
+            KM_HIP(hipMemcpyAsync(cb, cols, sizeof(int64_t) * nnz, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        }
+        KM_HIP(hipMemcpyAsync(ob, offs, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+        v.vals = vb;
+        v.cols = cb;
+        v.offs = ob;
+    } else if (nnz == 0) {
+        v.vals = ctx->csr_vals.get<F>(1);
+        v.cols = ctx->csr_cols.get<int64_t>(1);
+    }
+    int32_t* flag = ctx->csr_flag.get<int32_t>(1);
+    fill<int32_t>(ctx, flag, 1, 0);
+    hipLaunchKernelGGL(csr_check_kernel, dim3(grid_for(ctx, n, 256, 4)), dim3(256), 0, ctx->stream,
+                       v.cols, v.offs, v.base, n, d, nnz, flag);
+    KM_LAUNCH_CHECK();
+    int32_t* pin = ctx->pinned;
+    KM_HIP(hipMemcpyAsync(pin, flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    if (pin[0] & 1)
+        throw status_error(KM_INVALID_ARGUMENT,
+                           "Row offsets of the sparse data must be non-decreasing, from the "
+                           "indexing base to base + non-zero count");
+    if (pin[0] & 2)
+        throw status_error(KM_INVALID_ARGUMENT,
+                           "Column indices of the sparse data must lie in [base, base + column count)");
+    return v;
+}
+
+// labels and closest distances of every row against C (k x d)
+template <typename F>
+static void csr_assign(km_ctx* ctx, const csr_view<F>& x, const F* C, int k, const F* xs,
+                       int32_t* labels, F* closest) {
+    F* ct = ctx->csr_ct.get<F>(int64_t(k) * x.d);
+    F* cs = ctx->csr_cs.get<F>(k);
+    hipLaunchKernelGGL(csr_centroid_prep_kernel<F>, dim3(grid_for(ctx, int64_t(k) * x.d, 256, 4)),
+                       dim3(256), 0, ctx->stream, C, int64_t(k), x.d, ct, cs);
+    KM_LAUNCH_CHECK();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        e0 = ctx->event();
+        e1 = ctx->event();
+        KM_HIP(hipEventRecord(e0, ctx->stream));
+    }
+    const int grid = grid_for(ctx, x.n * 64, 256, 8);
+#define KM_CSRA(JT_)                                                                                \
+    hipLaunchKernelGGL((csr_assign_kernel<F, JT_>), dim3(grid), dim3(256), 0, ctx->stream, x.vals,   \
+                       x.cols, x.offs, x.base, x.n, k, ct, cs, xs, labels, closest)
+    if (k <= 64) KM_CSRA(1);
+    else if (k <= 128) KM_CSRA(2);
+    else KM_CSRA(4);
+#undef KM_CSRA
+    KM_LAUNCH_CHECK();
+    if (ctx->timing) KM_HIP(hipEventRecord(e1, ctx->stream));
+    ctx->stats.assign_launches++;
+}
+
+// objective = sum of closest (as double) in the fixed order; d64 per row if asked
+template <typename F>
+static double* csr_objective(km_ctx* ctx, const F* closest, int64_t n, double* d64_out) {
+    const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+    double* chunks = ctx->obj_chunks.get<double>(nchunks);
+    double* out = ctx->obj_dev.get<double>(1);
+    const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
+    hipLaunchKernelGGL(csr_objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream,
+                       closest, n, d64_out, chunks);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sum_sequential_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks, nchunks, out);
+    KM_LAUNCH_CHECK();
+    return out;
+}
+
+static void csr_timing_stats(km_ctx* ctx, size_t first_pair) {
+    double tot = 0.0;
+    for (size_t i = first_pair; i + 1 < ctx->ev_used; i += 2) {
+        float a = 0.f;
+        KM_HIP(hipEventElapsedTime(&a, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
+        tot += a;
+    }
+    ctx->stats.assign_kernel_ms = tot;
+}
+
+template <typename F>
+static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int64_t* offs,
+                      int64_t n, int64_t d64_, int64_t nnz, int32_t indexing, const F* C0,
+                      int64_t k64, int64_t max_iter, double thr, uint32_t flags, F* C_out,
+                      int32_t* labels_out, double* obj_out, int64_t* iters_out) {
+    const int d = int(d64_), k = int(k64);
+    const bool dev = flags & KM_FLAG_DEVICE_POINTERS;
+    ctx->stats = km_stats{};
+    ctx->ev_used = 0;
+    hipEvent_t loop0 = nullptr, loop1 = nullptr;
+    if (ctx->timing) {
+        loop0 = ctx->event();
+        loop1 = ctx->event();
+    }
+    const csr_view<F> x = csr_prepare<F>(ctx, vals, cols, offs, n, d, nnz, indexing, dev);
+    F* xs = ctx->csr_xs.get<F>(n);
+    F* closest = ctx->csr_closest.get<F>(n);
+    int32_t* labels = ctx->labels.get<int32_t>(n);
+    hipLaunchKernelGGL(csr_row_squares_kernel<F>, dim3(grid_for(ctx, n, 256, 4)), dim3(256), 0,
+                       ctx->stream, x.vals, x.offs, x.base, n, xs);
+    KM_LAUNCH_CHECK();
+    F* Ccur = ctx->c_cur.get<F>(int64_t(k) * d);
+    F* Cnext = ctx->c_next.get<F>(int64_t(k) * d);
+    KM_HIP(hipMemcpyAsync(Ccur, C0, sizeof(F) * k * d,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    int32_t* counts = ctx->counts.get<int32_t>(k);
+    int64_t* offsets = ctx->offsets.get<int64_t>(k + 1);
+    int32_t* chunk_base = ctx->chunk_base.get<int32_t>(k + 1);
+    int32_t* empty_list = ctx->empty_list.get<int32_t>(k);
+    int32_t* meta = ctx->meta.get<int32_t>(2);
+    int32_t* perm = ctx->perm.get<int32_t>(n);
+    double* part = ctx->part.get<double>((n / REDUCE_CHUNK + k + 1) * int64_t(d));
+    int32_t* pin = ctx->pinned;
+    const bool need_obj = thr > 0;
+    const size_t lds = size_t(reduce_groups(d)) * d * sizeof(double);
+    KM_HIP(hipFuncSetAttribute((const void*)csr_chunk_sums_kernel<F>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    if (ctx->timing) KM_HIP(hipEventRecord(loop0, ctx->stream));
+
+    double prev = double(std::numeric_limits<F>::max());
+    int64_t iter = 0;
+    for (iter = 0; iter < max_iter; ++iter) {
+        csr_assign<F>(ctx, x, Ccur, k, xs, labels, closest);
+        fill<int32_t>(ctx, counts, k, 0);
+        {
+            const bool in_lds = k <= 16384;
+            hipLaunchKernelGGL(count_kernel, dim3(grid_for(ctx, n, 256, 4)), dim3(256),
+                               in_lds ? k * sizeof(int32_t) : 0, ctx->stream, labels, n, k, counts,
+                               in_lds ? 1 : 0);
+            KM_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(scan_clusters_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, k,
+                           offsets, chunk_base, empty_list, meta);
+        KM_LAUNCH_CHECK();
+        sort_by_label(ctx, labels, n, k, perm);
+        const int64_t max_chunks = n / REDUCE_CHUNK + k + 1;
+        hipLaunchKernelGGL(csr_chunk_sums_kernel<F>, dim3(grid_for(ctx, max_chunks * 64, 64, 16)),
+                           dim3(64), lds, ctx->stream, x.vals, x.cols, x.offs, x.base, d, k, perm,
+                           offsets, chunk_base, meta, part);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(finalize_centroids_kernel<F>, dim3(grid_for(ctx, int64_t(k) * d, 256, 4)),
+                           dim3(256), 0, ctx->stream, part, counts, chunk_base, k, d, Cnext);
+        KM_LAUNCH_CHECK();
+        double* objp = nullptr;
+        if (need_obj) objp = csr_objective<F>(ctx, closest, n, nullptr);
+        KM_HIP(hipMemcpyAsync(pin, meta, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        if (need_obj)
+            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        const int m = pin[0];
+        double obj = 0.0;
+        if (need_obj) std::memcpy(&obj, pin + 6, sizeof(double));
+        if (m > 0) {
+            double* d64 = ctx->d64.get<double>(n);
+            double* op = csr_objective<F>(ctx, closest, n, d64);
+            std::vector<double> cand_d;
+            int32_t* cand_dev = select_far_rows(ctx, d64, n, m, cand_d);
+            hipLaunchKernelGGL(csr_fill_empty_kernel<F>, dim3(std::min(m, ctx->cu_count * 4)),
+                               dim3(64), 0, ctx->stream, x.vals, x.cols, x.offs, x.base, d,
+                               empty_list, cand_dev, m, Cnext);
+            KM_LAUNCH_CHECK();
+            if (!need_obj)
+                KM_HIP(hipMemcpyAsync(pin + 8, op, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KM_HIP(hipStreamSynchronize(ctx->stream));
+            if (!need_obj) std::memcpy(&obj, pin + 8, sizeof(double));
+            for (int e = 0; e < m; ++e) obj = obj - cand_d[e];
+            ctx->stats.empty_cluster_events++;
+        }
+        std::swap(Ccur, Cnext);
+        if (need_obj) {
+            if (obj + thr > prev) {
+                iter++;
+                break;
+            }
+            prev = obj;
+        }
+    }
+    csr_assign<F>(ctx, x, Ccur, k, xs, labels, closest);
+    double* objp = csr_objective<F>(ctx, closest, n, nullptr);
+    if (ctx->timing) KM_HIP(hipEventRecord(loop1, ctx->stream));
+    KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    const hipMemcpyKind out_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    KM_HIP(hipMemcpyAsync(C_out, Ccur, sizeof(F) * k * d, out_kind, ctx->stream));
+    KM_HIP(hipMemcpyAsync(labels_out, labels, sizeof(int32_t) * n, out_kind, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    double obj;
+    std::memcpy(&obj, pin + 6, sizeof(double));
+    if (obj_out) *obj_out = obj;
+    if (iters_out) *iters_out = iter;
+    if (ctx->timing) {
+        float ms = 0.f;
+        KM_HIP(hipEventElapsedTime(&ms, loop0, loop1));
+        ctx->stats.iterations_ms = ms;
+        csr_timing_stats(ctx, 2);
+    }
+}
+
+template <typename F>
+static void infer_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int64_t* offs,
+                      int64_t n, int64_t d64_, int64_t nnz, int32_t indexing, const F* C, int64_t k64,
+                      uint32_t flags, int32_t* labels_out, double* obj_out) {
+    const int d = int(d64_), k = int(k64);
+    const bool dev = flags & KM_FLAG_DEVICE_POINTERS;
+    ctx->stats = km_stats{};
+    ctx->ev_used = 0;
+    const csr_view<F> x = csr_prepare<F>(ctx, vals, cols, offs, n, d, nnz, indexing, dev);
+    F* xs = ctx->csr_xs.get<F>(n);
+    F* closest = ctx->csr_closest.get<F>(n);
+    int32_t* labels = ctx->labels.get<int32_t>(n);
+    hipLaunchKernelGGL(csr_row_squares_kernel<F>, dim3(grid_for(ctx, n, 256, 4)), dim3(256), 0,
+                       ctx->stream, x.vals, x.offs, x.base, n, xs);
+    KM_LAUNCH_CHECK();
+    F* Cd = ctx->c_cur.get<F>(int64_t(k) * d);
+    KM_HIP(hipMemcpyAsync(Cd, C, sizeof(F) * k * d,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    csr_assign<F>(ctx, x, Cd, k, xs, labels, closest);
+    int32_t* pin = ctx->pinned;
+    const bool want_obj = flags & KM_FLAG_EXACT_OBJECTIVE;
+    if (want_obj) {
+        double* objp = csr_objective<F>(ctx, closest, n, nullptr);
+        KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    KM_HIP(hipMemcpyAsync(labels_out, labels, sizeof(int32_t) * n,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+    if (want_obj && obj_out) std::memcpy(obj_out, pin + 6, sizeof(double));
+    if (ctx->timing) csr_timing_stats(ctx, 0);
 }
 
 // --- precondition checks (reference order: train_ops.hpp:40-61, common.cpp:82-115) ---
@@ -1476,7 +1751,9 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
                         &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l,
-                        &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids};
+                        &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids,
+                        &ctx->gt_d, &ctx->csr_vals, &ctx->csr_cols, &ctx->csr_offs, &ctx->csr_xs,
+                        &ctx->csr_ct, &ctx->csr_cs, &ctx->csr_closest, &ctx->csr_flag};
     for (auto* b : bufs) b->release();
     ctx->bws.release();
     ctx->bws_fin.release();
@@ -1551,6 +1828,35 @@ km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d,
         km::infer<double>(ctx, X, n, d, C, k, flags, labels_out, obj_out);
     });
 }
+
+#define KM_CSR_ABI(SUF, F)                                                                          \
+    km_status km_lloyd_train_csr_##SUF(km_ctx* ctx, const F* values, const int64_t* col_indices,     \
+                                       const int64_t* row_offsets, int64_t n, int64_t d,           \
+                                       int64_t nnz, int32_t indexing, const F* C0, int64_t k,      \
+                                       int64_t max_iter, double acc_thr, uint32_t flags,           \
+                                       F* C_out, int32_t* labels_out, double* obj_out,             \
+                                       int64_t* iters_out) {                                       \
+        return km::guarded(ctx, [&] {                                                               \
+            km::check_csr_args(values, col_indices, row_offsets, nnz, indexing);                    \
+            km::check_train_args(row_offsets, n, d, C0, k, max_iter, acc_thr, C_out, labels_out);   \
+            km::train_csr<F>(ctx, values, col_indices, row_offsets, n, d, nnz, indexing, C0, k,     \
+                             max_iter, acc_thr, flags, C_out, labels_out, obj_out, iters_out);      \
+        });                                                                                         \
+    }                                                                                               \
+    km_status km_lloyd_infer_csr_##SUF(km_ctx* ctx, const F* values, const int64_t* col_indices,     \
+                                       const int64_t* row_offsets, int64_t n, int64_t d,           \
+                                       int64_t nnz, int32_t indexing, const F* C, int64_t k,       \
+                                       uint32_t flags, int32_t* labels_out, double* obj_out) {     \
+        return km::guarded(ctx, [&] {                                                               \
+            km::check_csr_args(values, col_indices, row_offsets, nnz, indexing);                    \
+            km::check_infer_args(row_offsets, n, d, C, k, flags, labels_out, obj_out);              \
+            km::infer_csr<F>(ctx, values, col_indices, row_offsets, n, d, nnz, indexing, C, k,      \
+                             flags, labels_out, obj_out);                                           \
+        });                                                                                         \
+    }
+KM_CSR_ABI(f32, float)
+KM_CSR_ABI(f64, double)
+#undef KM_CSR_ABI
 
 km_status km_batch_train_f32(km_ctx* ctx, const km_batch_instance_f32* instances, int64_t count,
                              int64_t max_iter, uint32_t flags, double* obj_out,

@@ -41,6 +41,8 @@ EXPORTED_SYMBOLS = [
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
     "km_lloyd_infer_f64", "km_device_alloc", "km_device_free", "km_memcpy",
     "km_init_centroids_f32", "km_init_centroids_f64",
+    "km_lloyd_train_csr_f32", "km_lloyd_train_csr_f64", "km_lloyd_infer_csr_f32",
+    "km_lloyd_infer_csr_f64",
     "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
     "km_server_create",
     "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",
@@ -155,6 +157,12 @@ def load_library():
         f.restype = I
         f = getattr(L, "km_lloyd_infer_" + sfx)
         f.argtypes = [P, P, I64, I64, P, I64, U32, P, P]
+        f.restype = I
+        f = getattr(L, "km_lloyd_train_csr_" + sfx)
+        f.argtypes = [P, P, P, P, I64, I64, I64, ctypes.c_int32, P, I64, I64, D, U32, P, P, P, P]
+        f.restype = I
+        f = getattr(L, "km_lloyd_infer_csr_" + sfx)
+        f.argtypes = [P, P, P, P, I64, I64, I64, ctypes.c_int32, P, I64, U32, P, P]
         f.restype = I
     L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
                                      ctypes.POINTER(D), ctypes.POINTER(I64)]
@@ -285,6 +293,74 @@ class Engine:
         st = getattr(self.lib, "km_lloyd_infer_" + sfx)(
             self.ctx, x.ctypes.data, n, d, c.ctypes.data, c.shape[0], flags, labels.ctypes.data,
             ctypes.byref(obj))
+        self._check(st)
+        return labels, (obj.value if objective else None)
+
+    # -- CSR rows (method lloyd_csr) ------------------------------------------------
+    def train_csr(self, values, cols, offs, d, c0, max_iter, accuracy_threshold=0.0, base=0):
+        """CSR arrays in (host numpy, or CUDA tensors on this device: then the
+        outputs are CUDA tensors too): (centroids, labels, objective, iterations)."""
+        dev = _is_torch_cuda(values) or _is_torch_cuda(offs)
+        n = int(offs.shape[0]) - 1
+        nnz = int(values.shape[0])
+        k = int(c0.shape[0])
+        obj = ctypes.c_double()
+        iters = ctypes.c_int64()
+        if dev:
+            import torch
+
+            sfx = _sfx(str(values.dtype).replace("torch.", ""))
+            c = torch.empty((k, d), dtype=values.dtype, device=values.device)
+            labels = torch.empty(n, dtype=torch.int32, device=values.device)
+            ptrs = [values.data_ptr(), cols.data_ptr(), offs.data_ptr(), c0.data_ptr(),
+                    c.data_ptr(), labels.data_ptr()]
+            flags = KM_FLAG_DEVICE_POINTERS
+        else:
+            values = np.ascontiguousarray(values)
+            sfx = _sfx(values.dtype)
+            cols = np.ascontiguousarray(cols, dtype=np.int64)
+            offs = np.ascontiguousarray(offs, dtype=np.int64)
+            c0 = np.ascontiguousarray(c0, dtype=values.dtype)
+            c = np.empty((k, d), values.dtype)
+            labels = np.empty(n, np.int32)
+            ptrs = [values.ctypes.data, cols.ctypes.data, offs.ctypes.data, c0.ctypes.data,
+                    c.ctypes.data, labels.ctypes.data]
+            flags = 0
+        st = getattr(self.lib, "km_lloyd_train_csr_" + sfx)(
+            self.ctx, ptrs[0], ptrs[1], ptrs[2], n, int(d), nnz, int(base), ptrs[3], k,
+            int(max_iter), float(accuracy_threshold), flags, ptrs[4], ptrs[5], ctypes.byref(obj),
+            ctypes.byref(iters))
+        self._check(st)
+        return c, labels, obj.value, iters.value
+
+    def infer_csr(self, values, cols, offs, d, c, objective=False, base=0):
+        """Labels (and the objective if asked) of CSR rows against centroids c."""
+        dev = _is_torch_cuda(values) or _is_torch_cuda(offs)
+        n = int(offs.shape[0]) - 1
+        nnz = int(values.shape[0])
+        k = int(c.shape[0])
+        obj = ctypes.c_double()
+        flags = KM_FLAG_EXACT_OBJECTIVE if objective else 0
+        if dev:
+            import torch
+
+            sfx = _sfx(str(values.dtype).replace("torch.", ""))
+            labels = torch.empty(n, dtype=torch.int32, device=values.device)
+            ptrs = [values.data_ptr(), cols.data_ptr(), offs.data_ptr(), c.data_ptr(),
+                    labels.data_ptr()]
+            flags |= KM_FLAG_DEVICE_POINTERS
+        else:
+            values = np.ascontiguousarray(values)
+            sfx = _sfx(values.dtype)
+            cols = np.ascontiguousarray(cols, dtype=np.int64)
+            offs = np.ascontiguousarray(offs, dtype=np.int64)
+            c = np.ascontiguousarray(c, dtype=values.dtype)
+            labels = np.empty(n, np.int32)
+            ptrs = [values.ctypes.data, cols.ctypes.data, offs.ctypes.data, c.ctypes.data,
+                    labels.ctypes.data]
+        st = getattr(self.lib, "km_lloyd_infer_csr_" + sfx)(
+            self.ctx, ptrs[0], ptrs[1], ptrs[2], n, int(d), nnz, int(base), ptrs[3], k, flags,
+            ptrs[4], ctypes.byref(obj))
         self._check(st)
         return labels, (obj.value if objective else None)
 

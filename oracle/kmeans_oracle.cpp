@@ -55,6 +55,9 @@
 //        pre-update objective plus the empty-cluster correction; a final assignment
 //        with the final centroids gives labels and the returned objective.
 //
+// 2b. canonical_lloyd_csr: the same contract for CSR rows (method lloyd_csr), see
+//     the section of that name below.
+//
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off).  The -ffp-contract=off flag
 // matters: every mul/add in this file must round on its own.
 
@@ -513,6 +516,205 @@ int canonical_lloyd(const F* x, int64_t n, int64_t d, const F* c0, int64_t k, in
 }
 
 // ---------------------------------------------------------------------------
+// 2b. canonical restatement for CSR rows (method lloyd_csr).  Follows the
+//     reference's GPU CSR driver (cpp/oneapi/dal/algo/kmeans/backend/gpu/
+//     train_kernel_lloyd_csr_dpc.cpp:131-226) and kernels (kernels_csr_impl.hpp):
+//       - xs_i = sum of v*v over the row's stored entries in stored order (F)
+//         (compute_data_squares, :38-55)
+//       - cs_j = sum_f c_jf*c_jf in feature order (F) (compute_squares)
+//       - s_ij = cs_j - 2*dot_ij, dot_ij = sum over stored entries in stored
+//         order of v*c_j[col] (F, no FMA) (sparse gemm with alpha -2 + select,
+//         :141-180); label = first j with the smallest s (strict '<' from j = 0:
+//         ties go low, a NaN s_i0 keeps label 0, later NaNs never win)
+//       - closest_i = s_i,label + xs_i (F) (complete_closest_distances, :182-186)
+//       - objective = sum of double(closest_i) in the dense objective's order
+//         (calc_objective_function, :193-203)
+//       - centroid sums: the dense chunk/partial order of canonical_sums with the
+//         stored entries of a member row added in stored order (update_centroids,
+//         :219-322, adds them with atomics in no fixed order); counts 0 -> the
+//         rows with the largest closest_i fill the empty clusters, densified as
+//         copy_candidates_from_data does (empty_cluster_handling_dpc.cpp:152-189:
+//         zeros, then the stored entries, a later duplicate overwriting)
+//       - loop and stop rule of the dense driver above.
+// ---------------------------------------------------------------------------
+template <typename F>
+struct csr_rows {
+    const F* vals;
+    const int64_t* cols;
+    const int64_t* offs;
+    int64_t base;
+    int64_t begin(int64_t i) const { return offs[i] - base; }
+    int64_t end(int64_t i) const { return offs[i + 1] - base; }
+    int64_t col(int64_t p) const { return cols[p] - base; }
+};
+
+template <typename F>
+void csr_assign(const csr_rows<F>& x, int64_t n, int64_t d, const F* c, int64_t k,
+                int32_t* labels, F* closest, int nthreads) {
+    std::vector<F> cs(k);
+    for (int64_t j = 0; j < k; ++j) {
+        F acc(0);
+        for (int64_t f = 0; f < d; ++f) {
+            const F sq = c[j * d + f] * c[j * d + f];
+            acc = acc + sq;
+        }
+        cs[j] = acc;
+    }
+    parallel_ranges(n, nthreads, [&](int, int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            F xs(0);
+            for (int64_t p = x.begin(i); p < x.end(i); ++p) {
+                const F sq = x.vals[p] * x.vals[p];
+                xs = xs + sq;
+            }
+            F best(0);
+            int32_t lab = 0;
+            for (int64_t j = 0; j < k; ++j) {
+                F dot(0);
+                for (int64_t p = x.begin(i); p < x.end(i); ++p) {
+                    const F pr = x.vals[p] * c[j * d + x.col(p)];
+                    dot = dot + pr;
+                }
+                const F two_dot = F(2) * dot;
+                const F s = cs[j] - two_dot;
+                if (j == 0 || s < best) {
+                    if (j == 0) {
+                        best = s;
+                    } else {
+                        best = s;
+                        lab = int32_t(j);
+                    }
+                }
+            }
+            labels[i] = lab;
+            closest[i] = best + xs;
+        }
+    });
+}
+
+// the per-row keys of the empty-cluster selection are closest_i with a negative
+// (rounding) value or -0 taken as +0; the sum uses closest_i itself
+template <typename F>
+double csr_objective(const F* closest, int64_t n, int64_t obj_ch, std::vector<double>* d64_out) {
+    std::vector<double> d64(n);
+    for (int64_t i = 0; i < n; ++i) d64[i] = double(closest[i]);
+    double total = 0.0;
+    double part[256];
+    for (int64_t r0 = 0; r0 < n; r0 += obj_ch) {
+        const int64_t r1 = std::min(n, r0 + obj_ch);
+        for (int t = 0; t < 256; ++t) {
+            double p = 0.0;
+            for (int64_t i = r0 + t; i < r1; i += 256) p = p + d64[i];
+            part[t] = p;
+        }
+        for (int s = 128; s > 0; s >>= 1)
+            for (int t = 0; t < s; ++t) part[t] = part[t] + part[t + s];
+        total = total + part[0];
+    }
+    if (d64_out) {
+        for (auto& v : d64) v = (v != v || v > 0.0) ? v : 0.0;
+        *d64_out = std::move(d64);
+    }
+    return total;
+}
+
+template <typename F>
+void csr_sums(const csr_rows<F>& x, int64_t n, int64_t d, int64_t k, const int32_t* labels,
+              int64_t ch, int64_t g, std::vector<double>& s1, std::vector<int64_t>& counts) {
+    std::vector<std::vector<int64_t>> members(k);
+    counts.assign(k, 0);
+    for (int64_t i = 0; i < n; ++i) counts[labels[i]]++;
+    for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
+    s1.assign(k * d, 0.0);
+    std::vector<double> part(g * d), chunk(d);
+    for (int64_t j = 0; j < k; ++j) {
+        const auto& mem = members[j];
+        const int64_t cnt = int64_t(mem.size());
+        double* sj = s1.data() + j * d;
+        for (int64_t q0 = 0; q0 < cnt; q0 += ch) {
+            const int64_t q1 = std::min(cnt, q0 + ch);
+            std::fill(part.begin(), part.end(), 0.0);
+            for (int64_t m = q0; m < q1; ++m) {
+                double* pg = part.data() + ((m - q0) % g) * d;
+                const int64_t row = mem[m];
+                for (int64_t p = x.begin(row); p < x.end(row); ++p)
+                    pg[x.col(p)] = pg[x.col(p)] + double(x.vals[p]);
+            }
+            for (int64_t f = 0; f < d; ++f) {
+                double t = part[f];
+                for (int64_t gg = 1; gg < g; ++gg) t = t + part[gg * d + f];
+                chunk[f] = t;
+            }
+            for (int64_t f = 0; f < d; ++f) sj[f] = sj[f] + chunk[f];
+        }
+    }
+}
+
+template <typename F>
+int csr_lloyd(const csr_rows<F>& x, int64_t n, int64_t d, const F* c0, int64_t k,
+              int64_t max_iter, double thr, int64_t ch, int64_t g, int64_t obj_ch, int nthreads,
+              F* c_out, int32_t* labels_out, double* obj_out, int64_t* iters_out) {
+    std::vector<F> c(c0, c0 + k * d), closest(n);
+    std::vector<int32_t> labels(n);
+    std::vector<double> s1, d64;
+    std::vector<int64_t> counts;
+    double prev = double(std::numeric_limits<F>::max());
+    int64_t iter = 0;
+    for (iter = 0; iter < max_iter; ++iter) {
+        csr_assign(x, n, d, c.data(), k, labels.data(), closest.data(), nthreads);
+        double obj = csr_objective(closest.data(), n, obj_ch, &d64);
+        csr_sums(x, n, d, k, labels.data(), ch, g, s1, counts);
+        std::vector<F> cn(c);
+        std::vector<int64_t> empties;
+        for (int64_t j = 0; j < k; ++j) {
+            if (counts[j] > 0) {
+                for (int64_t f = 0; f < d; ++f) cn[j * d + f] = F(s1[j * d + f] / double(counts[j]));
+            } else {
+                empties.push_back(j);
+            }
+        }
+        if (!empties.empty()) {
+            const int64_t m = int64_t(empties.size());
+            if (m > n) return 1;
+            std::vector<int64_t> order(n);
+            std::iota(order.begin(), order.end(), int64_t(0));
+            // keys compare as the raw bits of the doubles (the engine's radix
+            // select); for non-negative values that is the numeric order
+            auto key = [&](int64_t i) {
+                uint64_t b;
+                std::memcpy(&b, &d64[i], sizeof(b));
+                return b;
+            };
+            std::partial_sort(order.begin(), order.begin() + m, order.end(),
+                              [&](int64_t a, int64_t b) {
+                                  if (key(a) != key(b)) return key(a) > key(b);
+                                  return a < b;
+                              });
+            for (int64_t e = 0; e < m; ++e) {
+                const int64_t j = empties[e];
+                const int64_t src = order[e];
+                std::fill(cn.begin() + j * d, cn.begin() + (j + 1) * d, F(0));
+                for (int64_t p = x.begin(src); p < x.end(src); ++p) cn[j * d + x.col(p)] = x.vals[p];
+                obj = obj - d64[src];
+            }
+        }
+        c.swap(cn);
+        if (thr > 0 && obj + thr > prev) {
+            iter++;
+            break;
+        }
+        prev = obj;
+    }
+    csr_assign(x, n, d, c.data(), k, labels.data(), closest.data(), nthreads);
+    const double obj = csr_objective(closest.data(), n, obj_ch, (std::vector<double>*)nullptr);
+    std::copy(c.begin(), c.end(), c_out);
+    std::copy(labels.begin(), labels.end(), labels_out);
+    *obj_out = obj;
+    *iters_out = iter;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // 3. FP64 brute force: labels and top-2 gap from exact-in-double distances
 // ---------------------------------------------------------------------------
 template <typename F>
@@ -718,6 +920,44 @@ int oracle_daal_lloyd_f64(const double* x, int64_t n, int64_t d, const double* c
                           int64_t* iters_out) {
     return daal_lloyd<double>(x, n, d, c0, k, max_iter, thr, stop_rule, block_size, nthreads,
                               c_out, labels_out, obj_out, iters_out);
+}
+
+int oracle_csr_lloyd_f32(const float* vals, const int64_t* cols, const int64_t* offs, int64_t base,
+                         int64_t n, int64_t d, const float* c0, int64_t k, int64_t max_iter,
+                         double thr, int64_t ch, int64_t g, int64_t obj_ch, int nthreads,
+                         float* c_out, int32_t* labels_out, double* obj_out, int64_t* iters_out) {
+    const csr_rows<float> x{vals, cols, offs, base};
+    return csr_lloyd(x, n, d, c0, k, max_iter, thr, ch, g, obj_ch, nthreads, c_out, labels_out,
+                     obj_out, iters_out);
+}
+
+int oracle_csr_lloyd_f64(const double* vals, const int64_t* cols, const int64_t* offs, int64_t base,
+                         int64_t n, int64_t d, const double* c0, int64_t k, int64_t max_iter,
+                         double thr, int64_t ch, int64_t g, int64_t obj_ch, int nthreads,
+                         double* c_out, int32_t* labels_out, double* obj_out, int64_t* iters_out) {
+    const csr_rows<double> x{vals, cols, offs, base};
+    return csr_lloyd(x, n, d, c0, k, max_iter, thr, ch, g, obj_ch, nthreads, c_out, labels_out,
+                     obj_out, iters_out);
+}
+
+void oracle_csr_assign_f32(const float* vals, const int64_t* cols, const int64_t* offs,
+                           int64_t base, int64_t n, int64_t d, const float* c, int64_t k,
+                           int nthreads, int32_t* labels, float* closest) {
+    csr_assign(csr_rows<float>{vals, cols, offs, base}, n, d, c, k, labels, closest, nthreads);
+}
+
+void oracle_csr_assign_f64(const double* vals, const int64_t* cols, const int64_t* offs,
+                           int64_t base, int64_t n, int64_t d, const double* c, int64_t k,
+                           int nthreads, int32_t* labels, double* closest) {
+    csr_assign(csr_rows<double>{vals, cols, offs, base}, n, d, c, k, labels, closest, nthreads);
+}
+
+double oracle_csr_objective_f32(const float* closest, int64_t n, int64_t obj_ch) {
+    return csr_objective(closest, n, obj_ch, (std::vector<double>*)nullptr);
+}
+
+double oracle_csr_objective_f64(const double* closest, int64_t n, int64_t obj_ch) {
+    return csr_objective(closest, n, obj_ch, (std::vector<double>*)nullptr);
 }
 
 int oracle_canonical_lloyd_f32(const float* x, int64_t n, int64_t d, const float* c0, int64_t k,

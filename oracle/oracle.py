@@ -72,6 +72,15 @@ def _declare(L):
         f = getattr(L, "oracle_fp64_assign_" + sfx)
         f.argtypes = [P, I64, I64, P, I64, P, P, P, I]
         f.restype = None
+        f = getattr(L, "oracle_csr_lloyd_" + sfx)
+        f.argtypes = [P, P, P, I64, I64, I64, P, I64, I64, D, I64, I64, I64, I, P, P, P, P]
+        f.restype = I
+        f = getattr(L, "oracle_csr_assign_" + sfx)
+        f.argtypes = [P, P, P, I64, I64, I64, P, I64, I, P, P]
+        f.restype = None
+        f = getattr(L, "oracle_csr_objective_" + sfx)
+        f.argtypes = [P, I64, I64]
+        f.restype = D
 
 
 def init_centroids(x, k, method, seed):
@@ -182,3 +191,68 @@ def fp64_assign(x, c, nthreads=None):
         _ptr(x), n, d, _ptr(c), c.shape[0], _ptr(labels), _ptr(best), _ptr(second),
         _threads(nthreads))
     return labels, best, second
+
+
+class Csr:
+    """CSR rows as csr_table holds them: values, column indices and row offsets
+    (int64), zero- or one-based (base 0 or 1), n rows of d columns."""
+
+    def __init__(self, values, cols, offs, d, base=0):
+        self.values = np.ascontiguousarray(values)
+        self.cols = np.ascontiguousarray(cols, dtype=np.int64)
+        self.offs = np.ascontiguousarray(offs, dtype=np.int64)
+        self.n = int(self.offs.shape[0] - 1)
+        self.d = int(d)
+        self.base = int(base)
+        if self.values.dtype not in (np.float32, np.float64):
+            raise TypeError("CSR values must be float32 or float64")
+
+    @classmethod
+    def from_dense(cls, x, base=0):
+        x = np.asarray(x)
+        mask = x != 0
+        offs = np.zeros(x.shape[0] + 1, np.int64)
+        offs[1:] = np.cumsum(mask.sum(axis=1))
+        rows, cols = np.nonzero(mask)
+        return cls(x[rows, cols], cols.astype(np.int64) + base, offs + base, x.shape[1], base)
+
+    def to_dense(self):
+        x = np.zeros((self.n, self.d), self.values.dtype)
+        for i in range(self.n):
+            for p in range(self.offs[i] - self.base, self.offs[i + 1] - self.base):
+                x[i, self.cols[p] - self.base] += self.values[p]
+        return x
+
+
+def csr_lloyd(x, c0, max_iter, thr=0.0, nthreads=None):
+    """The engine's CSR arithmetic contract (method lloyd_csr) restated on the CPU."""
+    c0 = np.ascontiguousarray(c0, dtype=x.values.dtype)
+    k = c0.shape[0]
+    c = np.empty_like(c0)
+    labels = np.empty(x.n, np.int32)
+    obj = ctypes.c_double()
+    iters = ctypes.c_int64()
+    st = getattr(lib(), "oracle_csr_lloyd_" + _sfx(x.values))(
+        _ptr(x.values), _ptr(x.cols), _ptr(x.offs), x.base, x.n, x.d, _ptr(c0), k, int(max_iter),
+        float(thr), REDUCE_CHUNK, reduce_groups(x.d), OBJ_CHUNK, _threads(nthreads), _ptr(c),
+        _ptr(labels), ctypes.byref(obj), ctypes.byref(iters))
+    if st != 0:
+        raise RuntimeError("oracle: more empty clusters than points")
+    return c, labels, obj.value, iters.value
+
+
+def csr_assign(x, c, nthreads=None):
+    """Labels and closest distances (expanded form over the stored entries)."""
+    c = np.ascontiguousarray(c, dtype=x.values.dtype)
+    labels = np.empty(x.n, np.int32)
+    closest = np.empty(x.n, x.values.dtype)
+    getattr(lib(), "oracle_csr_assign_" + _sfx(x.values))(
+        _ptr(x.values), _ptr(x.cols), _ptr(x.offs), x.base, x.n, x.d, _ptr(c), c.shape[0],
+        _threads(nthreads), _ptr(labels), _ptr(closest))
+    return labels, closest
+
+
+def csr_objective(closest):
+    closest = np.ascontiguousarray(closest)
+    return getattr(lib(), "oracle_csr_objective_" + _sfx(closest))(
+        _ptr(closest), closest.shape[0], OBJ_CHUNK)
