@@ -22,7 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = ("k-means point-iters/s + %roofline, bf16+refine vs fp32 path; p50/p99 job latency")
-PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0,
+        "fp32_vector_tflops": 157.3}
 
 # HBM bytes per launch of the main assignment kernel, from two rocprofv3 counter
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy of the same run
@@ -67,6 +68,13 @@ SERVING = {"serving_c5": dict(rates=(50, 100, 200, 400, 1000, 2000), window_s=2.
 # is point-steps/s = n * (k - 1) / time (every step reads all rows once).
 INIT = {"init_pp_c2": dict(n=10_000_000, d=64, k=256, method=2, center_range=20.0, sigma=(0.5, 2.0),
                            cpu_rows=200_000)}
+
+
+# CSR rows (method lloyd_csr): n rows over d columns, each row storing the nnz
+# columns of its cluster's support (256 clusters), values = center + noise;
+# one step = one training call of `iters` Lloyd iterations from the first k rows.
+CSR = {"csr_n1M_d2000_k256": dict(n=1_000_000, d=2000, k=256, nnz=20, iters=10, cpu_rows=1_000_000,
+                                  cpu_iters=12)}
 
 
 def log(*a):
@@ -473,13 +481,102 @@ def run_serving(torch, args, wl):
     print(json.dumps(out), flush=True)
 
 
+def run_csr(torch, args, wl):
+    from onedal_amd.engine import Engine, KM_OPT_TIMING
+
+    n, d, k, m, iters = wl["n"], wl["d"], wl["k"], wl["nnz"], wl["iters"]
+    dev = torch.device("cuda:0")
+    log("generating %s: n=%d d=%d k=%d, %d entries per row" % (args.workload, n, d, k, m))
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    supports = torch.sort(torch.rand(k, d, generator=g, device=dev).argsort(dim=1)[:, :m], dim=1).values
+    centers = torch.rand(k, d, generator=g, device=dev) * 8.0 + 2.0
+    truth = torch.randint(0, k, (n,), generator=g, device=dev)
+    cols = supports[truth]                                     # n x m, sorted per row
+    vals = (centers[truth.unsqueeze(1), cols] +
+            0.3 * torch.randn(n, m, generator=g, device=dev)).float().reshape(-1).contiguous()
+    cols = cols.reshape(-1).to(torch.int64).contiguous()
+    offs = torch.arange(0, n * m + 1, m, device=dev, dtype=torch.int64)
+    # initial centroids: the first k rows, expanded
+    c0 = torch.zeros(k, d, device=dev, dtype=torch.float32)
+    c0.scatter_(1, cols[: k * m].reshape(k, m), vals[: k * m].reshape(k, m))
+    torch.cuda.synchronize()
+    eng = Engine(0)
+    eng.set_option(KM_OPT_TIMING, 1)
+    for w in range(args.warmup):
+        eng.train_csr(vals, cols, offs, d, c0, iters, 0.0)
+        torch.cuda.synchronize()
+        log("warmup %d done" % w)
+    times, asg_ms, launches = [], 0.0, 0
+    for s_ in range(args.steps):
+        t0 = time.perf_counter()
+        c, labels, obj, it = eng.train_csr(vals, cols, offs, d, c0, iters, 0.0)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        st = eng.stats()
+        asg_ms += st["assign_kernel_ms"]
+        launches += st["assign_launches"]
+        log("step %d: %.3f s, objective %.9e, assign %.3f ms/launch" % (
+            s_, times[-1], obj, st["assign_kernel_ms"] / max(1, st["assign_launches"])))
+    elapsed = sum(times)
+    value = n * iters * args.steps / elapsed
+    avg_ms = asg_ms / max(1, launches)
+    flops = 2.0 * n * m * k  # one multiply and one add per stored entry and centroid
+    tflops = flops / (avg_ms * 1e-3) / 1e12
+    # the assignment's dominant traffic: per stored entry, one centroid column of k
+    # values gathered through the caches (the transposed centroids, k*d*4 B, stay in L2)
+    gather_gbs = n * m * k * 4 / (avg_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        from oracle import oracle as O
+
+        rows = wl["cpu_rows"]
+        hv = vals[: rows * m].cpu().numpy()
+        hc = cols[: rows * m].cpu().numpy()
+        ho = offs[: rows + 1].cpu().numpy()
+        xs = O.Csr(hv, hc, ho, d, 0)
+        ci = wl["cpu_iters"]
+        t0 = time.perf_counter()
+        O.csr_lloyd(xs, c0.cpu().numpy(), ci, 0.0, nthreads=16)
+        dt = time.perf_counter() - t0
+        cpu = {"value": rows * ci / dt, "unit": "point-iterations/s", "cores": 16, "kind": "port",
+               "sample": "first %d rows, %d Lloyd iterations plus the final assignment, oracle CSR "
+                         "restatement, 16 threads, %.1f s" % (rows, ci, dt)}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "d": d, "k": k, "entries_per_row": m,
+                   "iterations": iters, "method": "lloyd_csr"},
+        "roofline": {"bound": "valu", "achieved": round(tflops, 2), "peak": PEAK["fp32_vector_tflops"],
+                     "unit": "TFLOP/s", "frac": round(tflops / PEAK["fp32_vector_tflops"], 4), "traffic": None,
+                     "kernel": "csr_assign_kernel: 2*nnz*k flops per launch (separate multiply and "
+                               "add, so the vector units' ceiling is half the FMA peak)",
+                     "avg_launch_ms": round(avg_ms, 4), "cache_gather_gbs": round(gather_gbs, 1)},
+        "cpu_baseline": cpu,
+        "objective": obj,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="precision_c3",
-                    choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + sorted(INIT))
+                    choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + sorted(INIT) +
+                    sorted(CSR))
     ap.add_argument("--rates", default=None,
                     help="serving workloads: comma-separated offered rates in jobs/s")
     ap.add_argument("--instances", type=int, default=None,
@@ -510,6 +607,8 @@ def main():
         return run_init(torch, args, INIT[args.workload])
     if args.workload in SERVING:
         return run_serving(torch, args, SERVING[args.workload])
+    if args.workload in CSR:
+        return run_csr(torch, args, CSR[args.workload])
     wl = WORKLOADS[args.workload]
     n, d, k, iters = wl["n"], wl["d"], wl["k"], wl["iters"]
     dev = torch.device("cuda:0")

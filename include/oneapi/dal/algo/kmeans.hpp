@@ -24,11 +24,14 @@
 // descriptors; results are identical to the default path.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <limits>
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../../onedal_amd/kmeans_hip.h"
 #include "../exceptions.hpp"
@@ -456,12 +459,206 @@ std::shared_ptr<const Float> to_device(const hip::queue& q, const table& t) {
     return dptr;
 }
 
+template <typename F>
+km_status abi_train_csr(km_ctx* ctx, const F* v, const std::int64_t* ci, const std::int64_t* ro, std::int64_t n,
+                        std::int64_t d, std::int64_t nnz, std::int32_t ix, const F* C0, std::int64_t k,
+                        std::int64_t max_iter, double thr, std::uint32_t flags, F* C, std::int32_t* L, double* obj,
+                        std::int64_t* iters) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_lloyd_train_csr_f32(ctx, v, ci, ro, n, d, nnz, ix, C0, k, max_iter, thr, flags, C, L, obj, iters);
+    else
+        return km_lloyd_train_csr_f64(ctx, v, ci, ro, n, d, nnz, ix, C0, k, max_iter, thr, flags, C, L, obj, iters);
+}
+
+template <typename F>
+km_status abi_infer_csr(km_ctx* ctx, const F* v, const std::int64_t* ci, const std::int64_t* ro, std::int64_t n,
+                        std::int64_t d, std::int64_t nnz, std::int32_t ix, const F* C, std::int64_t k,
+                        std::uint32_t flags, std::int32_t* L, double* obj) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_lloyd_infer_csr_f32(ctx, v, ci, ro, n, d, nnz, ix, C, k, flags, L, obj);
+    else
+        return km_lloyd_infer_csr_f64(ctx, v, ci, ro, n, d, nnz, ix, C, k, flags, L, obj);
+}
+
+// The CSR arrays of a table as the engine takes them: device tables in place
+// (values of the descriptor's float type only), host tables by pointer (the
+// engine uploads them; values converted to Float if needed).
+template <typename Float>
+struct csr_args {
+    const Float* values = nullptr;
+    const std::int64_t* cols = nullptr;
+    const std::int64_t* offs = nullptr;
+    std::int64_t nnz = 0;
+    std::int32_t indexing = KM_SPARSE_ONE_BASED;
+    std::vector<Float> converted;
+};
+
+template <typename Float>
+void csr_of(const hip::queue& q, const table& t, csr_args<Float>& a) {
+    if (t.get_kind() != csr_table::kind())
+        throw invalid_argument("Method lloyd_csr needs a csr_table");
+    const csr_table c(t);
+    a.cols = c.get_column_indices();
+    a.offs = c.get_row_offsets();
+    a.nnz = c.get_non_zero_count();
+    a.indexing = c.get_indexing() == sparse_indexing::zero_based ? KM_SPARSE_ZERO_BASED : KM_SPARSE_ONE_BASED;
+    if (c.get_data_type() == oneapi::dal::detail::data_type_of<Float>()) {
+        a.values = static_cast<const Float*>(c.get_data());
+        return;
+    }
+    if (c.is_device())
+        throw invalid_argument("A device table's element type must be the descriptor's float type");
+    a.converted.resize(std::size_t(a.nnz));
+    if (c.get_data_type() == data_type::float32) {
+        const float* s = static_cast<const float*>(c.get_data());
+        for (std::int64_t i = 0; i < a.nnz; ++i) a.converted[i] = Float(s[i]);
+    } else {
+        const double* s = static_cast<const double*>(c.get_data());
+        for (std::int64_t i = 0; i < a.nnz; ++i) a.converted[i] = Float(s[i]);
+    }
+    a.values = a.converted.data();
+    (void)q;
+}
+
+// Default initial centroids of CSR data: the reference runs DAAL's plusPlusCSR
+// on the host (train_kernel_lloyd_csr_dpc.cpp:41-68, train_init_centroids.hpp:40-41);
+// here the rows are expanded on the host and k-means++ runs on the device with
+// the dense initialiser's seed.
+template <typename Float>
+array<Float> csr_plus_plus(hip::queue& q, const table& t, std::int64_t k) {
+    const csr_table c(t);
+    const std::int64_t n = c.get_row_count(), d = c.get_column_count();
+    const std::int64_t base = c.get_indexing() == sparse_indexing::one_based ? 1 : 0;
+    std::vector<std::int64_t> offs(std::size_t(n + 1));
+    std::vector<std::int64_t> cols(std::size_t(c.get_non_zero_count()));
+    std::vector<Float> vals(cols.size());
+    auto fetch = [&](void* dst, const void* src, std::int64_t bytes) {
+        if (!bytes) return;
+        if (c.is_device())
+            hip::check(km_memcpy(q.get_context(), dst, src, bytes, KM_COPY_DEVICE_TO_HOST), q.get_context());
+        else
+            std::memcpy(dst, src, std::size_t(bytes));
+    };
+    fetch(offs.data(), c.get_row_offsets(), std::int64_t(offs.size() * sizeof(std::int64_t)));
+    fetch(cols.data(), c.get_column_indices(), std::int64_t(cols.size() * sizeof(std::int64_t)));
+    csr_args<Float> a;
+    if (c.is_device()) {
+        fetch(vals.data(), c.get_data(), std::int64_t(vals.size() * sizeof(Float)));
+    } else {
+        csr_of<Float>(q, t, a);
+        std::copy(a.values, a.values + a.nnz, vals.begin());
+    }
+    std::vector<Float> dense(std::size_t(n * d), Float(0));
+    for (std::int64_t i = 0; i < n; ++i)
+        for (std::int64_t p = offs[i] - base; p < offs[i + 1] - base; ++p) {
+            if (p < 0 || p >= std::int64_t(cols.size()))
+                throw invalid_argument("Row offsets of the sparse data must be non-decreasing, from the "
+                                       "indexing base to base + non-zero count");
+            const std::int64_t col = cols[p] - base;
+            if (col < 0 || col >= d)
+                throw invalid_argument("Column indices of the sparse data must lie in [base, base + column count)");
+            dense[i * d + col] += vals[p];
+        }
+    array<Float> c0(k * d);
+    hip::check(abi_init_plus_plus<Float>(q.get_context(), dense.data(), n, d, k, 0u, c0.get_mutable_data()),
+               q.get_context());
+    return c0;
+}
+
+template <typename Float>
+train_result<> train_csr_on(hip::queue& q, const descriptor<Float, method::lloyd_csr>& desc,
+                            const train_input<>& in) {
+    const table& xt = in.get_data();
+    const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
+    const std::int64_t k = desc.get_cluster_count();
+    km_ctx* ctx = q.get_context();
+    csr_args<Float> a;
+    csr_of<Float>(q, xt, a);
+    double obj = 0.0;
+    std::int64_t iters = 0;
+    train_result<> r;
+    const bool has_c0 = in.get_initial_centroids().has_data();
+    if (xt.is_device()) {
+        const std::uint32_t flags = KM_FLAG_DEVICE_POINTERS;
+        std::shared_ptr<const Float> c0;
+        if (has_c0) {
+            c0 = to_device<Float>(q, in.get_initial_centroids());
+        } else {
+            const array<Float> h = csr_plus_plus<Float>(q, xt, k);
+            auto dc = device_array<Float>(q, k * d);
+            hip::check(km_memcpy(ctx, dc.get(), h.get_data(), std::int64_t(sizeof(Float)) * k * d,
+                                 KM_COPY_HOST_TO_DEVICE),
+                       ctx);
+            c0 = dc;
+        }
+        auto c = device_array<Float>(q, k * d);
+        auto labels = device_array<std::int32_t>(q, n);
+        hip::check(abi_train_csr<Float>(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, c0.get(), k,
+                                        desc.get_max_iteration_count(), desc.get_accuracy_threshold(), flags,
+                                        c.get(), labels.get(), &obj, &iters),
+                   ctx);
+        r.set_model(model<>().set_centroids(homogen_table::wrap_device<Float>(c, k, d, q.get_device())));
+        r.set_responses(homogen_table::wrap_device<std::int32_t>(labels, n, 1, q.get_device()));
+    } else {
+        const array<Float> c0 = has_c0 ? row_accessor<const Float>(in.get_initial_centroids()).pull(q)
+                                       : csr_plus_plus<Float>(q, xt, k);
+        array<Float> c(k * d);
+        array<std::int32_t> labels(n);
+        hip::check(abi_train_csr<Float>(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, c0.get_data(), k,
+                                        desc.get_max_iteration_count(), desc.get_accuracy_threshold(), 0u,
+                                        c.get_mutable_data(), labels.get_mutable_data(), &obj, &iters),
+                   ctx);
+        r.set_model(model<>().set_centroids(homogen_table::wrap(c, k, d)));
+        r.set_responses(homogen_table::wrap(labels, n, 1));
+    }
+    r.set_iteration_count(iters);
+    r.set_objective_function_value(obj);
+    return r;
+}
+
+template <typename Float>
+infer_result<> infer_csr_on(hip::queue& q, const descriptor<Float, method::lloyd_csr>& desc,
+                            const infer_input<>& in) {
+    const table& xt = in.get_data();
+    const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
+    const std::int64_t k = desc.get_cluster_count();
+    km_ctx* ctx = q.get_context();
+    csr_args<Float> a;
+    csr_of<Float>(q, xt, a);
+    const bool want_obj = desc.get_result_options().test(result_options::compute_exact_objective_function);
+    std::uint32_t flags = want_obj ? KM_FLAG_EXACT_OBJECTIVE : 0u;
+    double obj = 0.0;
+    infer_result<> r;
+    r.set_result_options(desc.get_result_options());
+    if (xt.is_device()) {
+        flags |= KM_FLAG_DEVICE_POINTERS;
+        const auto c = to_device<Float>(q, in.get_model().get_centroids());
+        auto labels = device_array<std::int32_t>(q, n);
+        hip::check(abi_infer_csr<Float>(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, c.get(), k, flags,
+                                        labels.get(), &obj),
+                   ctx);
+        r.set_responses(homogen_table::wrap_device<std::int32_t>(labels, n, 1, q.get_device()));
+    } else {
+        const auto c = row_accessor<const Float>(in.get_model().get_centroids()).pull(q);
+        array<std::int32_t> labels(n);
+        hip::check(abi_infer_csr<Float>(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, c.get_data(), k,
+                                        flags, labels.get_mutable_data(), &obj),
+                   ctx);
+        r.set_responses(homogen_table::wrap(labels, n, 1));
+    }
+    if (want_obj) r.set_objective_function_value(obj);
+    return r;
+}
+
 template <typename Float, typename Method>
 train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
                         const train_input<>& in) {
-    static_assert(std::is_same_v<Method, method::lloyd_dense>,
-                  "only the dense Lloyd method runs on this engine");
     check_train_preconditions(desc, in);
+    if constexpr (std::is_same_v<Method, method::lloyd_csr>) {
+        return train_csr_on<Float>(q, desc, in);
+    } else {
+    if (in.get_data().get_kind() != homogen_table::kind())
+        throw invalid_argument("Method lloyd_dense needs a homogeneous table");
     const table& xt = in.get_data();
     const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
     const std::int64_t k = desc.get_cluster_count();
@@ -511,14 +708,18 @@ train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
     r.set_iteration_count(iters);
     r.set_objective_function_value(obj);
     return r;
+    }
 }
 
 template <typename Float, typename Method>
 infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
                         const infer_input<>& in) {
-    static_assert(std::is_same_v<Method, method::lloyd_dense>,
-                  "only the dense Lloyd method runs on this engine");
     check_infer_preconditions(desc, in);
+    if constexpr (std::is_same_v<Method, method::lloyd_csr>) {
+        return infer_csr_on<Float>(q, desc, in);
+    } else {
+    if (in.get_data().get_kind() != homogen_table::kind())
+        throw invalid_argument("Method lloyd_dense needs a homogeneous table");
     const table& xt = in.get_data();
     const std::int64_t n = xt.get_row_count(), d = xt.get_column_count();
     const std::int64_t k = desc.get_cluster_count();
@@ -545,6 +746,7 @@ infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
     }
     if (want_obj) r.set_objective_function_value(obj);
     return r;
+    }
 }
 
 }  // namespace detail

@@ -51,10 +51,17 @@ private:
     std::shared_ptr<std::vector<T>> data_;
 };
 
+// table/common.hpp:37
+enum class sparse_indexing { zero_based, one_based };
+
 class table {
 public:
     table() = default;
-    bool has_data() const { return rows_ > 0 && cols_ > 0 && data_ != nullptr; }
+    bool has_data() const {
+        return rows_ > 0 && cols_ > 0 && (kind_ == 1 ? csr_offs_ != nullptr : data_ != nullptr);
+    }
+    // 0: homogen_table, 1: csr_table (homogen_table::kind(), csr_table::kind())
+    std::int64_t get_kind() const { return kind_; }
     std::int64_t get_row_count() const { return rows_; }
     std::int64_t get_column_count() const { return cols_; }
     data_type get_data_type() const { return dtype_; }
@@ -68,11 +75,18 @@ protected:
     std::int64_t rows_ = 0, cols_ = 0;
     data_type dtype_ = data_type::float32;
     int device_ = -1;
+    // CSR tables: column indices and row offsets (int64), stored entry count and
+    // indexing; kept in the base so a table passed by value keeps them
+    std::int64_t kind_ = 0;
+    std::shared_ptr<const void> csr_cols_, csr_offs_;
+    std::int64_t nnz_ = 0;
+    sparse_indexing indexing_ = sparse_indexing::one_based;
 };
 
 class homogen_table : public table {
 public:
     homogen_table() = default;
+    static std::int64_t kind() { return 0; }
 
     // Non-owning view of row-major host data (the caller keeps it alive).
     template <typename T>
@@ -120,6 +134,75 @@ public:
     }
 };
 
+// Compressed sparse rows (reference table/csr.hpp:30-352): values, column
+// indices and row offsets, zero- or one-based (one-based by default, as there).
+// wrap() makes non-owning views of host arrays, or of device arrays on the
+// queue's device.
+class csr_table : public table {
+public:
+    static std::int64_t kind() { return 1; }
+    csr_table() { kind_ = 1; }
+    // a table of CSR kind viewed as a csr_table (the reference's static_cast)
+    explicit csr_table(const table& t) : table(t) {
+        if (t.get_kind() != kind()) throw invalid_argument("The table is not a CSR table");
+    }
+
+    template <typename T>
+    static csr_table wrap(const T* data, const std::int64_t* column_indices,
+                          const std::int64_t* row_offsets, std::int64_t row_count,
+                          std::int64_t column_count,
+                          sparse_indexing indexing = sparse_indexing::one_based) {
+        csr_table t = make<T>(data, column_indices, row_offsets, row_count, column_count, indexing);
+        t.nnz_ = row_count > 0 && row_offsets ? row_offsets[row_count] - row_offsets[0] : 0;
+        return t;
+    }
+
+    template <typename Queue, typename T>
+    static csr_table wrap(const Queue& q, const T* data, const std::int64_t* column_indices,
+                          const std::int64_t* row_offsets, std::int64_t row_count,
+                          std::int64_t column_count,
+                          sparse_indexing indexing = sparse_indexing::one_based) {
+        csr_table t = make<T>(data, column_indices, row_offsets, row_count, column_count, indexing);
+        t.device_ = q.get_device();
+        std::int64_t ends[2] = {0, 0};
+        if (row_count > 0 && row_offsets) {
+            for (int e = 0; e < 2; ++e) {
+                const km_status st = km_memcpy(q.get_context(), &ends[e], row_offsets + (e ? row_count : 0),
+                                               sizeof(std::int64_t), KM_COPY_DEVICE_TO_HOST);
+                if (st != KM_OK)
+                    throw internal_error(std::string("device copy failed: ") + km_last_error(q.get_context()));
+            }
+        }
+        t.nnz_ = ends[1] - ends[0];
+        return t;
+    }
+
+    std::int64_t get_non_zero_count() const { return nnz_; }
+    sparse_indexing get_indexing() const { return indexing_; }
+    const void* get_data() const { return data_.get(); }
+    const std::int64_t* get_column_indices() const {
+        return static_cast<const std::int64_t*>(csr_cols_.get());
+    }
+    const std::int64_t* get_row_offsets() const {
+        return static_cast<const std::int64_t*>(csr_offs_.get());
+    }
+
+private:
+    template <typename T>
+    static csr_table make(const T* data, const std::int64_t* cols, const std::int64_t* offs,
+                          std::int64_t rows, std::int64_t ncols, sparse_indexing indexing) {
+        csr_table t;
+        t.data_ = std::shared_ptr<const void>(data, [](const void*) {});
+        t.csr_cols_ = std::shared_ptr<const void>(cols, [](const void*) {});
+        t.csr_offs_ = std::shared_ptr<const void>(offs, [](const void*) {});
+        t.rows_ = rows;
+        t.cols_ = ncols;
+        t.dtype_ = detail::data_type_of<T>();
+        t.indexing_ = indexing;
+        return t;
+    }
+};
+
 struct range {
     std::int64_t start_idx = 0;
     std::int64_t end_idx = -1;  // -1: up to the last row
@@ -131,7 +214,9 @@ class row_accessor {
     using V = std::remove_const_t<T>;
 
 public:
-    explicit row_accessor(const table& t) : t_(t) {}
+    explicit row_accessor(const table& t) : t_(t) {
+        if (t.get_kind() != 0) throw invalid_argument("row_accessor reads homogeneous tables");
+    }
 
     // Rows [r.start_idx, r.end_idx) of a host table as a host array of V,
     // converted if needed.  Device tables need the queue overload.

@@ -212,9 +212,109 @@ static void device_tables(dal::hip::queue& q) {
     CHECK(km_device_free(q.get_context(), dc) == KM_OK);
 }
 
+// method lloyd_csr on csr_table: the reference's CSR known answer
+// (kmeans/test/batch.cpp:404-478: objective 100, responses and centroids as the
+// dense run's), host and device CSR tables, and the default initialisation.
+template <typename F>
+static void csr_checks(dal::hip::queue& q) {
+    const F vals[] = {F(10), F(10), F(20)};
+    const F dense[] = {F(10), 0, 0, 0, F(10), 0, 0, 0, F(20)};
+    const F init[] = {F(5), F(5), 0, 0, 0, F(20)};
+    for (int one : {0, 1}) {
+        const int64_t cols[] = {0 + one, 1 + one, 2 + one};
+        const int64_t offs[] = {0 + one, 1 + one, 2 + one, 3 + one};
+        const auto ix = one ? dal::sparse_indexing::one_based : dal::sparse_indexing::zero_based;
+        const auto xc = dal::csr_table::wrap(vals, cols, offs, 3, 3, ix);
+        CHECK(xc.get_non_zero_count() == 3 && xc.get_kind() == dal::csr_table::kind());
+        const auto cdesc = km::descriptor<F, km::method::lloyd_csr>(2)
+                               .set_max_iteration_count(1)
+                               .set_accuracy_threshold(0.0)
+                               .set_result_options(km::result_options::compute_exact_objective_function);
+        const auto ddesc = km::descriptor<F, km::method::lloyd_dense>(2)
+                               .set_max_iteration_count(1)
+                               .set_accuracy_threshold(0.0);
+        const auto cr = dal::train(q, cdesc, xc, dal::homogen_table::wrap(init, 2, 3));
+        const auto dr = dal::train(q, ddesc, dal::homogen_table::wrap(dense, 3, 3),
+                                   dal::homogen_table::wrap(init, 2, 3));
+        CHECK(cr.get_objective_function_value() == 100.0);
+        CHECK(dr.get_objective_function_value() == 100.0);
+        const auto cl = dal::row_accessor<const int>(cr.get_responses()).pull(q);
+        const auto dl = dal::row_accessor<const int>(dr.get_responses()).pull(q);
+        for (int i = 0; i < 3; ++i) CHECK(cl[i] == dl[i]);
+        CHECK(cl[0] == 0 && cl[1] == 0 && cl[2] == 1);
+        const auto cc = dal::row_accessor<const F>(cr.get_model().get_centroids()).pull(q);
+        const auto dc = dal::row_accessor<const F>(dr.get_model().get_centroids()).pull(q);
+        for (int i = 0; i < 6; ++i) CHECK(cc[i] == dc[i]);
+        const auto ir = dal::infer(q, cdesc, cr.get_model(), xc);
+        CHECK(ir.get_objective_function_value() == 100.0);
+        // a dense table with the CSR method, and the reverse, are rejected
+        bool threw = false;
+        try {
+            dal::train(q, cdesc, dal::homogen_table::wrap(dense, 3, 3), dal::homogen_table::wrap(init, 2, 3));
+        } catch (const dal::invalid_argument&) {
+            threw = true;
+        }
+        CHECK(threw);
+        threw = false;
+        try {
+            dal::train(q, ddesc, xc, dal::homogen_table::wrap(init, 2, 3));
+        } catch (const dal::invalid_argument&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+    // a larger CSR table, on the host and on the device: same results
+    const int64_t n = 4000, d = 40, k = 8;
+    std::vector<F> v;
+    std::vector<int64_t> ci, ro{0};
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t g = i % k;
+        for (int64_t f = 0; f < d; ++f)
+            if ((f + g) % 5 == 0) {
+                v.push_back(F(g * 3 + 1) + F(((i * 7919 + f) % 100) / 100.0));
+                ci.push_back(f);
+            }
+        ro.push_back(int64_t(v.size()));
+    }
+    const int64_t nnz = int64_t(v.size());
+    void *dv = nullptr, *dci = nullptr, *dro = nullptr;
+    CHECK(km_device_alloc(q.get_context(), int64_t(sizeof(F)) * nnz, &dv) == KM_OK);
+    CHECK(km_device_alloc(q.get_context(), 8 * nnz, &dci) == KM_OK);
+    CHECK(km_device_alloc(q.get_context(), 8 * (n + 1), &dro) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dv, v.data(), int64_t(sizeof(F)) * nnz, KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dci, ci.data(), 8 * nnz, KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dro, ro.data(), 8 * (n + 1), KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    {
+        const auto hx = dal::csr_table::wrap(v.data(), ci.data(), ro.data(), n, d, dal::sparse_indexing::zero_based);
+        const auto dx = dal::csr_table::wrap(q, static_cast<const F*>(dv), static_cast<const int64_t*>(dci),
+                                             static_cast<const int64_t*>(dro), n, d, dal::sparse_indexing::zero_based);
+        CHECK(dx.is_device() && dx.get_non_zero_count() == nnz);
+        const auto desc = km::descriptor<F, km::method::lloyd_csr>(k).set_max_iteration_count(5);
+        const auto hr = dal::train(q, desc, hx);  // default initial centroids (k-means++)
+        const auto dr = dal::train(q, desc, dx, hr.get_model().get_centroids());
+        const auto hr2 = dal::train(q, desc, hx, hr.get_model().get_centroids());
+        CHECK(dr.get_iteration_count() == hr2.get_iteration_count());
+        CHECK(dr.get_objective_function_value() == hr2.get_objective_function_value());
+        const auto dl = dal::row_accessor<const int>(dr.get_responses()).pull(q);
+        const auto hl = dal::row_accessor<const int>(hr2.get_responses()).pull(q);
+        bool same = true;
+        for (int64_t i = 0; i < n; ++i) same = same && dl[i] == hl[i];
+        CHECK(same);
+        const auto ll = dal::row_accessor<const int>(hr.get_responses()).pull(q);
+        bool in_range = true;
+        for (int64_t i = 0; i < n; ++i) in_range = in_range && ll[i] >= 0 && ll[i] < k;
+        CHECK(in_range);
+    }
+    CHECK(km_device_free(q.get_context(), dv) == KM_OK);
+    CHECK(km_device_free(q.get_context(), dci) == KM_OK);
+    CHECK(km_device_free(q.get_context(), dro) == KM_OK);
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     dal::hip::queue q(0);
+    csr_checks<float>(q);
+    csr_checks<double>(q);
     device_tables<float>(q);
     device_tables<double>(q);
     init_checks(q);
