@@ -2,7 +2,8 @@
 //
 // Replaces, on the oneDAL side (backend/gpu/kernels_csr_impl.hpp):
 //   compute_data_squares            :38-55    -> csr_row_squares_kernel
-//   transpose + compute_squares     :70-92, kernels_fp_impl.hpp:402-414 -> csr_centroid_prep_kernel
+//   transpose + compute_squares     :70-92, kernels_fp_impl.hpp:402-414
+//                                   -> csr_transpose_kernel, csr_centroid_squares_kernel
 //   assign_clusters (sparse gemm, select, complete_closest_distances)
 //                                   :118-187  -> csr_assign_kernel
 //   calc_objective_function         :193-203  -> csr_objective_chunks_kernel
@@ -74,26 +75,48 @@ __global__ __launch_bounds__(256) void csr_row_squares_kernel(const F* __restric
     }
 }
 
-// Ct (d x k) = C^T for coalesced reads over centroids, cs_j = |c_j|^2 in feature order
+// Ct (d x k) = C^T for coalesced reads over centroids: 64 x 64 tiles through LDS
+// (grid: ceil(d/64) x up to 65535 workgroups of 256 threads, striding over k)
 template <typename F>
-__global__ __launch_bounds__(256) void csr_centroid_prep_kernel(const F* __restrict__ C, int64_t k,
-                                                                int64_t d, F* __restrict__ Ct,
-                                                                F* __restrict__ cs) {
-    const int64_t total = k * d;
-    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
-         e += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t f = e / k, j = e % k;
-        Ct[e] = C[j * d + f];
-    }
-    for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < k;
-         j += int64_t(gridDim.x) * blockDim.x) {
-        F acc(0);
-        for (int64_t f = 0; f < d; ++f) {
-            const F c = C[j * d + f];
-            const F sq = c * c;
-            acc = acc + sq;
+__global__ __launch_bounds__(256) void csr_transpose_kernel(const F* __restrict__ C, int64_t k,
+                                                            int64_t d, F* __restrict__ Ct) {
+    __shared__ F tile[64][65];
+    const int64_t f0 = int64_t(blockIdx.x) * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int64_t j0 = int64_t(blockIdx.y) * 64; j0 < k; j0 += int64_t(gridDim.y) * 64) {
+        for (int r = ty; r < 64; r += 4) {
+            const int64_t j = j0 + r, f = f0 + tx;
+            if (j < k && f < d) tile[r][tx] = C[j * d + f];
         }
-        cs[j] = acc;
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            const int64_t f = f0 + r, j = j0 + tx;
+            if (j < k && f < d) Ct[f * k + j] = tile[tx][r];
+        }
+        __syncthreads();
+    }
+}
+
+// cs_j = |c_j|^2, features in order: one wave per centroid copies the row to LDS
+// (coalesced), then lane 0 adds the squares in feature order
+template <typename F>
+__global__ __launch_bounds__(64) void csr_centroid_squares_kernel(const F* __restrict__ C, int64_t k,
+                                                                  int d, F* __restrict__ cs) {
+    extern __shared__ unsigned char sq_raw[];
+    F* row = reinterpret_cast<F*>(sq_raw);  // [d]
+    for (int64_t j = blockIdx.x; j < k; j += gridDim.x) {
+        for (int f = threadIdx.x; f < d; f += 64) row[f] = C[j * d + f];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            F acc(0);
+            for (int f = 0; f < d; ++f) {
+                const F c = row[f];
+                const F sq = c * c;
+                acc = acc + sq;
+            }
+            cs[j] = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -107,6 +130,17 @@ __device__ __forceinline__ void csr_take(F& bs, int& bj, bool& has, F s, int j) 
         bj = j;
         has = true;
     }
+}
+
+// value of lane u (wave-uniform u) in scalar registers
+__device__ __forceinline__ float lane_value(float x, int u) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), u));
+}
+__device__ __forceinline__ double lane_value(double x, int u) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(int(b & 0xFFFFFFFFll), u);
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), u);
+    return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
 }
 
 // One wave per row.  Lane l accumulates the dots of centroids jb + l + 64 t
@@ -132,16 +166,17 @@ __global__ __launch_bounds__(256) void csr_assign_kernel(
             for (int t = 0; t < JT; ++t) acc[t] = F(0);
             for (int64_t p0 = o0; p0 < o1; p0 += 64) {
                 const int cnt = int(min<int64_t>(64, o1 - p0));
-                int64_t my_c = 0;
+                int my_c = 0;  // a column index (< d <= 4096)
                 F my_v(0);
                 if (lane < cnt) {
-                    my_c = cols[p0 + lane] - base;
+                    my_c = int(cols[p0 + lane] - base);
                     my_v = vals[p0 + lane];
                 }
                 for (int u = 0; u < cnt; ++u) {
-                    const int64_t c = __shfl(my_c, u);
-                    const F v = __shfl(my_v, u);
-                    const F* crow = Ct + c * k;
+                    // entry u of the group, read from its lane into scalar registers
+                    const int c = __builtin_amdgcn_readlane(my_c, u);
+                    const F v = lane_value(my_v, u);
+                    const F* crow = Ct + int64_t(c) * k;
 #pragma unroll
                     for (int t = 0; t < JT; ++t) {
                         const int j = jb + 64 * t + lane;

@@ -861,8 +861,11 @@ static void csr_assign(km_ctx* ctx, const csr_view<F>& x, const F* C, int k, con
                        int32_t* labels, F* closest) {
     F* ct = ctx->csr_ct.get<F>(int64_t(k) * x.d);
     F* cs = ctx->csr_cs.get<F>(k);
-    hipLaunchKernelGGL(csr_centroid_prep_kernel<F>, dim3(grid_for(ctx, int64_t(k) * x.d, 256, 4)),
-                       dim3(256), 0, ctx->stream, C, int64_t(k), x.d, ct, cs);
+    hipLaunchKernelGGL(csr_transpose_kernel<F>, dim3(unsigned((x.d + 63) / 64), unsigned(std::min((k + 63) / 64, 65535))),
+                       dim3(256), 0, ctx->stream, C, int64_t(k), x.d, ct);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(csr_centroid_squares_kernel<F>, dim3(unsigned(std::min(k, ctx->cu_count * 8))),
+                       dim3(64), size_t(x.d) * sizeof(F), ctx->stream, C, int64_t(k), int(x.d), cs);
     KM_LAUNCH_CHECK();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
