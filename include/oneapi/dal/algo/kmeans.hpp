@@ -651,12 +651,8 @@ infer_result<> infer_csr_on(hip::queue& q, const descriptor<Float, method::lloyd
 }
 
 template <typename Float, typename Method>
-train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
-                        const train_input<>& in) {
-    check_train_preconditions(desc, in);
-    if constexpr (std::is_same_v<Method, method::lloyd_csr>) {
-        return train_csr_on<Float>(q, desc, in);
-    } else {
+train_result<> train_dense_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                              const train_input<>& in) {
     if (in.get_data().get_kind() != homogen_table::kind())
         throw invalid_argument("Method lloyd_dense needs a homogeneous table");
     const table& xt = in.get_data();
@@ -708,16 +704,11 @@ train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
     r.set_iteration_count(iters);
     r.set_objective_function_value(obj);
     return r;
-    }
 }
 
 template <typename Float, typename Method>
-infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
-                        const infer_input<>& in) {
-    check_infer_preconditions(desc, in);
-    if constexpr (std::is_same_v<Method, method::lloyd_csr>) {
-        return infer_csr_on<Float>(q, desc, in);
-    } else {
+infer_result<> infer_dense_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                              const infer_input<>& in) {
     if (in.get_data().get_kind() != homogen_table::kind())
         throw invalid_argument("Method lloyd_dense needs a homogeneous table");
     const table& xt = in.get_data();
@@ -746,7 +737,26 @@ infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
     }
     if (want_obj) r.set_objective_function_value(obj);
     return r;
-    }
+}
+
+template <typename Float, typename Method>
+train_result<> train_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                        const train_input<>& in) {
+    check_train_preconditions(desc, in);
+    if constexpr (std::is_same_v<Method, method::lloyd_csr>)
+        return train_csr_on<Float>(q, desc, in);
+    else
+        return train_dense_on<Float>(q, desc, in);
+}
+
+template <typename Float, typename Method>
+infer_result<> infer_on(hip::queue& q, const descriptor<Float, Method>& desc,
+                        const infer_input<>& in) {
+    check_infer_preconditions(desc, in);
+    if constexpr (std::is_same_v<Method, method::lloyd_csr>)
+        return infer_csr_on<Float>(q, desc, in);
+    else
+        return infer_dense_on<Float>(q, desc, in);
 }
 
 }  // namespace detail
