@@ -252,12 +252,25 @@ __global__ __launch_bounds__(256) void csr_objective_chunks_kernel(const F* __re
     }
 }
 
+// members whose entries are loaded together by the chunk-sum kernel
+constexpr int CSR_AHEAD = 8;
+
+__device__ __forceinline__ int64_t lane_value(int64_t x, int u) {
+    const int lo = __builtin_amdgcn_readlane(int(x & 0xFFFFFFFFll), u);
+    const int hi = __builtin_amdgcn_readlane(int(x >> 32), u);
+    return int64_t((unsigned long long)(unsigned(lo)) | ((unsigned long long)(unsigned(hi)) << 32));
+}
+
 // Chunk sums of the CSR rows: one single-wave workgroup per chunk of
 // REDUCE_CHUNK members, G = reduce_groups(d) partial vectors of d doubles in LDS
 // (member m of the chunk goes to partial m % G).  A row whose column indices are
 // strictly increasing is added by all lanes at once (distinct columns); any
 // other row by lane 0 alone, entry by entry, so every column is summed in member
 // order and, inside a row, in stored order.  Partials are added in g order.
+// The wave's LDS operations execute in issue order, so a member's adds see the
+// previous member's.  Groups of 64 members whose rows all hold at most 64
+// entries load CSR_AHEAD rows' entries at a time from registers (the offsets of
+// the whole group first); other groups go member by member.
 template <typename F>
 __global__ __launch_bounds__(64) void csr_chunk_sums_kernel(
     const F* __restrict__ vals, const int64_t* __restrict__ cols, const int64_t* __restrict__ offs,
@@ -280,26 +293,79 @@ __global__ __launch_bounds__(64) void csr_chunk_sums_kernel(
         const int64_t m_end = min(m_begin + REDUCE_CHUNK, offsets[j + 1]);
         for (int e = lane; e < width; e += 64) acc[e] = 0.0;
         __syncthreads();
-        for (int64_t m = m_begin; m < m_end; ++m) {
-            const int64_t row = perm[m];
-            double* pg = acc + int((m - m_begin) % G) * d;
-            const int64_t o0 = offs[row] - base, o1 = offs[row + 1] - base;
-            bool sorted = true;
-            for (int64_t p = o0 + 1 + lane; p < o1; p += 64)
-                if (cols[p] <= cols[p - 1]) sorted = false;
-            sorted = __all(sorted);
-            if (sorted) {
-                for (int64_t p = o0 + lane; p < o1; p += 64) {
-                    const int64_t c = cols[p] - base;
-                    pg[c] = pg[c] + double(vals[p]);
-                }
-            } else if (lane == 0) {
-                for (int64_t p = o0; p < o1; ++p) {
-                    const int64_t c = cols[p] - base;
-                    pg[c] = pg[c] + double(vals[p]);
-                }
+        // groups of up to 64 members: lane t holds member t's row and offsets
+        for (int64_t m0 = m_begin; m0 < m_end; m0 += 64) {
+            const int cnt = int(min<int64_t>(64, m_end - m0));
+            int64_t my_o0 = 0;
+            int my_len = 0;
+            if (lane < cnt) {
+                const int64_t row = perm[m0 + lane];
+                my_o0 = offs[row] - base;
+                my_len = int(offs[row + 1] - base - my_o0);
             }
-            __syncthreads();
+            if (__all(my_len <= 64)) {
+                // every row fits the lanes: the entries of CSR_AHEAD members are
+                // loaded together, then added member by member
+                for (int u0 = 0; u0 < cnt; u0 += CSR_AHEAD) {
+                    int cc[CSR_AHEAD], ln[CSR_AHEAD];
+                    F vv[CSR_AHEAD];
+#pragma unroll
+                    for (int q = 0; q < CSR_AHEAD; ++q) {
+                        cc[q] = 0;
+                        vv[q] = F(0);
+                        ln[q] = 0;
+                        if (u0 + q < cnt) {
+                            ln[q] = __builtin_amdgcn_readlane(my_len, u0 + q);
+                            const int64_t o0 = lane_value(my_o0, u0 + q);
+                            if (lane < ln[q]) {
+                                cc[q] = int(cols[o0 + lane] - base);
+                                vv[q] = vals[o0 + lane];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < CSR_AHEAD; ++q) {
+                        if (u0 + q < cnt) {
+                            const int len = ln[q];
+                            double* pg = acc + int((m0 + u0 + q - m_begin) % G) * d;
+                            const int prev = __shfl_up(cc[q], 1);
+                            const bool inc = lane == 0 || lane >= len || prev < cc[q];
+                            if (__all(inc)) {
+                                // strictly increasing columns: distinct, added at once
+                                if (lane < len) pg[cc[q]] = pg[cc[q]] + double(vv[q]);
+                            } else {
+                                for (int e = 0; e < len; ++e) {
+                                    const int c = __builtin_amdgcn_readlane(cc[q], e);
+                                    const F v = lane_value(vv[q], e);
+                                    if (lane == 0) pg[c] = pg[c] + double(v);
+                                }
+                            }
+                        }
+                    }
+                }
+                continue;
+            }
+            for (int64_t m = m0; m < m0 + cnt; ++m) {
+                const int64_t row = perm[m];
+                double* pg = acc + int((m - m_begin) % G) * d;
+                const int64_t o0 = offs[row] - base, o1 = offs[row + 1] - base;
+                bool sorted = true;
+                for (int64_t p = o0 + 1 + lane; p < o1; p += 64)
+                    if (cols[p] <= cols[p - 1]) sorted = false;
+                sorted = __all(sorted);
+                if (sorted) {
+                    for (int64_t p = o0 + lane; p < o1; p += 64) {
+                        const int64_t c = cols[p] - base;
+                        pg[c] = pg[c] + double(vals[p]);
+                    }
+                } else if (lane == 0) {
+                    for (int64_t p = o0; p < o1; ++p) {
+                        const int64_t c = cols[p] - base;
+                        pg[c] = pg[c] + double(vals[p]);
+                    }
+                }
+                __syncthreads();
+            }
         }
         for (int f = lane; f < d; f += 64) {
             double t = acc[f];
