@@ -75,15 +75,16 @@ __global__ __launch_bounds__(256) void csr_row_squares_kernel(const F* __restric
     }
 }
 
-// Ct (d x k) = C^T for coalesced reads over centroids: 64 x 64 tiles through LDS
+// Ct (d x kp) = C^T, rows padded with zeros to kp (a multiple of the 16-byte
+// vector), for coalesced vector reads over centroids: 64 x 64 tiles through LDS
 // (grid: ceil(d/64) x up to 65535 workgroups of 256 threads, striding over k)
 template <typename F>
 __global__ __launch_bounds__(256) void csr_transpose_kernel(const F* __restrict__ C, int64_t k,
-                                                            int64_t d, F* __restrict__ Ct) {
+                                                            int64_t kp, int64_t d, F* __restrict__ Ct) {
     __shared__ F tile[64][65];
     const int64_t f0 = int64_t(blockIdx.x) * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int64_t j0 = int64_t(blockIdx.y) * 64; j0 < k; j0 += int64_t(gridDim.y) * 64) {
+    for (int64_t j0 = int64_t(blockIdx.y) * 64; j0 < kp; j0 += int64_t(gridDim.y) * 64) {
         for (int r = ty; r < 64; r += 4) {
             const int64_t j = j0 + r, f = f0 + tx;
             if (j < k && f < d) tile[r][tx] = C[j * d + f];
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void csr_transpose_kernel(const F* __restrict_
         __syncthreads();
         for (int r = ty; r < 64; r += 4) {
             const int64_t f = f0 + r, j = j0 + tx;
-            if (j < k && f < d) Ct[f * k + j] = tile[tx][r];
+            if (j < kp && f < d) Ct[f * kp + j] = j < k ? tile[tx][r] : F(0);
         }
         __syncthreads();
     }
@@ -143,14 +144,32 @@ __device__ __forceinline__ double lane_value(double x, int u) {
     return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
 }
 
-// One wave per row.  Lane l accumulates the dots of centroids jb + l + 64 t
-// (t < JT) over the row's entries, which are loaded 64 at a time and broadcast
-// in stored order; then the lane's best (ascending j) and a wave reduction.
+// 16-byte vectors of F
+template <typename F>
+struct csr_vec;
+template <>
+struct csr_vec<float> {
+    static constexpr int N = 4;
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct csr_vec<double> {
+    static constexpr int N = 2;
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// One wave per row.  Lane l accumulates the dots of the VEC consecutive
+// centroids jb + VEC (l + 64 t) + e (t < JT, e < VEC) over the row's entries,
+// which are loaded 64 at a time and taken in stored order from their lanes;
+// each entry reads one 16-byte vector of its transposed-centroid row per t.
+// Then each lane's best and a wave reduction (ties to the lower j).
 template <typename F, int JT>
 __global__ __launch_bounds__(256) void csr_assign_kernel(
     const F* __restrict__ vals, const int64_t* __restrict__ cols, const int64_t* __restrict__ offs,
-    int64_t base, int64_t n, int k, const F* __restrict__ Ct, const F* __restrict__ cs,
+    int64_t base, int64_t n, int k, int kp, const F* __restrict__ Ct, const F* __restrict__ cs,
     const F* __restrict__ xs, int32_t* __restrict__ labels, F* __restrict__ closest) {
+    constexpr int VEC = csr_vec<F>::N;
+    typedef typename csr_vec<F>::type V;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
@@ -160,10 +179,12 @@ __global__ __launch_bounds__(256) void csr_assign_kernel(
         int bj = 0;
         bool has = false;
         F s0(0);  // s of centroid 0 (lane 0, first block)
-        for (int jb = 0; jb < k; jb += 64 * JT) {
-            F acc[JT];
+        for (int jb = 0; jb < k; jb += 64 * VEC * JT) {
+            F acc[JT][VEC];
 #pragma unroll
-            for (int t = 0; t < JT; ++t) acc[t] = F(0);
+            for (int t = 0; t < JT; ++t)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) acc[t][e] = F(0);
             for (int64_t p0 = o0; p0 < o1; p0 += 64) {
                 const int cnt = int(min<int64_t>(64, o1 - p0));
                 int my_c = 0;  // a column index (< d <= 4096)
@@ -176,27 +197,33 @@ __global__ __launch_bounds__(256) void csr_assign_kernel(
                     // entry u of the group, read from its lane into scalar registers
                     const int c = __builtin_amdgcn_readlane(my_c, u);
                     const F v = lane_value(my_v, u);
-                    const F* crow = Ct + int64_t(c) * k;
+                    const F* crow = Ct + int64_t(c) * kp;
 #pragma unroll
                     for (int t = 0; t < JT; ++t) {
-                        const int j = jb + 64 * t + lane;
+                        const int j = jb + VEC * (64 * t + lane);
                         if (j < k) {
-                            const F pr = v * crow[j];
-                            acc[t] = acc[t] + pr;
+                            const V cv = *reinterpret_cast<const V*>(crow + j);
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e) {
+                                const F pr = v * cv[e];
+                                acc[t][e] = acc[t][e] + pr;
+                            }
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int t = 0; t < JT; ++t) {
-                const int j = jb + 64 * t + lane;
-                if (j < k) {
-                    const F two_dot = F(2) * acc[t];
-                    const F s = cs[j] - two_dot;
-                    if (j == 0) s0 = s;
-                    csr_take(bs, bj, has, s, j);
+            for (int t = 0; t < JT; ++t)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const int j = jb + VEC * (64 * t + lane) + e;
+                    if (j < k) {
+                        const F two_dot = F(2) * acc[t][e];
+                        const F s = cs[j] - two_dot;
+                        if (j == 0) s0 = s;
+                        csr_take(bs, bj, has, s, j);
+                    }
                 }
-            }
         }
         // wave reduction of (bs, bj, has)
         for (int off = 32; off > 0; off >>= 1) {

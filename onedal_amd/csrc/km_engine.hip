@@ -859,10 +859,13 @@ static csr_view<F> csr_prepare(km_ctx* ctx, const F* vals, const int64_t* cols,
 template <typename F>
 static void csr_assign(km_ctx* ctx, const csr_view<F>& x, const F* C, int k, const F* xs,
                        int32_t* labels, F* closest) {
-    F* ct = ctx->csr_ct.get<F>(int64_t(k) * x.d);
+    constexpr int VEC = csr_vec<F>::N;
+    const int kp = (k + VEC - 1) / VEC * VEC;  // Ct row stride: whole 16-byte vectors
+    F* ct = ctx->csr_ct.get<F>(int64_t(kp) * x.d);
     F* cs = ctx->csr_cs.get<F>(k);
-    hipLaunchKernelGGL(csr_transpose_kernel<F>, dim3(unsigned((x.d + 63) / 64), unsigned(std::min((k + 63) / 64, 65535))),
-                       dim3(256), 0, ctx->stream, C, int64_t(k), x.d, ct);
+    hipLaunchKernelGGL(csr_transpose_kernel<F>,
+                       dim3(unsigned((x.d + 63) / 64), unsigned(std::min((kp + 63) / 64, 65535))),
+                       dim3(256), 0, ctx->stream, C, int64_t(k), int64_t(kp), x.d, ct);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(csr_centroid_squares_kernel<F>, dim3(unsigned(std::min(k, ctx->cu_count * 8))),
                        dim3(64), size_t(x.d) * sizeof(F), ctx->stream, C, int64_t(k), int(x.d), cs);
@@ -876,9 +879,10 @@ static void csr_assign(km_ctx* ctx, const csr_view<F>& x, const F* C, int k, con
     const int grid = grid_for(ctx, x.n * 64, 256, 8);
 #define KM_CSRA(JT_)                                                                                \
     hipLaunchKernelGGL((csr_assign_kernel<F, JT_>), dim3(grid), dim3(256), 0, ctx->stream, x.vals,   \
-                       x.cols, x.offs, x.base, x.n, k, ct, cs, xs, labels, closest)
-    if (k <= 64) KM_CSRA(1);
-    else if (k <= 128) KM_CSRA(2);
+                       x.cols, x.offs, x.base, x.n, k, kp, ct, cs, xs, labels, closest)
+    // centroids per lane and block: VEC * JT
+    if (k <= 64 * VEC) KM_CSRA(1);
+    else if (k <= 128 * VEC) KM_CSRA(2);
     else KM_CSRA(4);
 #undef KM_CSRA
     KM_LAUNCH_CHECK();
