@@ -653,6 +653,15 @@ def main():
         roof["traffic_detail"] = {"unit": "bytes per launch", "read": mt["read"],
                                   "write": mt["write"], "source": mt["source"],
                                   "hbm_gbs_at_avg_launch": round(mt["bytes"] / (avg_ms * 1e-3) / 1e9, 1)}
+    if wl["bf16"]:
+        # what a pure BF16 MFMA loop reaches on this device right now (the shader
+        # clock under matrix-core load sits below the one the published peak assumes)
+        ceil_tf, ceil_ghz = eng.measure_bf16_mfma_peak()
+        roof["measured_mfma_ceiling"] = {"tflops": round(ceil_tf, 1), "clock_ghz": round(ceil_ghz, 3),
+                                         "frac": round(achieved / ceil_tf, 4),
+                                         "probe": "km_measure_bf16_mfma_peak: 4 independent "
+                                                  "32x32x16 BF16 MFMA chains per wave, 8 waves "
+                                                  "per CU, no memory traffic"}
 
     compare = None
     if wl["bf16"] and not args.no_compare:

@@ -9,6 +9,8 @@
  *        implemented at train_kernel_lloyd_dense_dpc.cpp:76-207)
  *   km_lloyd_infer_f32 / _f64  <->  infer_kernel_gpu<Float, method::lloyd_dense, ...>
  *       (backend/gpu/infer_kernel.hpp:24-29, infer_kernel_dpc.cpp:36-107)
+ *   km_lloyd_train_csr_f32 / _f64, km_lloyd_infer_csr_f32 / _f64  <->  the same
+ *       kernels for method::lloyd_csr (train_kernel_lloyd_csr_dpc.cpp:76-242)
  *   km_context_create/destroy  <->  dal::backend::context_gpu + sycl::queue
  *       (cpp/oneapi/dal/backend/dispatcher.hpp:118-132): one context = one
  *       device + one in-order HIP stream.
@@ -98,6 +100,15 @@ km_status km_context_destroy(km_ctx* ctx);
 const char* km_last_error(const km_ctx* ctx);
 km_status km_context_set_option(km_ctx* ctx, int option, int64_t value);
 km_status km_get_stats(const km_ctx* ctx, km_stats* out);
+
+/* Measured BF16 matrix-core ceiling of the context's device: a loop of
+ * independent v_mfma_f32_32x32x16_bf16 chains (two workgroups of four waves per
+ * compute unit, no memory traffic) runs `iters` iterations per wave;
+ * *tflops_out = dense flops / elapsed time, *clock_ghz_out (may be NULL) = the
+ * shader clock one wave saw over its loop.  A measurement aid for the
+ * assignment's roofline, with no counterpart in the reference. */
+km_status km_measure_bf16_mfma_peak(km_ctx* ctx, int64_t iters, double* tflops_out,
+                                    double* clock_ghz_out);
 
 /* Device memory on the context's device, for callers that keep data there (the
  * C++ surface's device tables).  km_memcpy copies synchronously on the context's

@@ -38,6 +38,7 @@
 #include "km_common.hpp"
 #include "km_csr.hpp"
 #include "km_init.hpp"
+#include "km_probe.hpp"
 #include "km_rng.hpp"
 #include "km_refine.hpp"
 #include "km_update.hpp"
@@ -1796,6 +1797,47 @@ km_status km_get_stats(const km_ctx* ctx, km_stats* out) {
     if (!ctx || !out) return KM_INVALID_ARGUMENT;
     *out = ctx->stats;
     return KM_OK;
+}
+
+km_status km_measure_bf16_mfma_peak(km_ctx* ctx, int64_t iters, double* tflops_out,
+                                    double* clock_ghz_out) {
+    return km::guarded(ctx, [&] {
+        if (iters <= 0 || iters > (int64_t(1) << 24) || !tflops_out)
+            throw km::status_error(KM_INVALID_ARGUMENT, "iterations must lie in [1, 2^24]");
+        const int grid = ctx->cu_count * 2;
+        // operands: small bf16 values (exponent near 1), all finite
+        std::vector<uint32_t> h(2 * km::PROBE_WG_THREADS * 4);
+        for (size_t i = 0; i < h.size(); ++i) h[i] = 0x3F803F80u ^ ((uint32_t(i) * 2654435761u) & 0x00070007u);
+        uint32_t* src = ctx->tune_x.get<uint32_t>(h.size());
+        float* out = ctx->tune_c.get<float>(int64_t(grid) * km::PROBE_WG_THREADS);
+        long long* clk = ctx->tune_l.get<long long>(2);
+        KM_HIP(hipMemcpyAsync(src, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        KM_HIP(hipEventCreate(&e0));
+        KM_HIP(hipEventCreate(&e1));
+        auto launch = [&] {
+            hipLaunchKernelGGL(km::mfma_ceiling_kernel, dim3(grid), dim3(km::PROBE_WG_THREADS), 0,
+                               ctx->stream, reinterpret_cast<const km::uintx4*>(src), out, clk, int(iters));
+            KM_LAUNCH_CHECK();
+        };
+        launch();  // warm-up (clock ramp)
+        KM_HIP(hipEventRecord(e0, ctx->stream));
+        launch();
+        KM_HIP(hipEventRecord(e1, ctx->stream));
+        long long hc[2] = {0, 0};
+        KM_HIP(hipMemcpyAsync(hc, clk, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        KM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double flops = double(grid) * (km::PROBE_WG_THREADS / 64) * 4.0 * double(iters) * 32 * 32 * 16 * 2;
+        *tflops_out = flops / (double(ms) * 1e-3) / 1e12;
+        if (clock_ghz_out) *clock_ghz_out = hc[1] > 0 ? double(hc[0]) / (double(hc[1]) / 100e6) / 1e9 : 0.0;
+        ctx->tune_x.release();
+        ctx->tune_c.release();
+        ctx->tune_l.release();
+    });
 }
 
 km_status km_lloyd_train_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, const float* C0,

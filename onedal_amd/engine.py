@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "km_lloyd_infer_f64", "km_device_alloc", "km_device_free", "km_memcpy",
     "km_init_centroids_f32", "km_init_centroids_f64",
     "km_lloyd_train_csr_f32", "km_lloyd_train_csr_f64", "km_lloyd_infer_csr_f32",
-    "km_lloyd_infer_csr_f64",
+    "km_lloyd_infer_csr_f64", "km_measure_bf16_mfma_peak",
     "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
     "km_server_create",
     "km_server_destroy", "km_server_last_error", "km_server_submit_fit_f32",
@@ -171,6 +171,8 @@ def load_library():
         f = getattr(L, "km_init_centroids_" + sfx)
         f.argtypes = [P, P, I64, I64, I64, ctypes.c_int32, ctypes.c_uint64, U32, P]
         f.restype = I
+    L.km_measure_bf16_mfma_peak.argtypes = [P, I64, ctypes.POINTER(D), ctypes.POINTER(D)]
+    L.km_measure_bf16_mfma_peak.restype = I
     L.km_batch_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32), I64]
     L.km_batch_autotune.restype = I
     L.km_batch_step_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64,
@@ -252,6 +254,14 @@ class Engine:
 
     def set_option(self, option, value):
         self._check(self.lib.km_context_set_option(self.ctx, int(option), int(value)))
+
+    def measure_bf16_mfma_peak(self, iters=20000):
+        """(TFLOP/s, shader GHz) of a pure BF16 MFMA loop on this device."""
+        tf = ctypes.c_double()
+        ghz = ctypes.c_double()
+        self._check(self.lib.km_measure_bf16_mfma_peak(self.ctx, int(iters), ctypes.byref(tf),
+                                                       ctypes.byref(ghz)))
+        return tf.value, ghz.value
 
     def stats(self):
         s = KmStats()
@@ -584,6 +594,14 @@ class Server:
         return {"centroids": outs[0], "labels": outs[1], "objective": r.objective,
                 "iterations": r.iterations, "queue_ms": r.queue_ms, "latency_ms": r.latency_ms,
                 "lane": r.lane}
+
+    def measure_bf16_mfma_peak(self, iters=20000):
+        """(TFLOP/s, shader GHz) of a pure BF16 MFMA loop on this device."""
+        tf = ctypes.c_double()
+        ghz = ctypes.c_double()
+        self._check(self.lib.km_measure_bf16_mfma_peak(self.ctx, int(iters), ctypes.byref(tf),
+                                                       ctypes.byref(ghz)))
+        return tf.value, ghz.value
 
     def stats(self):
         s = KmServerStats()
