@@ -35,6 +35,7 @@
 
 #include "../../../onedal_amd/kmeans_hip.h"
 #include "../exceptions.hpp"
+#include "../archive.hpp"
 #include "../table.hpp"
 
 namespace oneapi::dal {
@@ -241,7 +242,68 @@ public:
     }
     std::int64_t get_cluster_count() const { return impl_->centroids.get_row_count(); }
 
+    // model<Task>::serialize / deserialize (common.cpp:44-56, 144-156): the
+    // reference's serialization id, then the centroid table (shape, element
+    // type, row-major bytes; device centroids are copied to the host).
+    // deserialize yields host centroids.
+    static constexpr std::int64_t serialization_id = 8010000000;  // kmeans_clustering_model_impl_id
+    template <typename Archive>
+    void serialize(Archive& ar) const {
+        const table& c = impl_->centroids;
+        const std::int64_t rows = c.has_data() ? c.get_row_count() : 0;
+        const std::int64_t cols = c.has_data() ? c.get_column_count() : 0;
+        const std::int32_t dtype = std::int32_t(c.get_data_type());
+        ar(serialization_id);
+        ar(rows);
+        ar(cols);
+        ar(dtype);
+        const std::int64_t bytes = rows * cols * std::int64_t(oneapi::dal::detail::size_of(c.get_data_type()));
+        if (bytes == 0) return;
+        if (!c.is_device()) {
+            ar.write(c.get_raw_data(), bytes);
+            return;
+        }
+        std::vector<unsigned char> host(static_cast<std::size_t>(bytes));
+        km_ctx* ctx = nullptr;
+        if (km_context_create(c.get_device(), &ctx) != KM_OK)
+            throw internal_error("no HIP context for the device holding the centroids");
+        const km_status st = km_memcpy(ctx, host.data(), c.get_raw_data(), bytes, KM_COPY_DEVICE_TO_HOST);
+        km_context_destroy(ctx);
+        if (st != KM_OK) throw internal_error("device copy of the centroids failed");
+        ar.write(host.data(), bytes);
+    }
+    template <typename Archive>
+    void deserialize(Archive& ar) {
+        std::int64_t id = 0, rows = 0, cols = 0;
+        std::int32_t dtype = 0;
+        ar(id);
+        if (id != serialization_id) throw invalid_argument("Archive does not hold a k-means model");
+        ar(rows);
+        ar(cols);
+        ar(dtype);
+        if (rows < 0 || cols < 0 || dtype < 0 || dtype > 2)
+            throw invalid_argument("Archive holds a malformed centroid table");
+        table c;
+        if (rows > 0 && cols > 0) {
+            switch (data_type(dtype)) {
+                case data_type::float32: c = read_table<float>(ar, rows, cols); break;
+                case data_type::float64: c = read_table<double>(ar, rows, cols); break;
+                case data_type::int32: c = read_table<std::int32_t>(ar, rows, cols); break;
+            }
+        }
+        impl_ = std::make_shared<impl>();
+        impl_->centroids = c;
+    }
+
 private:
+    template <typename T, typename Archive>
+    static table read_table(Archive& ar, std::int64_t rows, std::int64_t cols) {
+        if (cols > 0 && rows > std::numeric_limits<std::int64_t>::max() / cols / std::int64_t(sizeof(T)))
+            throw invalid_argument("Archive holds a malformed centroid table");
+        array<T> a(rows * cols);
+        ar.read(a.get_mutable_data(), rows * cols * std::int64_t(sizeof(T)));
+        return homogen_table::wrap(a, rows, cols);
+    }
     struct impl {
         table centroids;
     };

@@ -310,9 +310,48 @@ static void csr_checks(dal::hip::queue& q) {
     CHECK(km_device_free(q.get_context(), dro) == KM_OK);
 }
 
+// kmeans/test/serialization.cpp:104-116: train, serialize, deserialize, and
+// inference with the original and the restored model agree (host and device
+// centroids)
+template <typename F>
+static void serialization_infer(dal::hip::queue& q) {
+    const int64_t n = 3000, d = 10, k = 7;
+    std::vector<F> x(n * d);
+    for (int64_t i = 0; i < n * d; ++i) x[i] = F(((i * 7919ull) % 1000) / 50.0) + F((i / d) % 7) * F(4);
+    const auto xt = dal::homogen_table::wrap(x.data(), n, d);
+    const auto desc = km::descriptor<F>(k).set_max_iteration_count(8).set_result_options(
+        km::result_options::compute_exact_objective_function);
+    const auto tr = dal::train(q, desc, xt, dal::homogen_table::wrap(x.data(), k, d));
+    void* dx = nullptr;
+    CHECK(km_device_alloc(q.get_context(), int64_t(sizeof(F)) * n * d, &dx) == KM_OK);
+    CHECK(km_memcpy(q.get_context(), dx, x.data(), int64_t(sizeof(F)) * n * d, KM_COPY_HOST_TO_DEVICE) == KM_OK);
+    const auto dtr = dal::train(q, desc, dal::homogen_table::wrap(q, static_cast<const F*>(dx), n, d),
+                                dal::homogen_table::wrap(x.data(), k, d));
+    for (const auto* m : {&tr.get_model(), &dtr.get_model()}) {
+        dal::detail::binary_output_archive out;
+        dal::detail::serialize(*m, out);
+        const auto bytes = out.to_array();
+        km::model<> back;
+        dal::detail::binary_input_archive in(bytes.data(), int64_t(bytes.size()));
+        dal::detail::deserialize(back, in);
+        CHECK(!back.get_centroids().is_device());
+        const auto a = dal::infer(q, desc, *m, xt);
+        const auto b = dal::infer(q, desc, back, xt);
+        CHECK(a.get_objective_function_value() == b.get_objective_function_value());
+        const auto al = dal::row_accessor<const int>(a.get_responses()).pull(q);
+        const auto bl = dal::row_accessor<const int>(b.get_responses()).pull(q);
+        bool same = true;
+        for (int64_t i = 0; i < n; ++i) same = same && al[i] == bl[i];
+        CHECK(same);
+    }
+    CHECK(km_device_free(q.get_context(), dx) == KM_OK);
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     dal::hip::queue q(0);
+    serialization_infer<float>(q);
+    serialization_infer<double>(q);
     csr_checks<float>(q);
     csr_checks<double>(q);
     device_tables<float>(q);

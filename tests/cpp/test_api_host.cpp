@@ -40,8 +40,60 @@ static void expect_throw(const std::function<void()>& fn, const char* msg, int l
 }
 #define EXPECT_THROW(E, expr, msg) expect_throw<E>([&] { expr; }, msg, __LINE__)
 
+// model serialization round trip (the reference's kmeans/test/serialization.cpp
+// trains, serializes, deserializes and compares inference; the inference half
+// runs in test_api_gpu.cpp)
+static void serialization_checks() {
+    const double cent[] = {1.5, -2.0, 3.25, 0.0, 7.0, -1e-300};
+    km::model<> m;
+    m.set_centroids(dal::homogen_table::wrap(cent, 2, 3));
+    dal::detail::binary_output_archive out;
+    dal::detail::serialize(m, out);
+    const auto bytes = out.to_array();
+    CHECK(out.get_size() == int64_t(bytes.size()));
+    km::model<> back;
+    dal::detail::binary_input_archive in(bytes.data(), int64_t(bytes.size()));
+    dal::detail::deserialize(back, in);
+    const auto& c = back.get_centroids();
+    CHECK(c.get_row_count() == 2 && c.get_column_count() == 3);
+    CHECK(c.get_data_type() == dal::data_type::float64);
+    const auto v = dal::row_accessor<const double>(c).pull();
+    for (int i = 0; i < 6; ++i) CHECK(std::memcmp(&v[i], &cent[i], sizeof(double)) == 0);
+    CHECK(back.get_cluster_count() == 2);
+    // a truncated archive, and one that holds something else, are refused
+    bool threw = false;
+    try {
+        km::model<> t;
+        dal::detail::binary_input_archive cut(bytes.data(), int64_t(bytes.size()) - 1);
+        dal::detail::deserialize(t, cut);
+    } catch (const dal::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    auto other = bytes;
+    other[0] ^= 1;
+    threw = false;
+    try {
+        km::model<> t;
+        dal::detail::binary_input_archive wrong(other.data(), int64_t(other.size()));
+        dal::detail::deserialize(t, wrong);
+    } catch (const dal::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    // an empty model round-trips to an empty model
+    km::model<> empty, empty_back;
+    dal::detail::binary_output_archive eo;
+    dal::detail::serialize(empty, eo);
+    const auto eb = eo.to_array();
+    dal::detail::binary_input_archive ei(eb.data(), int64_t(eb.size()));
+    dal::detail::deserialize(empty_back, ei);
+    CHECK(!empty_back.get_centroids().has_data());
+}
+
 int main() {
     using namespace dal::detail::error_messages;
+    serialization_checks();
     // descriptor defaults and setters
     km::descriptor<float> d;
     CHECK(d.get_cluster_count() == 2);
