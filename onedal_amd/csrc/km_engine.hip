@@ -1812,9 +1812,8 @@ km_status km_measure_bf16_mfma_peak(km_ctx* ctx, int64_t iters, double* tflops_o
         float* out = ctx->tune_c.get<float>(int64_t(grid) * km::PROBE_WG_THREADS);
         long long* clk = ctx->tune_l.get<long long>(2);
         KM_HIP(hipMemcpyAsync(src, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        KM_HIP(hipEventCreate(&e0));
-        KM_HIP(hipEventCreate(&e1));
+        ctx->ev_used = 0;
+        hipEvent_t e0 = ctx->event(), e1 = ctx->event();
         auto launch = [&] {
             hipLaunchKernelGGL(km::mfma_ceiling_kernel, dim3(grid), dim3(km::PROBE_WG_THREADS), 0,
                                ctx->stream, reinterpret_cast<const km::uintx4*>(src), out, clk, int(iters));
@@ -1829,8 +1828,6 @@ km_status km_measure_bf16_mfma_peak(km_ctx* ctx, int64_t iters, double* tflops_o
         KM_HIP(hipStreamSynchronize(ctx->stream));
         float ms = 0.f;
         KM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         const double flops = double(grid) * (km::PROBE_WG_THREADS / 64) * 4.0 * double(iters) * 32 * 32 * 16 * 2;
         *tflops_out = flops / (double(ms) * 1e-3) / 1e12;
         if (clock_ghz_out) *clock_ghz_out = hc[1] > 0 ? double(hc[0]) / (double(hc[1]) / 100e6) / 1e9 : 0.0;

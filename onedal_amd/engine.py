@@ -219,6 +219,30 @@ def _is_torch_cuda(a):
     return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
 
 
+def _check_csr(values, cols, offs):
+    """True for device arrays (all three CUDA tensors: contiguous, int64 indices),
+    False for host arrays; array lengths must agree (values and column indices of
+    one length, at least one row offset)."""
+    on_dev = [_is_torch_cuda(a) for a in (values, cols, offs)]
+    if any(on_dev) and not all(on_dev):
+        raise TypeError("CSR arrays must all be host arrays or all CUDA tensors")
+    if len(values.shape) != 1 or len(cols.shape) != 1 or len(offs.shape) != 1:
+        raise ValueError("CSR arrays must be one-dimensional")
+    if int(values.shape[0]) != int(cols.shape[0]):
+        raise ValueError("values and column indices must have the same length")
+    if int(offs.shape[0]) < 2:
+        raise ValueError("row offsets need at least two entries")
+    if all(on_dev):
+        import torch
+
+        if cols.dtype != torch.int64 or offs.dtype != torch.int64:
+            raise TypeError("device column indices and row offsets must be int64 tensors")
+        if not (values.is_contiguous() and cols.is_contiguous() and offs.is_contiguous()):
+            raise ValueError("device CSR arrays must be contiguous")
+        return True
+    return False
+
+
 class Engine:
     """One HIP device context (km_ctx) and its stream."""
 
@@ -310,7 +334,7 @@ class Engine:
     def train_csr(self, values, cols, offs, d, c0, max_iter, accuracy_threshold=0.0, base=0):
         """CSR arrays in (host numpy, or CUDA tensors on this device: then the
         outputs are CUDA tensors too): (centroids, labels, objective, iterations)."""
-        dev = _is_torch_cuda(values) or _is_torch_cuda(offs)
+        dev = _check_csr(values, cols, offs)
         n = int(offs.shape[0]) - 1
         nnz = int(values.shape[0])
         k = int(c0.shape[0])
@@ -345,7 +369,7 @@ class Engine:
 
     def infer_csr(self, values, cols, offs, d, c, objective=False, base=0):
         """Labels (and the objective if asked) of CSR rows against centroids c."""
-        dev = _is_torch_cuda(values) or _is_torch_cuda(offs)
+        dev = _check_csr(values, cols, offs)
         n = int(offs.shape[0]) - 1
         nnz = int(values.shape[0])
         k = int(c.shape[0])
