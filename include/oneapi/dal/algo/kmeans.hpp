@@ -590,6 +590,9 @@ template <typename Float>
 array<Float> csr_plus_plus(hip::queue& q, const table& t, std::int64_t k) {
     const csr_table c(t);
     const std::int64_t n = c.get_row_count(), d = c.get_column_count();
+    if (double(n) * double(d) * sizeof(Float) > double(std::int64_t(1) << 32))
+        throw unimplemented("Default initial centroids of CSR data need the rows expanded on the host; "
+                            "above 4 GiB expanded, pass initial centroids");
     const std::int64_t base = c.get_indexing() == sparse_indexing::one_based ? 1 : 0;
     std::vector<std::int64_t> offs(std::size_t(n + 1));
     std::vector<std::int64_t> cols(std::size_t(c.get_non_zero_count()));
