@@ -339,7 +339,7 @@ __host__ inline size_t bf16_collect_lds_bytes(int dpb, int rows_per_wg) {
 // Lane l = 32*h + r: half h covers features [h*DPB/2, (h+1)*DPB/2), 8 per step.
 // Slots: lane keeps 4 (centroid block index mod 4); a row has 8.
 // The key work of block mb runs while the MFMAs of block mb+1 execute (two
-// accumulator sets), interleaved by scheduling groups.
+// accumulator sets), interleaved by the compiler.
 // Cb and cnb_g hold bf16_padded_k(k) rows, a multiple of four blocks.
 // COLLECT: rows come from L.coll_rows; every j with v_j <= T is recorded in
 // per-row LDS lists (the wave that owns a row is the only writer, so no
@@ -366,8 +366,6 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
     constexpr int RING = bf16_ring(DPB, COLLECT);  // LDS slots
     constexpr int GRP = RING >= 8 ? 2 : 1;         // blocks per barrier
     constexpr int AHEAD = RING - GRP;              // blocks copied ahead of use
-    // VALU ops of one block's key work (5 per pair of distances) per MFMA
-    constexpr int VPM = (NB * 40 + STEPS * NB - 1) / (STEPS * NB);
     static_assert(NDATA >= 2 && (NDATA < 4 || NDATA % 4 == 0), "block copy layout");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char* ring = smem_raw;                                 // [RING][BLKB]
@@ -551,18 +549,6 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                 }
             }
         };
-        // LDS reads of the block first; the previous block's key work then fills
-        // the issue slots between the MFMAs
-        auto interleave = [&]() {
-            __builtin_amdgcn_sched_group_barrier(0x100, STEPS + 4, 0);  // LDS reads
-            __builtin_amdgcn_sched_group_barrier(0x002, 2 * VPM, 0);     // VALU
-#pragma unroll
-            for (int i = 0; i < STEPS * NB; ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);       // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);     // VALU
-            }
-        };
-
         // step mb (u = mb mod 4): at the first block of a group, wait for the
         // group's copies (the next AHEAD - GRP blocks may stay in flight), barrier
         // (all waves' copies visible; everyone is done with the previous group),
@@ -581,9 +567,10 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
                     issue_block(min(mb + AHEAD + g, mblocks - 1), (mb + AHEAD + g) & (RING - 1));
                 __builtin_amdgcn_sched_barrier(0);
             }
+            // the compiler interleaves the key work of block mb - 1 with the
+            // MFMAs of block mb (explicit scheduling groups measured slower)
             mfma_block(mb & (RING - 1), cur);
             keys_block(prev, mb - 1, (u + 3) & 3);
-            if (!COLLECT) interleave();
         };
         // row loads complete here (a waitcnt the compiler sees, so it does not re-wait
         // for them inside the block loop, where vmcnt also counts the ring copies);
