@@ -558,11 +558,14 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
         floatx16 acc0[NB], acc1[NB];
         auto step = [&](int mb, int u, floatx16(&cur)[NB], floatx16(&prev)[NB]) {
             if (u % GRP == 0) {
+                // nothing is scheduled across the wait, the barrier and the copies
+                __builtin_amdgcn_sched_barrier(0);
                 wait_block();
                 __syncthreads();
 #pragma unroll
                 for (int g = 0; g < GRP; ++g)
                     issue_block(min(mb + AHEAD + g, mblocks - 1), (mb + AHEAD + g) & (RING - 1));
+                __builtin_amdgcn_sched_barrier(0);
             }
             // the compiler interleaves the key work of block mb - 1 with the
             // MFMAs of block mb (explicit scheduling groups measured slower)
