@@ -145,3 +145,25 @@ def test_structure_checks(engine):
     # the context stays usable
     labels, _ = engine.infer_csr(vals, np.array([0, 1, 2, 2]), np.array([0, 2, 4]), 3, c0)
     assert labels.tolist() == [0, 0]
+
+
+def test_nan_semantics(engine, oracle):
+    """NaN distances follow the sequential scan "best = s_0; if s_j < best": a NaN
+    s_0 keeps label 0, a later NaN never wins, a NaN row value gives label 0."""
+    from oracle.oracle import Csr
+
+    x, _ = sparse_blobs(800, 24, 4, 0.3, seed=71, dtype=np.float64)
+    used = int(x.cols[0])  # a column that row 0 (and its cluster) stores
+    for j_nan in (0, 2):
+        c = x.to_dense()[:4].copy()
+        c[j_nan, used] = np.nan
+        labels, _ = engine.infer_csr(x.values, x.cols, x.offs, x.d, c)
+        rl, _ = oracle.csr_assign(x, c)
+        assert np.array_equal(labels, rl)
+    vals = x.values.copy()
+    vals[x.offs[3]] = np.nan  # row 3's first entry
+    y = Csr(vals, x.cols, x.offs, x.d, 0)
+    c = x.to_dense()[:4].copy()
+    labels, _ = engine.infer_csr(y.values, y.cols, y.offs, y.d, c)
+    rl, _ = oracle.csr_assign(y, c)
+    assert np.array_equal(labels, rl) and labels[3] == 0
