@@ -33,6 +33,9 @@ CASES = [
     (20000, 1000, 300, 0.1, 5, 0.0, np.float32, 0),
     (5000, 33, 70, 0.5, 8, 0.0, np.float64, 1),
     (3000, 7, 3, 1.0, 6, 0.0, np.float32, 0),
+    # four 16-byte vectors per lane (k above 512 floats / 256 doubles)
+    (6000, 64, 600, 0.2, 4, 0.0, np.float32, 1),
+    (4000, 50, 300, 0.2, 4, 0.0, np.float64, 0),
 ]
 
 
