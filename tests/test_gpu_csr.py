@@ -170,3 +170,25 @@ def test_nan_semantics(engine, oracle):
     labels, _ = engine.infer_csr(y.values, y.cols, y.offs, y.d, c)
     rl, _ = oracle.csr_assign(y, c)
     assert np.array_equal(labels, rl) and labels[3] == 0
+
+
+def test_mixed_row_lengths(engine, oracle):
+    """Rows of 0 to 100 stored entries in the same clusters: member groups of the
+    chunk sums switch between the one-lane-per-entry path (rows of at most 64
+    entries) and the general path (longer rows) inside one chunk."""
+    from oracle.oracle import Csr
+
+    x, _ = sparse_blobs(12000, 100, 6, 1.0, seed=81)
+    rng = np.random.default_rng(82)
+    keep = rng.integers(0, 101, size=x.n)
+    vals, cols, offs = [], [], [0]
+    for i in range(x.n):
+        a = x.offs[i]
+        b = a + min(int(keep[i]), int(x.offs[i + 1] - a))
+        vals.append(x.values[a:b])
+        cols.append(x.cols[a:b])
+        offs.append(offs[-1] + (b - a))
+    y = Csr(np.concatenate(vals), np.concatenate(cols), np.array(offs), x.d, 0)
+    y = reorder(y, seed=83, base=1)
+    c0 = y.to_dense()[rng.choice(x.n, 6, replace=False)].copy()
+    check_train(engine, oracle, y, c0, 5, 0.0)
