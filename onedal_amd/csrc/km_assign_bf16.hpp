@@ -263,24 +263,25 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // 16 bytes per lane from global memory straight into LDS: lane l reads
 // gbase + voff (scalar 64-bit base, 32-bit lane offset) and writes
 // lds_base + 16 l (wave-uniform, through M0).  Issued as inline asm so the
-// compiler does not insert its own conservative vmcnt(0) before every LDS read;
-// ordering is by the explicit vmcnt waits and barriers of the caller.
+// compiler does not insert its own conservative waits (a full lgkmcnt wait before
+// the block's LDS reads, vmcnt(0) at barriers); ordering is by the explicit vmcnt
+// waits and barriers of the caller.  M0 is an input operand: the compiler writes
+// it before the asm and knows the asm leaves it unchanged.
 // The compiler's hazard recognizer does not look inside inline asm, so the asm
 // avoids the hazards itself: the base SGPRs typically come from v_readfirstlane
 // just before, and a VMEM read of an SGPR written by a VALU op needs 5 wait
 // states (otherwise the load may use the register's previous contents as the
 // address).  The load therefore reads a copy made by a scalar move (no wait
-// states after an SALU write), and that move is also the one wait state that
-// M0 -> LDS DMA needs.
+// states after an SALU write), and that move is also the one wait state that an
+// M0 write -> LDS DMA needs.
 __device__ __forceinline__ void lds_dma16(const void* gbase, uint32_t voff, uint32_t lds_base) {
     uint64_t base;
     asm volatile(
-        "s_mov_b32 m0, %1\n\t"
         "s_mov_b64 %0, %3\n\t"
-        "global_load_lds_dwordx4 %2, %0"
+        "global_load_lds_dwordx4 %1, %0"
         : "=&s"(base)
-        : "s"(lds_base), "v"(voff), "s"(gbase)
-        : "memory", "m0");
+        : "v"(voff), "{m0}"(lds_base), "s"(gbase)
+        : "memory");
 }
 
 // wait until at most N vector-memory operations of this wave are outstanding
@@ -294,12 +295,11 @@ __device__ __forceinline__ void wait_vm() {
 __device__ __forceinline__ void lds_dma4(const void* gbase, uint32_t voff, uint32_t lds_base) {
     uint64_t base;
     asm volatile(
-        "s_mov_b32 m0, %1\n\t"
         "s_mov_b64 %0, %3\n\t"
-        "global_load_lds_dword %2, %0"
+        "global_load_lds_dword %1, %0"
         : "=&s"(base)
-        : "s"(lds_base), "v"(voff), "s"(gbase)
-        : "memory", "m0");
+        : "v"(voff), "{m0}"(lds_base), "s"(gbase)
+        : "memory");
 }
 
 // Re-score lists of the main kernel, one segment of `cap` entries per wave
