@@ -11,6 +11,19 @@ import pytest
 from conftest import gaussian_mixture
 from test_oracle_golden import load_gold, match_map, rel_close
 
+
+def band_mask(oracle, x, c):
+    """Rows whose FP64 top-2 gap against centroids c is below
+    eps * (|x|^2 + |c_1|^2), eps = 4 d u (u = 2^-24 for float, 2^-53 for double):
+    the rows whose label two correct F-precision evaluations may order differently."""
+    lab, b1, b2 = oracle.fp64_assign(x, c)
+    x64 = x.astype(np.float64)
+    c64 = c.astype(np.float64)
+    xn = np.einsum("ij,ij->i", x64, x64)
+    cn = np.einsum("ij,ij->i", c64, c64)
+    u = 2.0 ** -24 if x.dtype == np.float32 else 2.0 ** -53
+    return (b2 - b1) < 4.0 * x.shape[1] * u * (xn + cn[lab])
+
 pytestmark = pytest.mark.gpu
 
 PATHS = {"generic": 1, "mfma_f32": 2, "bf16": 0}
@@ -106,9 +119,13 @@ def test_train_vs_daal_restatement(engine, oracle, dtype):
     c, lab, obj, it = engine.train(x, c0, 50, 0.0)
     dc, dl, do, di = oracle.daal_lloyd(x, c0, 50, 0.0, stop_rule=2, block_size=512)
     assert it == 50
-    _, b1, b2 = oracle.fp64_assign(x, dc)
-    resolvable = (b2 - b1) > 1e-3 * (b1 + 1.0)
-    assert np.array_equal(lab[resolvable], dl[resolvable])
+    # labels equal except rows whose FP64 top-2 gap is inside the band
+    # eps * (|x|^2 + |c_1|^2), eps = 4 d 2^-24 (F = float; 4 d 2^-53 for double)
+    band = band_mask(oracle, x, dc)
+    assert band.mean() < 1e-3  # almost every row is compared
+    assert np.array_equal(lab[~band], dl[~band])
+    print("daal restatement: %d labels differ, %d rows inside the band" % (int((lab != dl).sum()),
+                                                                          int(band.sum())))
     assert np.allclose(c, dc, rtol=1e-5, atol=1e-5)
     assert abs(obj - do) / do < 1e-5
 

@@ -144,11 +144,18 @@ def _make_mix(seed):
 LARGE_WORK = 1.5e8
 
 
-def _expected(engine, j):
+def _expected(engine, oracle, j):
+    """The job's result from the oracle (the engine's contract restated on the
+    CPU), checked first against the single-instance engine on the job alone."""
     if j["kind"] == "fit":
-        c, lab, ob, it = engine.train(j["x"], j["c0"], j["iters"], 0.0)
+        c, lab, ob, it, _ = oracle.canonical_lloyd(j["x"], j["c0"], j["iters"], 0.0)
+        got = engine.train(j["x"], j["c0"], j["iters"], 0.0)
+        assert _same(got[0], c) and np.array_equal(got[1], lab) and got[2] == ob and got[3] == it
         return c, lab, ob, it
-    lab, ob = engine.infer(j["x"], j["c"], objective=True)
+    lab, _ = oracle.canonical_assign(j["x"], j["c"])
+    ob = oracle.canonical_objective(j["x"], j["c"], lab)
+    got = engine.infer(j["x"], j["c"], objective=True)
+    assert np.array_equal(got[0], lab) and got[1] == ob
     return None, lab, ob, 0
 
 
@@ -172,7 +179,7 @@ def _host(a):
     return a.cpu().numpy() if hasattr(a, "cpu") else a
 
 
-def _check(j, res, want):
+def _check(j, res, want, large_work=None):
     c, lab, ob, it = want
     assert np.array_equal(_host(res["labels"]), lab)
     assert res["objective"] == ob
@@ -181,16 +188,16 @@ def _check(j, res, want):
         assert _same(_host(res["centroids"]), c)
     n, d = j["x"].shape
     k = (j["c0"] if j["kind"] == "fit" else j["c"]).shape[0]
-    assert res["lane"] == (1 if n * k * d >= LARGE_WORK else 0)
+    assert res["lane"] == (1 if n * k * d >= (large_work or LARGE_WORK) else 0)
 
 
-def test_server_mix_equals_alone(engine):
+def test_server_mix_equals_alone(engine, oracle):
     """Jobs submitted from three threads with staggered arrivals (the mix) and the
-    same jobs one at a time (alone) give the single-instance results bit for bit."""
+    same jobs one at a time (alone) give the oracle's results bit for bit."""
     from onedal_amd.engine import Server
 
     jobs = _make_mix(3)
-    want = [_expected(engine, j) for j in jobs]
+    want = [_expected(engine, oracle, j) for j in jobs]
     ids = [None] * len(jobs)
     with Server(budget_bytes=8 << 30, large_job_work=int(LARGE_WORK)) as srv:
         errors = []
@@ -224,7 +231,7 @@ def test_server_mix_equals_alone(engine):
         assert srv.stats()["bytes_admitted"] == 0
 
 
-def test_server_admission_budget(engine):
+def test_server_admission_budget(engine, oracle):
     """A budget of a few jobs' footprints: every job still completes with the
     single-instance result, the admitted footprint never exceeds the budget, and a
     job larger than the whole budget is refused at submission."""
@@ -243,7 +250,7 @@ def test_server_admission_budget(engine):
         assert stats["peak_bytes_admitted"] <= budget
         assert stats["completed"] == len(jobs)
         for j, res in zip(jobs, results):
-            _check(j, res, _expected(engine, j))
+            _check(j, res, _expected(engine, oracle, j))
         assert max(r["queue_ms"] for r in results) > 0.0
         big = gaussian_mixture(200000, 16, 4, seed=1)
         with pytest.raises(EngineError) as e:
@@ -254,7 +261,7 @@ def test_server_admission_budget(engine):
         r = srv.wait(jid, 0)
         if r is None:
             r = srv.wait(jid, WAIT_MS)
-        _check(jobs[0], r, _expected(engine, jobs[0]))
+        _check(jobs[0], r, _expected(engine, oracle, jobs[0]))
 
 
 def test_server_badargs():
@@ -291,3 +298,48 @@ def test_server_close_finishes_outstanding_jobs(engine):
     for x, c0, (cout, labels) in jobs:
         c, lab, _, _ = engine.train(x, c0, 4, 0.0)
         assert np.array_equal(labels, lab) and _same(cout, c)
+
+
+def test_server_serving_recipe_vs_oracle(engine, oracle):
+    """The serving workload's recipe at reduced sizes: Poisson arrivals, 70% fit
+    jobs of 20 iterations and 30% infer jobs, n log-uniform (here [1e3, 3e4]), d
+    from {8, 16, 32, 64, 128}, k uniform in [4, 128], Gaussian-mixture data with k
+    centres per job, BF16+refine requested, plus large jobs (here n=6e4, d=128,
+    k=512) on the large-job lane; every result equals the oracle bit for bit."""
+    from onedal_amd.engine import Server
+
+    rng = np.random.default_rng(2024)
+    jobs = []
+    for i in range(40):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(3e4))))
+        d = int(rng.choice([8, 16, 32, 64, 128]))
+        k = int(rng.integers(4, 129))
+        x = gaussian_mixture(n, d, k, seed=5000 + i, center_range=10.0)
+        if rng.uniform() < 0.7:
+            c0 = x[rng.choice(n, size=k, replace=False)].copy()
+            jobs.append(dict(kind="fit", x=x, c0=c0, iters=20))
+        else:
+            c = gaussian_mixture(k, d, k, seed=6000 + i, center_range=10.0)
+            jobs.append(dict(kind="infer", x=x, c=c))
+    for i in range(2):
+        x = gaussian_mixture(60000, 128, 512, seed=7000 + i, center_range=10.0)
+        c0 = x[rng.choice(60000, size=512, replace=False)].copy()
+        jobs.insert(10 + 15 * i, dict(kind="fit", x=x, c0=c0, iters=20))
+    for i, j in enumerate(jobs):
+        j["device"] = (i % 2 == 0)
+        j["bf16"] = True
+    want = [_expected(engine, oracle, j) for j in jobs]
+    large = 1e9
+    with Server(budget_bytes=4 << 30, large_job_work=int(large)) as srv:
+        ids = []
+        for j in jobs:
+            ids.append(_submit(srv, j))
+            time.sleep(rng.exponential(1.0 / 400.0))
+        results = [srv.wait(i, WAIT_MS) for i in ids]
+        stats = srv.stats()
+    for j, res, w in zip(jobs, results, want):
+        assert res is not None
+        _check(j, res, w, large_work=large)
+    assert stats["completed"] == len(jobs) and stats["failed"] == 0
+    assert stats["large_jobs"] == 2
+    assert stats["max_step_jobs"] >= 2, stats
