@@ -242,25 +242,31 @@ public:
     }
     std::int64_t get_cluster_count() const { return impl_->centroids.get_row_count(); }
 
-    // model<Task>::serialize / deserialize (common.cpp:44-56, 144-156): the
-    // reference's serialization id, then the centroid table (shape, element
-    // type, row-major bytes; device centroids are copied to the host).
-    // deserialize yields host centroids.
-    static constexpr std::int64_t serialization_id = 8010000000;  // kmeans_clustering_model_impl_id
+    // model<Task>::serialize / deserialize (common.cpp:44-56, 144-156), in the
+    // reference's archive format: the model implementation as a polymorphic member
+    // (present flag, kmeans_clustering_model_impl_id), whose one field is the
+    // centroid table (an empty table, or a row-major homogeneous table).  Device
+    // centroids are copied to the host first; deserialize yields host centroids.
+    static constexpr std::int64_t serialization_id =
+        std::int64_t(oneapi::dal::detail::serialization_ids::kmeans_clustering_model_impl_id);
     template <typename Archive>
     void serialize(Archive& ar) const {
+        namespace dd = oneapi::dal::detail;
         const table& c = impl_->centroids;
-        const std::int64_t rows = c.has_data() ? c.get_row_count() : 0;
-        const std::int64_t cols = c.has_data() ? c.get_column_count() : 0;
-        const std::int32_t dtype = std::int32_t(c.get_data_type());
-        ar(serialization_id);
-        ar(rows);
-        ar(cols);
-        ar(dtype);
-        const std::int64_t bytes = rows * cols * std::int64_t(oneapi::dal::detail::size_of(c.get_data_type()));
-        if (bytes == 0) return;
+        ar(std::uint8_t(1));
+        ar(dd::serialization_ids::kmeans_clustering_model_impl_id);
+        if (!c.has_data()) {
+            dd::write_table(ar, 0, 0, 0, 0, nullptr);
+            return;
+        }
+        const std::int64_t rows = c.get_row_count(), cols = c.get_column_count();
+        const std::int64_t elem = std::int64_t(dd::size_of(c.get_data_type()));
+        const std::int32_t code = c.get_data_type() == data_type::float32   ? dd::archived_type::float32
+                                  : c.get_data_type() == data_type::float64 ? dd::archived_type::float64
+                                                                            : dd::archived_type::int32;
+        const std::int64_t bytes = rows * cols * elem;
         if (!c.is_device()) {
-            ar.write(c.get_raw_data(), bytes);
+            dd::write_table(ar, rows, cols, code, elem, c.get_raw_data());
             return;
         }
         std::vector<unsigned char> host(static_cast<std::size_t>(bytes));
@@ -270,39 +276,29 @@ public:
         const km_status st = km_memcpy(ctx, host.data(), c.get_raw_data(), bytes, KM_COPY_DEVICE_TO_HOST);
         km_context_destroy(ctx);
         if (st != KM_OK) throw internal_error("device copy of the centroids failed");
-        ar.write(host.data(), bytes);
+        dd::write_table(ar, rows, cols, code, elem, host.data());
     }
     template <typename Archive>
     void deserialize(Archive& ar) {
-        std::int64_t id = 0, rows = 0, cols = 0;
-        std::int32_t dtype = 0;
-        ar(id);
-        if (id != serialization_id) throw invalid_argument("Archive does not hold a k-means model");
-        ar(rows);
-        ar(cols);
-        ar(dtype);
-        if (rows < 0 || cols < 0 || dtype < 0 || dtype > 2)
-            throw invalid_argument("Archive holds a malformed centroid table");
+        namespace dd = oneapi::dal::detail;
+        if (ar.template pop<std::uint8_t>() != 1 ||
+            ar.template pop<std::uint64_t>() != dd::serialization_ids::kmeans_clustering_model_impl_id)
+            throw invalid_argument(dd::archive_content_does_not_match_type());
+        dd::archived_table t = dd::read_table(ar);
         table c;
-        if (rows > 0 && cols > 0) {
-            switch (data_type(dtype)) {
-                case data_type::float32: c = read_table<float>(ar, rows, cols); break;
-                case data_type::float64: c = read_table<double>(ar, rows, cols); break;
-                case data_type::int32: c = read_table<std::int32_t>(ar, rows, cols); break;
-            }
-        }
+        if (t.dtype_code == dd::archived_type::float32) c = to_table<float>(t);
+        else if (t.dtype_code == dd::archived_type::float64) c = to_table<double>(t);
+        else if (t.dtype_code == dd::archived_type::int32) c = to_table<std::int32_t>(t);
         impl_ = std::make_shared<impl>();
         impl_->centroids = c;
     }
 
 private:
-    template <typename T, typename Archive>
-    static table read_table(Archive& ar, std::int64_t rows, std::int64_t cols) {
-        if (cols > 0 && rows > std::numeric_limits<std::int64_t>::max() / cols / std::int64_t(sizeof(T)))
-            throw invalid_argument("Archive holds a malformed centroid table");
-        array<T> a(rows * cols);
-        ar.read(a.get_mutable_data(), rows * cols * std::int64_t(sizeof(T)));
-        return homogen_table::wrap(a, rows, cols);
+    template <typename T>
+    static table to_table(const oneapi::dal::detail::archived_table& t) {
+        array<T> a(t.rows * t.cols);
+        std::memcpy(a.get_mutable_data(), t.bytes.data(), t.bytes.size());
+        return homogen_table::wrap(a, t.rows, t.cols);
     }
     struct impl {
         table centroids;
@@ -342,6 +338,7 @@ public:
         return *this;
     }
     [[deprecated]] const table& get_labels() const { return get_responses(); }
+    [[deprecated]] auto& set_labels(const table& v) { return set_responses(v); }
     const table& get_responses() const { return responses_; }
     auto& set_responses(const table& v) {
         responses_ = v;
@@ -394,6 +391,7 @@ class infer_result {
 public:
     using task_t = Task;
     [[deprecated]] const table& get_labels() const { return get_responses(); }
+    [[deprecated]] auto& set_labels(const table& v) { return set_responses(v); }
     const table& get_responses() const { return responses_; }
     auto& set_responses(const table& v) {
         responses_ = v;

@@ -1,12 +1,14 @@
 // Device checks of the oneapi::dal::kmeans drop-in surface: the reference's gold
 // dataset (test/fixture.hpp:156-203), its empty-cluster fixture
-// (fixture.hpp:205-219) and its smoke-data inference objective
-// (fixture.hpp:221-252), through dal::train / dal::infer on a HIP queue.
+// (fixture.hpp:205-219) and its smoke-data training with default initialisation
+// and inference objective (fixture.hpp:221-252), through dal::train / dal::infer
+// on a HIP queue.
 // argv[1]: text file "n d k" then data, initial centroids, expected centroids,
 // expected responses, expected objective (whitespace separated).
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <limits>
 #include <vector>
 
 #include "oneapi/dal/algo/kmeans.hpp"
@@ -93,6 +95,49 @@ static void smoke_infer(dal::hip::queue& q) {
     const auto r = dal::row_accessor<const int>(ir.get_responses()).pull();
     const int wr[] = {1, 1, 1, 1, 0, 0, 0, 0};
     for (int i = 0; i < 8; ++i) CHECK(r[i] == wr[i]);
+}
+
+// fixture.hpp:221-252 (run by test/batch.cpp:86-91): train without initial
+// centroids (default k-means++ initialisation), k=2, 4 iterations, threshold
+// 0.001, on the smoke data; centroids {(-1.5,-1.5),(1.5,1.5)} at the reference's
+// strict tolerance 5 eps iterations 100 and responses {1,1,1,1,0,0,0,0}, then
+// inference with the trained model gives objective 4.  The initialisation's draws
+// are SplitMix64, not DAAL's mt19937, so the cluster order may be swapped: the
+// centroids and the partition are matched up to that permutation.
+template <typename F>
+static void smoke_train_default_init(dal::hip::queue& q) {
+    const F data[] = {1.0, 1.0, 2.0, 2.0, 1.0, 2.0, 2.0, 1.0, -1.0, -1.0, -1.0, -2.0, -2.0, -1.0, -2.0, -2.0};
+    const auto x = dal::homogen_table::wrap(data, 8, 2);
+    const auto desc = km::descriptor<F>(2).set_max_iteration_count(4).set_accuracy_threshold(0.001);
+    const auto res = dal::train(q, desc, x);
+    const auto c = dal::row_accessor<const F>(res.get_model().get_centroids()).pull();
+    const auto r = dal::row_accessor<const int>(res.get_responses()).pull();
+    const int want_r[] = {1, 1, 1, 1, 0, 0, 0, 0};
+    const double want_c[] = {-1.5, -1.5, 1.5, 1.5};
+    const bool swapped = r[0] == 0;
+    const double tol = 5.0 * std::numeric_limits<F>::epsilon() * double(res.get_iteration_count()) * 100.0;
+    CHECK(res.get_iteration_count() >= 1 && res.get_iteration_count() <= 4);
+    for (int j = 0; j < 2; ++j)
+        for (int f = 0; f < 2; ++f) {
+            const double want = want_c[2 * (swapped ? 1 - j : j) + f];
+            CHECK(std::fabs(double(c[2 * j + f]) - want) <= tol * std::fabs(want));
+        }
+    for (int i = 0; i < 8; ++i) CHECK(r[i] == (swapped ? 1 - want_r[i] : want_r[i]));
+    const auto idesc = km::descriptor<F>(2).set_result_options(km::result_options::compute_exact_objective_function);
+    const auto ir = dal::infer(q, idesc, res.get_model(), x);
+    CHECK(std::fabs(ir.get_objective_function_value() - 4.0) < 1e-5 * 4.0);
+    const auto ri = dal::row_accessor<const int>(ir.get_responses()).pull();
+    for (int i = 0; i < 8; ++i) CHECK(ri[i] == r[i]);
+    // the deprecated label accessors (train_types.hpp:116-121, infer_types.hpp:94-99)
+#pragma GCC diagnostic push
+#pragma GCC diagnostic ignored "-Wdeprecated-declarations"
+    km::train_result<> tr = res;
+    tr.set_labels(ir.get_responses());
+    CHECK(tr.get_labels().get_row_count() == 8);
+    km::infer_result<> ir2 = ir;
+    ir2.set_labels(res.get_responses());
+    CHECK(dal::row_accessor<const int>(ir2.get_labels()).pull()[0] == r[0]);
+#pragma GCC diagnostic pop
 }
 
 // kmeans_init through dal::compute: the reference's test data and check
@@ -362,6 +407,8 @@ int main(int argc, char** argv) {
     empty_clusters<float>(q);
     empty_clusters<double>(q);
     smoke_infer(q);
+    smoke_train_default_init<float>(q);
+    smoke_train_default_init<double>(q);
     dal::hip::queue qb(0);
     qb.set_bf16_refine(true);
     gold<float>(argv[1], qb);

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <vector>
 
 #include "oneapi/dal/algo/kmeans.hpp"
 #include "oneapi/dal/algo/kmeans_init.hpp"
@@ -39,6 +40,113 @@ static void expect_throw(const std::function<void()>& fn, const char* msg, int l
     }
 }
 #define EXPECT_THROW(E, expr, msg) expect_throw<E>([&] { expr; }, msg, __LINE__)
+
+// Expected archive bytes, built field by field from the reference's serializers:
+// prologue magic (detail/archives.hpp:23,35-39), polymorphic member = bool flag as
+// one byte + uint64 id (detail/serialization.hpp:379-389), kmeans model impl id
+// 8010000000 and its centroid table (algo/kmeans/common.cpp:44-56,144-156),
+// homogen table = metadata, data, row count, column count, layout
+// (table/backend/homogen_table_impl.hpp:136-138), simple metadata = dtypes, ftypes
+// (table/common.cpp:133-141), array = array_id, element type, byte count, bytes
+// (detail/array_impl.cpp:26-37); ids from backend/serialization.hpp:155-194,
+// type codes from common.hpp:97-109, enums from table/common.hpp:35-36.
+struct bytes_builder {
+    std::vector<unsigned char> b;
+    template <typename T>
+    bytes_builder& put(T v) {
+        unsigned char raw[sizeof(T)];
+        std::memcpy(raw, &v, sizeof(T));
+        b.insert(b.end(), raw, raw + sizeof(T));
+        return *this;
+    }
+    bytes_builder& raw(const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        b.insert(b.end(), c, c + n);
+        return *this;
+    }
+};
+
+static std::vector<unsigned char> expected_model_bytes(int32_t dtype_code, int64_t rows, int64_t cols,
+                                                       const void* data, int64_t elem) {
+    bytes_builder e;
+    e.put<uint32_t>(0x4441414Fu);
+    e.put<uint8_t>(1).put<uint64_t>(8010000000ull);  // model impl
+    if (rows == 0) {
+        e.put<uint8_t>(1).put<uint64_t>(2020000000ull);  // empty table
+        return e.b;
+    }
+    e.put<uint8_t>(1).put<uint64_t>(2030000000ull);  // homogen table
+    e.put<uint8_t>(1).put<uint64_t>(2010000000ull);  // simple metadata
+    e.put<uint64_t>(1000000000ull).put<int32_t>(0x4).put<int64_t>(4 * cols);
+    for (int64_t j = 0; j < cols; ++j) e.put<int32_t>(dtype_code);
+    e.put<uint64_t>(1000000000ull).put<int32_t>(0x4).put<int64_t>(4 * cols);
+    const int32_t ftype = (dtype_code == 0x100 || dtype_code == 0x200) ? 3 : 1;  // ratio / ordinal
+    for (int64_t j = 0; j < cols; ++j) e.put<int32_t>(ftype);
+    e.put<uint64_t>(1000000000ull).put<int32_t>(0x10).put<int64_t>(rows * cols * elem);
+    e.raw(data, size_t(rows * cols * elem));
+    e.put<int64_t>(rows).put<int64_t>(cols).put<int32_t>(1);  // row_major
+    return e.b;
+}
+
+static void archive_format_checks() {
+    const double cd[] = {1.5, -2.0, 3.25, 0.0, 7.0, -1e-300};
+    const float cf[] = {0.5f, -1.25f, 8.0f, 3.0f};
+    {
+        km::model<> m;
+        m.set_centroids(dal::homogen_table::wrap(cd, 2, 3));
+        dal::detail::binary_output_archive out;
+        dal::detail::serialize(m, out);
+        CHECK(out.to_array() == expected_model_bytes(0x200, 2, 3, cd, 8));
+    }
+    {
+        km::model<> m;
+        m.set_centroids(dal::homogen_table::wrap(cf, 4, 1));
+        dal::detail::binary_output_archive out;
+        dal::detail::serialize(m, out);
+        CHECK(out.to_array() == expected_model_bytes(0x100, 4, 1, cf, 4));
+        // and a hand-built archive deserializes to those centroids
+        const auto e = expected_model_bytes(0x100, 4, 1, cf, 4);
+        km::model<> back;
+        dal::detail::binary_input_archive in(e.data(), int64_t(e.size()));
+        dal::detail::deserialize(back, in);
+        CHECK(back.get_cluster_count() == 4);
+        CHECK(back.get_centroids().get_data_type() == dal::data_type::float32);
+        const auto v = dal::row_accessor<const float>(back.get_centroids()).pull();
+        for (int i = 0; i < 4; ++i) CHECK(v[i] == cf[i]);
+    }
+    {
+        km::model<> empty;
+        dal::detail::binary_output_archive out;
+        dal::detail::serialize(empty, out);
+        CHECK(out.to_array() == expected_model_bytes(0x100, 0, 0, nullptr, 4));
+    }
+    // an archive that claims more centroid bytes than it holds is refused before
+    // any allocation of that size
+    {
+        bytes_builder e;
+        e.put<uint32_t>(0x4441414Fu).put<uint8_t>(1).put<uint64_t>(8010000000ull);
+        e.put<uint8_t>(1).put<uint64_t>(2030000000ull).put<uint8_t>(1).put<uint64_t>(2010000000ull);
+        e.put<uint64_t>(1000000000ull).put<int32_t>(0x4).put<int64_t>(4).put<int32_t>(0x100);
+        e.put<uint64_t>(1000000000ull).put<int32_t>(0x4).put<int64_t>(4).put<int32_t>(3);
+        e.put<uint64_t>(1000000000ull).put<int32_t>(0x10).put<int64_t>(int64_t(1) << 50);
+        e.put<int64_t>(int64_t(1) << 48).put<int64_t>(1).put<int32_t>(1);
+        km::model<> t;
+        dal::detail::binary_input_archive in(e.b.data(), int64_t(e.b.size()));
+        EXPECT_THROW(dal::invalid_argument, dal::detail::deserialize(t, in),
+                     "Archive content does not match requested type");
+    }
+    // a homogen table whose column types differ from its data size is refused
+    {
+        auto e = expected_model_bytes(0x200, 2, 3, cd, 8);
+        // first dtype entry (after magic 4, flag+id 9, flag+id 9, flag+id 9, array header 20)
+        const size_t at = 4 + 9 + 9 + 9 + 20;
+        const int32_t f32 = 0x100;
+        std::memcpy(&e[at], &f32, 4);
+        km::model<> t;
+        dal::detail::binary_input_archive in(e.data(), int64_t(e.size()));
+        EXPECT_THROW(dal::invalid_argument, dal::detail::deserialize(t, in), nullptr);
+    }
+}
 
 // model serialization round trip (the reference's kmeans/test/serialization.cpp
 // trains, serializes, deserializes and compares inference; the inference half
@@ -94,6 +202,7 @@ static void serialization_checks() {
 int main() {
     using namespace dal::detail::error_messages;
     serialization_checks();
+    archive_format_checks();
     // descriptor defaults and setters
     km::descriptor<float> d;
     CHECK(d.get_cluster_count() == 2);
