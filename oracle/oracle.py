@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libkmeans_oracle.so")
+FAST_PATH = os.path.join(HERE, "lib", "libkmeans_cpu_fast.so")
 
 # Order constants of the engine's FP64 reductions (must equal the ones in
 # onedal_amd/csrc/kmeans_config.hpp; the tests check that they do).
@@ -26,6 +27,7 @@ def reduce_groups(d):
 
 
 _lib = None
+_fast = None
 
 
 def build():
@@ -256,3 +258,45 @@ def csr_objective(closest):
     closest = np.ascontiguousarray(closest)
     return getattr(lib(), "oracle_csr_objective_" + _sfx(closest))(
         _ptr(closest), closest.shape[0], OBJ_CHUNK)
+
+
+def _fast_lib():
+    """The CPU baseline library (the reference's CPU algorithm with a vectorised
+    product kernel; oracle/kmeans_cpu_fast.cpp)."""
+    global _fast
+    if _fast is None:
+        if not os.path.exists(FAST_PATH):
+            build()
+        L = ctypes.CDLL(FAST_PATH)
+        P, I64 = ctypes.c_void_p, ctypes.c_int64
+        L.cpu_fast_isa.argtypes = []
+        L.cpu_fast_isa.restype = ctypes.c_int
+        L.cpu_fast_lloyd_f32.argtypes = [P, I64, I64, P, I64, I64, I64, ctypes.c_int, ctypes.c_int,
+                                         P, P, ctypes.POINTER(ctypes.c_double)]
+        L.cpu_fast_lloyd_f32.restype = ctypes.c_int
+        _fast = L
+    return _fast
+
+
+def cpu_fast_isa():
+    """Vector kernel the CPU baseline runs here: 'avx512', 'avx2' or 'portable'."""
+    return {2: "avx512", 1: "avx2", 0: "portable"}[_fast_lib().cpu_fast_isa()]
+
+
+def cpu_fast_lloyd(x, c0, iters, nthreads=None, block_rows=512, final=True):
+    """CPU baseline: `iters` Lloyd iterations of the reference's CPU algorithm
+    (float32) with a vectorised X.C^T kernel, then (final) the final assignment.
+    Returns (centroids, labels, exact objective; 0 without the final pass)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    c0 = np.ascontiguousarray(c0, dtype=np.float32)
+    n, d = x.shape
+    k = c0.shape[0]
+    c = np.empty_like(c0)
+    labels = np.empty(n, np.int32)
+    obj = ctypes.c_double()
+    st = _fast_lib().cpu_fast_lloyd_f32(_ptr(x), n, d, _ptr(c0), k, int(iters), int(block_rows),
+                                        _threads(nthreads), 1 if final else 0, _ptr(c), _ptr(labels),
+                                        ctypes.byref(obj))
+    if st != 0:
+        raise RuntimeError("cpu baseline: more empty clusters than candidates")
+    return c, labels, obj.value

@@ -7,7 +7,7 @@ iterations) and the structure checks."""
 import numpy as np
 import pytest
 
-from tests.test_oracle_csr import reorder, sparse_blobs
+from test_oracle_csr import reorder, sparse_blobs
 
 pytestmark = pytest.mark.gpu
 

@@ -15,6 +15,8 @@ What pins it:
     and zero-based/one-based storage do not change the result, and an empty
     cluster takes the farthest row.
 """
+import os
+
 import numpy as np
 
 
@@ -80,7 +82,8 @@ def test_reference_example_data_matches_dense(oracle):
     """The reference's CSR example data (the dense example rows stored as CSR,
     one-based): CSR and dense runs from the same start give the same labels and
     bit-identical centroids (same members, same chunk sums)."""
-    z = np.load("tests/golden/kmeans_dense_example.npz")
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "kmeans_dense_example.npz"))
     x, c0 = z["data"], z["initial_centroids"]
     for dtype in (np.float32, np.float64):
         xd = x.astype(dtype)
