@@ -243,6 +243,34 @@ def _check_csr(values, cols, offs):
     return False
 
 
+def _check_centroids(c, d, dtype, dev, device=None):
+    """Centroid arrays are read as k x d elements of the data's type: require a
+    2-D (k, d) array of that dtype (host) or a contiguous CUDA tensor of that
+    dtype on the data's device (dev), before any pointer reaches the engine."""
+    if dev != _is_torch_cuda(c):
+        raise TypeError("centroids must be a CUDA tensor exactly when the data is one")
+    if not dev:
+        c = np.asarray(c)
+    if len(c.shape) != 2 or int(c.shape[1]) != int(d):
+        raise ValueError("centroids must have shape (k, %d), got %s" % (int(d), tuple(c.shape)))
+    if dev:
+        if str(c.dtype).replace("torch.", "") != np.dtype(dtype).name:
+            raise TypeError("centroids must have the data's type %s, got %s" % (np.dtype(dtype).name, c.dtype))
+        if not c.is_contiguous():
+            raise ValueError("device centroids must be contiguous")
+        if device is not None and c.device != device:
+            raise ValueError("centroids must be on the data's device %s, got %s" % (device, c.device))
+
+
+def _check_device_rows(x):
+    """Device data: a contiguous 2-D float32 or float64 CUDA tensor."""
+    if len(x.shape) != 2:
+        raise ValueError("data must be two-dimensional, got shape %s" % (tuple(x.shape),))
+    if not x.is_contiguous():
+        raise ValueError("device data must be contiguous")
+    _sfx(str(x.dtype).replace("torch.", ""))
+
+
 class Engine:
     """One HIP device context (km_ctx) and its stream."""
 
@@ -300,6 +328,7 @@ class Engine:
         x = np.ascontiguousarray(x)
         sfx = _sfx(x.dtype)
         n, d = x.shape
+        _check_centroids(c0, d, x.dtype, False)
         c0 = np.ascontiguousarray(c0, dtype=x.dtype)
         k = c0.shape[0]
         c = np.empty((k, d), x.dtype)
@@ -320,6 +349,7 @@ class Engine:
         x = np.ascontiguousarray(x)
         sfx = _sfx(x.dtype)
         n, d = x.shape
+        _check_centroids(c, d, x.dtype, False)
         c = np.ascontiguousarray(c, dtype=x.dtype)
         labels = np.empty(n, np.int32)
         obj = ctypes.c_double()
@@ -335,6 +365,8 @@ class Engine:
         """CSR arrays in (host numpy, or CUDA tensors on this device: then the
         outputs are CUDA tensors too): (centroids, labels, objective, iterations)."""
         dev = _check_csr(values, cols, offs)
+        _check_centroids(c0, d, str(values.dtype).replace("torch.", ""), dev,
+                         values.device if dev else None)
         n = int(offs.shape[0]) - 1
         nnz = int(values.shape[0])
         k = int(c0.shape[0])
@@ -370,6 +402,8 @@ class Engine:
     def infer_csr(self, values, cols, offs, d, c, objective=False, base=0):
         """Labels (and the objective if asked) of CSR rows against centroids c."""
         dev = _check_csr(values, cols, offs)
+        _check_centroids(c, d, str(values.dtype).replace("torch.", ""), dev,
+                         values.device if dev else None)
         n = int(offs.shape[0]) - 1
         nnz = int(values.shape[0])
         k = int(c.shape[0])
@@ -404,9 +438,10 @@ class Engine:
         tensors (centroids, labels) plus objective and iteration count."""
         import torch
 
-        assert x.is_contiguous() and c0.is_contiguous()
+        _check_device_rows(x)
         sfx = _sfx(str(x.dtype).replace("torch.", ""))
         n, d = x.shape
+        _check_centroids(c0, d, str(x.dtype).replace("torch.", ""), True, x.device)
         k = c0.shape[0]
         if out is None:
             c = torch.empty((k, d), dtype=x.dtype, device=x.device)
@@ -433,8 +468,11 @@ class Engine:
         arr = (KmBatchInstance * count)()
         cs, ls = [], []
         for i, (x, c0) in enumerate(zip(xs, c0s)):
-            assert x.is_contiguous() and c0.is_contiguous() and x.dtype == torch.float32
+            _check_device_rows(x)
+            if x.dtype != torch.float32:
+                raise TypeError("batched training takes float32 data")
             n, d = x.shape
+            _check_centroids(c0, d, "float32", True, x.device)
             k = c0.shape[0]
             if outs is None:
                 c = torch.empty((k, d), dtype=torch.float32, device=x.device)
@@ -504,8 +542,10 @@ class Engine:
     def infer_device(self, x, c, objective=False, bf16=False):
         import torch
 
+        _check_device_rows(x)
         sfx = _sfx(str(x.dtype).replace("torch.", ""))
         n, d = x.shape
+        _check_centroids(c, d, str(x.dtype).replace("torch.", ""), True, x.device)
         labels = torch.empty(n, dtype=torch.int32, device=x.device)
         obj = ctypes.c_double()
         flags = KM_FLAG_DEVICE_POINTERS | (KM_FLAG_EXACT_OBJECTIVE if objective else 0) | (
