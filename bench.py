@@ -102,37 +102,57 @@ def make_data(torch, n, d, k, center_range, sigma, anisotropic, seed, device):
     return x, c0
 
 
-def cpu_baseline(x_host, c0_host, iters, budget_s=15.0):
-    """The oracle's restatement of the reference CPU Lloyd, timed on a bounded
-    sample of the workload (first rows, same initial centroids)."""
-    from oracle import oracle as O
+def cpu_model():
+    """Model name of the host CPU (first 'model name' line of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def host_cores():
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline(x_host, c0_host, iters, budget_s=15.0):
+    """The reference's CPU Lloyd algorithm with a vectorised X.C^T kernel
+    (oracle/kmeans_cpu_fast.cpp: DAAL's row blocks, per-thread partial sums merged
+    in double, candidate lists for empty clusters), timed on a bounded sample of
+    the workload (first rows, same initial centroids, one Lloyd iteration), on all
+    host cores (at most 16) and on one."""
+    from oracle import oracle as O
+
+    cores = host_cores()
     n = x_host.shape[0]
-    sample = min(n, 4096)
+    k = c0_host.shape[0]
+    sample = min(n, max(k, 65536))
     t0 = time.perf_counter()
-    O.daal_lloyd(x_host[:sample], c0_host, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+    O.cpu_fast_lloyd(x_host[:sample], c0_host, 1, nthreads=cores, final=False)
     dt = time.perf_counter() - t0
     # size the sample for about budget_s seconds of work (one iteration)
     sample = int(min(n, max(sample, sample * budget_s / max(dt, 1e-3))))
-    sample = max(sample, c0_host.shape[0])
     t0 = time.perf_counter()
-    O.daal_lloyd(x_host[:sample], c0_host, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+    O.cpu_fast_lloyd(x_host[:sample], c0_host, 1, nthreads=cores, final=False)
     dt = time.perf_counter() - t0
-    value = sample * 1 / dt
-    # the same restatement on one core, on a sample of about 4 s of work
-    sample1 = int(min(n, max(c0_host.shape[0], value / cores * 4.0)))
+    value = sample / dt
+    # the same kernel on one core, on a sample of about 4 s of work
+    sample1 = int(min(n, max(k, value / cores * 4.0)))
     t0 = time.perf_counter()
-    O.daal_lloyd(x_host[:sample1], c0_host, 1, 0.0, stop_rule=2, block_size=512, nthreads=1)
+    O.cpu_fast_lloyd(x_host[:sample1], c0_host, 1, nthreads=1, final=False)
     dt1 = time.perf_counter() - t0
-    return {"value": value, "unit": "point-iterations/s", "cores": cores,
-            "kind": "port",
-            "sample": "first %d rows of the workload, 1 Lloyd iteration, oracle DAAL-Lloyd "
-                      "restatement (float32, %d threads, %.1f s)" % (sample, cores, dt),
+    return {"value": value, "unit": "point-iterations/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "isa": O.cpu_fast_isa(),
+            "sample": "first %d rows of the workload, 1 Lloyd iteration, the reference's CPU "
+                      "Lloyd restated with a vectorised %s X.C^T kernel (oracle/kmeans_cpu_fast.cpp, "
+                      "g++ -O3; float32, %d threads, %.1f s)" % (sample, O.cpu_fast_isa(), cores, dt),
             "single_core": {"value": sample1 / dt1, "cores": 1,
                             "sample": "first %d rows, 1 iteration, %.1f s" % (sample1, dt1)}}
 
@@ -207,25 +227,23 @@ def run_batched(torch, args, wl):
         log("cpu baseline ...")
         from oracle import oracle as O
 
-        try:
-            cores = max(1, min(len(os.sched_getaffinity(0)), 16))
-        except AttributeError:
-            cores = 16
+        cores = host_cores()
         t0 = time.perf_counter()
         done = 0
         used = 0
         for x, c in zip(xs, cs):
             xh, ch = x.cpu().numpy(), c.cpu().numpy()
-            O.daal_lloyd(xh, ch, 1, 0.0, stop_rule=2, block_size=512, nthreads=cores)
+            O.cpu_fast_lloyd(xh, ch, 1, nthreads=cores, final=False)
             done += xh.shape[0]
             used += 1
             if time.perf_counter() - t0 > 15.0:
                 break
         dt = time.perf_counter() - t0
         cpu = {"value": done / dt, "unit": "point-iterations/s", "cores": cores, "kind": "port",
-               "sample": "first %d instances of the batch (%d rows), 1 Lloyd iteration each, oracle "
-                         "DAAL-Lloyd restatement (float32, %d threads, %.1f s incl. copies)" % (
-                             used, done, cores, dt)}
+               "cpu_model": cpu_model(), "isa": O.cpu_fast_isa(),
+               "sample": "first %d instances of the batch (%d rows), 1 Lloyd iteration each, the "
+                         "reference's CPU Lloyd with a vectorised X.C^T kernel (float32, %d threads, "
+                         "%.1f s incl. copies)" % (used, done, cores, dt)}
     out = {
         "metric": METRIC,
         "value": pit,
@@ -326,7 +344,8 @@ def run_init(torch, args, wl):
     print(json.dumps(out), flush=True)
 
 
-def run_serving(torch, args, wl):
+def run_serving(torch, args, wl, rates=None, min_jobs=None, window_s=None, cpu_baseline_on=True):
+    """The request-loop sweep; returns the result line (a dict)."""
     import queue
     import random
     import threading
@@ -424,9 +443,12 @@ def run_serving(torch, args, wl):
     log("warmup ...")
     run(200.0, 60, 12345)
     sweep = []
-    rates = [float(r) for r in (args.rates.split(",") if args.rates else wl["rates"])]
+    if rates is None:
+        rates = [float(r) for r in (args.rates.split(",") if args.rates else wl["rates"])]
+    mj = wl["min_jobs"] if min_jobs is None else min_jobs
+    win = wl["window_s"] if window_s is None else window_s
     for r in rates:
-        count = int(min(wl["max_jobs"], max(wl["min_jobs"], r * wl["window_s"])))
+        count = int(min(wl["max_jobs"], max(mj, r * win)))
         res = run(r, count, int(r))
         sweep.append(res)
         log("rate %6.0f/s: %d jobs in %.2f s, %.0f jobs/s, %.3g pit/s, p50 %.1f ms, p99 %.1f ms, "
@@ -436,25 +458,25 @@ def run_serving(torch, args, wl):
     srv.close()
     best = max(sweep, key=lambda z: z["point_iters_per_s"])
     cpu = None
-    if not args.no_cpu_baseline:
+    if cpu_baseline_on and not args.no_cpu_baseline:
         log("cpu baseline ...")
         from oracle import oracle as O
 
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = host_cores()
         t0 = time.perf_counter()
         done_rows = used = 0
         for x, c in zip(xs, cs):
-            O.daal_lloyd(x.cpu().numpy(), c.cpu().numpy(), 1, 0.0, stop_rule=2, block_size=512,
-                         nthreads=cores)
+            O.cpu_fast_lloyd(x.cpu().numpy(), c.cpu().numpy(), 1, nthreads=cores, final=False)
             done_rows += x.shape[0]
             used += 1
             if time.perf_counter() - t0 > 10.0:
                 break
         dt = time.perf_counter() - t0
         cpu = {"value": done_rows / dt, "unit": "point-iterations/s", "cores": cores, "kind": "port",
+               "cpu_model": cpu_model(), "isa": O.cpu_fast_isa(),
                "sample": "first %d small datasets of the pool (%d rows), 1 Lloyd iteration each, "
-                         "oracle DAAL-Lloyd restatement (float32, %d threads, %.1f s incl. copies)" % (
-                             used, done_rows, cores, dt)}
+                         "the reference's CPU Lloyd with a vectorised X.C^T kernel (float32, %d "
+                         "threads, %.1f s incl. copies)" % (used, done_rows, cores, dt)}
     out = {
         "metric": METRIC,
         "value": best["point_iters_per_s"],
@@ -486,7 +508,9 @@ def run_serving(torch, args, wl):
         "cpu_baseline": cpu,
         "sweep": sweep,
     }
-    print(json.dumps(out), flush=True)
+    del xs, cs, large
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_csr(torch, args, wl):
@@ -592,6 +616,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP32-path and FP64-path comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="precision_c3: skip the FP32-workload summary and the short serving "
+                         "sweep reported beside the headline line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched and serving workloads: keep the default tile per k instead of "
                          "autotuning the batched assignment's tiles first")
@@ -607,44 +634,96 @@ def main():
 
     import torch
 
-    from onedal_amd.engine import Engine, KM_OPT_TIMING
-
     if args.workload in BATCHED:
         return run_batched(torch, args, BATCHED[args.workload])
     if args.workload in INIT:
         return run_init(torch, args, INIT[args.workload])
     if args.workload in SERVING:
-        return run_serving(torch, args, SERVING[args.workload])
+        print(json.dumps(run_serving(torch, args, SERVING[args.workload])), flush=True)
+        return
     if args.workload in CSR:
         return run_csr(torch, args, CSR[args.workload])
-    wl = WORKLOADS[args.workload]
+    out = run_dense(torch, args, args.workload, args.steps, args.warmup,
+                    compare=not args.no_compare, cpu_on=not args.no_cpu_baseline)
+    if args.workload == "precision_c3" and not args.no_extras:
+        # the other workloads of the default run, reported beside the headline line:
+        # the plain FP32 workload (throughput, roofline, labels against the reference's
+        # CPU algorithm at the final centroids) and a short serving sweep
+        log("extra: fp32_c2 ...")
+        c2 = run_dense(torch, args, "fp32_c2", 2, 1, compare=False, cpu_on=False, label_check=True)
+        out["fp32_c2"] = {k_: c2[k_] for k_ in ("value", "unit", "ms_per_step", "steps", "warmup",
+                                                "config", "roofline", "labels_vs_cpu_reference")}
+        log("extra: serving sweep ...")
+        sv = run_serving(torch, args, SERVING["serving_c5"], rates=[50.0, 200.0, 400.0],
+                         min_jobs=150, window_s=1.5, cpu_baseline_on=False)
+        out["serving_sweep"] = {
+            "config": sv["config"],
+            "note": "short sweep (at least 150 jobs or 1.5 s of arrivals per rate); the full "
+                    "sweep is bench.py --workload serving_c5",
+            "rates": [{k_: z[k_] for k_ in ("offered_rate", "jobs", "jobs_per_s",
+                                            "point_iters_per_s", "p50_ms", "p99_ms",
+                                            "p50_small_ms", "p99_small_ms", "p50_large_ms",
+                                            "p99_large_ms", "mean_step_jobs")}
+                      for z in sv["sweep"]]}
+        out["extra_keys"] = ["fp32_c2", "serving_sweep"]
+    print(json.dumps(out), flush=True)
+
+
+def band_mask_fp64(x, c, labels_fp64=None):
+    """Rows whose FP64 top-2 squared-distance gap is below eps (|x|^2 + |c_1|^2),
+    eps = 4 d 2^-24: inside it float rounding may order the two nearest centroids
+    either way.  x, c: float32 CUDA tensors; returns a bool CUDA tensor."""
+    import torch
+
+    n, d = x.shape
+    eps = 4.0 * d * 2.0 ** -24
+    cd = c.double()
+    cn = (cd * cd).sum(1)
+    out = torch.empty(n, dtype=torch.bool, device=x.device)
+    step = 1 << 18
+    for r0 in range(0, n, step):
+        xd = x[r0:r0 + step].double()
+        xn = (xd * xd).sum(1)
+        dist = (xn[:, None] - 2.0 * (xd @ cd.T) + cn[None, :]).clamp_min(0.0)
+        top2, idx = torch.topk(dist, 2, dim=1, largest=False)
+        gap = top2[:, 1] - top2[:, 0]
+        out[r0:r0 + step] = gap < eps * (xn + cn[idx[:, 0]])
+    return out
+
+
+def run_dense(torch, args, name, steps, warmup, compare=True, cpu_on=True, label_check=False):
+    """One dense workload (WORKLOADS[name]): timed training calls; returns the
+    result line (a dict)."""
+    from onedal_amd.engine import Engine, KM_OPT_TIMING
+
+    wl = WORKLOADS[name]
     n, d, k, iters = wl["n"], wl["d"], wl["k"], wl["iters"]
     dev = torch.device("cuda:0")
-    log("generating %s: n=%d d=%d k=%d" % (args.workload, n, d, k))
+    log("generating %s: n=%d d=%d k=%d" % (name, n, d, k))
     x, c0 = make_data(torch, n, d, k, wl["center_range"], wl["sigma"], wl["anisotropic"], 0, dev)
     torch.cuda.synchronize()
     eng = Engine(0)
     eng.set_option(KM_OPT_TIMING, 1)
     c_out = torch.empty((k, d), dtype=torch.float32, device=dev)
     l_out = torch.empty(n, dtype=torch.int32, device=dev)
-    for w in range(args.warmup):
+    for w in range(warmup):
         eng.train_device(x, c0, iters, 0.0, bf16=wl["bf16"], out=(c_out, l_out))
         log("warmup %d done" % w)
     torch.cuda.synchronize()
     times = []
     stats = []
     t_all = time.perf_counter()
-    for s in range(args.steps):
+    for s_ in range(steps):
         t0 = time.perf_counter()
         _, _, obj, it = eng.train_device(x, c0, iters, 0.0, bf16=wl["bf16"], out=(c_out, l_out))
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         stats.append(eng.stats())
         log("step %d: %.3f s, iterations %d, objective %.9e, re-scored rows cand/collect/full %d/%d/%d" % (
-            s, times[-1], it, obj, stats[-1]["refined_candidate_rows"],
+            s_, times[-1], it, obj, stats[-1]["refined_candidate_rows"],
             stats[-1]["refined_collect_rows"], stats[-1]["refined_full_rows"]))
     elapsed = time.perf_counter() - t_all
-    pit = n * iters * args.steps / elapsed
+    pit = n * iters * steps / elapsed
 
     # roofline of the dominant kernel (assignment): algorithmic 2*k*d flops per row
     launches = sum(s["assign_launches"] for s in stats)
@@ -653,7 +732,7 @@ def main():
     flops_per_launch = 2.0 * k * d * n
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     peak = PEAK["bf16_mfma_tflops"] if wl["bf16"] else PEAK["fp32_mfma_tflops"]
-    mt = MEASURED_TRAFFIC.get(args.workload)
+    mt = MEASURED_TRAFFIC.get(name)
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": mt["bytes"] if mt else None,
             "kernel": "assign (fused distance GEMM + argmin)", "avg_launch_ms": round(avg_ms, 4)}
@@ -671,8 +750,8 @@ def main():
                                                   "32x32x16 BF16 MFMA chains per wave, 8 waves "
                                                   "per CU, no memory traffic"}
 
-    compare = None
-    if wl["bf16"] and not args.no_compare:
+    comparison = None
+    if wl["bf16"] and compare:
         # same data and init on the plain FP32 path: throughput ratio and label identity
         log("fp32 path on the same workload ...")
         c32 = torch.empty_like(c_out)
@@ -683,13 +762,15 @@ def main():
         t32 = time.perf_counter() - t0
         st32 = eng.stats()
         same = bool(torch.equal(l32, l_out)) and bool(torch.equal(c32, c_out))
-        compare = {"fp32_path_value": n * iters / t32, "fp32_path_ms_per_step": 1e3 * t32,
-                   "speedup_vs_fp32": (n * iters / t32 and pit / (n * iters / t32)),
-                   "labels_and_centroids_identical_to_fp32": same,
-                   "fp32_assign_avg_launch_ms": st32["assign_kernel_ms"] / max(1, st32["assign_launches"]),
-                   "objective_bf16": obj, "objective_fp32": obj32}
+        comparison = {"fp32_path_value": n * iters / t32, "fp32_path_ms_per_step": 1e3 * t32,
+                      "speedup_vs_fp32": (n * iters / t32 and pit / (n * iters / t32)),
+                      "labels_and_centroids_identical_to_fp32": same,
+                      "fp32_assign_avg_launch_ms": st32["assign_kernel_ms"] / max(1, st32["assign_launches"]),
+                      "objective_bf16": obj, "objective_fp32": obj32}
         del c32, l32
-        # FP64 Lloyd (double data, same initial rows) for the inertia error of the precision form
+        # FP64 Lloyd on the same rows and initial centroids (double data): the
+        # engine's FP64 path computes the oracle's FP64 restatement bit for bit
+        # (tests/test_gpu_parity.py), so it stands for the FP64 oracle at full size
         log("fp64 path on the same workload ...")
         xd = x.double()
         cd = c0.double()
@@ -698,18 +779,53 @@ def main():
         t0 = time.perf_counter()
         _, _, obj64, _ = eng.train_device(xd, cd, iters, 0.0, out=(cdo, ldo))
         torch.cuda.synchronize()
-        compare["fp64_path_ms_per_step"] = 1e3 * (time.perf_counter() - t0)
-        compare["objective_fp64"] = obj64
-        compare["inertia_rel_err_vs_fp64"] = abs(obj - obj64) / obj64
-        compare["label_agreement_vs_fp64"] = float((ldo == l_out).float().mean().item())
-        del xd, cd, cdo, ldo
+        comparison["fp64_path_ms_per_step"] = 1e3 * (time.perf_counter() - t0)
+        comparison["objective_fp64"] = obj64
+        comparison["inertia_rel_err_vs_fp64"] = abs(obj - obj64) / obj64
+        comparison["label_agreement_vs_fp64"] = float((ldo == l_out).float().mean().item())
+        comparison["label_mismatches_vs_fp64"] = int((ldo != l_out).sum().item())
+        # centroid error: per centroid, |c - c64| / |c64| (L2), and the largest
+        # element difference relative to the largest centroid element
+        dc = c_out.double() - cdo
+        rel = dc.norm(dim=1) / cdo.norm(dim=1).clamp_min(1e-300)
+        comparison["centroid_max_rel_err_vs_fp64"] = float(rel.max().item())
+        comparison["centroid_max_abs_err_over_max_abs_vs_fp64"] = float(
+            (dc.abs().max() / cdo.abs().max()).item())
+        comparison["fp64_reference"] = ("the engine's FP64 path on float64 copies of the same rows "
+                                        "and initial centroids (bit-identical to the oracle's FP64 "
+                                        "restatement)")
+        del xd, cd, cdo, ldo, dc
         torch.cuda.empty_cache()
 
+    labels_check = None
+    if label_check:
+        # labels against the reference's CPU algorithm at the final centroids, on the
+        # first rows: the oracle's DAAL restatement (float32, 16 threads) assigns the
+        # same rows to the same centroids; differences must lie inside the float band
+        from oracle import oracle as O
+
+        m = min(n, 1_000_000)
+        xs = x[:m].cpu().numpy()
+        t0 = time.perf_counter()
+        _, dl, _, _ = O.daal_lloyd(xs, c_out.cpu().numpy(), 0, 0.0, stop_rule=2, block_size=512,
+                                   nthreads=host_cores())
+        t_cpu = time.perf_counter() - t0
+        dl = torch.from_numpy(dl).to(dev)
+        band = band_mask_fp64(x[:m], c_out)
+        mism = dl != l_out[:m]
+        labels_check = {"rows": m, "mismatches": int(mism.sum().item()),
+                        "mismatches_outside_band": int((mism & ~band).sum().item()),
+                        "rows_inside_band": int(band.sum().item()),
+                        "band": "FP64 top-2 gap < 4 d 2^-24 (|x|^2 + |c_1|^2)",
+                        "reference": "oracle DAAL-Lloyd restatement (float32), assignment at the "
+                                     "GPU's final centroids, %.1f s on %d threads" % (t_cpu, host_cores())}
+        del xs, dl, band, mism
+
     cpu = None
-    if not args.no_cpu_baseline:
+    if cpu_on:
         log("cpu baseline ...")
         xs = x[: min(n, 4_000_000)].cpu().numpy()
-        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters, budget_s=35.0)
+        cpu = cpu_baseline(xs, c0.cpu().numpy(), iters, budget_s=15.0)
         del xs
 
     if args.copy_calibration:
@@ -726,15 +842,15 @@ def main():
         "value": pit,
         "unit": "point-iterations/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16+f32" if wl["bf16"] else "f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "n": n, "d": d, "k": k, "iterations": iters,
+        "config": {"workload": name, "n": n, "d": d, "k": k, "iterations": iters,
                    "path": "bf16+refine" if wl["bf16"] else "fp32",
                    "init": "random distinct rows (seed 0)"},
         "roofline": roof,
@@ -743,9 +859,13 @@ def main():
         "refined_rows_per_step": {k_: stats[-1][k_] for k_ in ("refined_candidate_rows",
                                                                  "refined_collect_rows",
                                                                  "refined_full_rows")},
-        "comparison": compare,
+        "comparison": comparison,
+        "labels_vs_cpu_reference": labels_check,
     }
-    print(json.dumps(out), flush=True)
+    eng.close()
+    del x, c0, c_out, l_out
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":
