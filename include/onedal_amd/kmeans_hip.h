@@ -142,6 +142,20 @@ km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, 
 km_status km_lloyd_infer_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, const double* C,
                              int64_t k, uint32_t flags, int32_t* labels_out, double* obj_out);
 
+/* km_lloyd_infer_f32 that allocates no memory.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * X, C and labels_out must be device pointers (KM_FLAG_DEVICE_POINTERS is
+ * required); the results equal km_lloyd_infer_f32's bit for bit.  Workspace is a
+ * fixed 64 MiB array per device, kept until the process ends; rows run through
+ * it in blocks, and a k x d whose centroid-side workspace leaves no room for a
+ * block of 4096 rows is refused with KM_INVALID_ARGUMENT.  All calls on one
+ * device share that array: calls that overlap on one device (several contexts or
+ * threads) race and must be serialised by the caller. */
+km_status km_lloyd_infer_f32_noalloc(km_ctx* ctx, const float* X, int64_t n, int64_t d,
+                                     const float* C, int64_t k, uint32_t flags, int32_t* labels_out,
+                                     double* obj_out);
+
 /* CSR data (method lloyd_csr).  values[nnz], col_indices[nnz], row_offsets[n + 1]
  * as csr_table holds them (table/csr.hpp), zero- or one-based (sparse_indexing);
  * column indices need not be sorted.  Offsets must run from the base to

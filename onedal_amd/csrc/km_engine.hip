@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <stdexcept>
@@ -56,6 +57,10 @@ namespace km {
     } while (0)
 
 #define KM_LAUNCH_CHECK() KM_HIP(hipGetLastError())
+
+// size of the fixed scratch array of km_lloyd_infer_f32_noalloc
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
 
 // grow-only device buffer
 struct dbuf {
@@ -99,6 +104,11 @@ struct batch_ws {
 
 }  // namespace km
 
+// Workspace of km_lloyd_infer_f32_noalloc: one copy per device, held until the
+// process ends.  In host code this name is a stand-in, not a device address;
+// the address comes from hipGetSymbolAddress (km::scratch_address).
+static __device__ __align__(256) unsigned char scratch[km::NOALLOC_BYTES];
+
 struct km_ctx {
     int device = 0;
     int cu_count = 256;
@@ -129,6 +139,9 @@ struct km_ctx {
     // transposed centroids, centroid squares, closest distances, structure flag
     km::dbuf csr_vals, csr_cols, csr_offs, csr_xs, csr_ct, csr_cs, csr_closest, csr_flag;
     int32_t* pinned = nullptr;  // small pinned host staging: meta[2], counters[2], obj (as 2 ints)
+    // this device's copy of the scratch array (null if its address is unknown)
+    void* scratch = nullptr;
+    std::string scratch_err;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
 
@@ -157,6 +170,29 @@ static void fill(km_ctx* ctx, T* p, int64_t count, T value) {
     KM_LAUNCH_CHECK();
 }
 
+// Fixed device memory handed out front to back (256-byte aligned pieces); never
+// allocates.  Running past the end is an error, not a fault.
+struct arena {
+    unsigned char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    template <typename T>
+    T* get(size_t count) {
+        const size_t off = (used + 255) & ~size_t(255);
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 1);
+        if (off > cap || bytes > cap - off)
+            throw status_error(KM_INTERNAL_ERROR, "workspace plan exceeds the fixed scratch buffer");
+        used = off + bytes;
+        return reinterpret_cast<T*>(base + off);
+    }
+};
+
+// workspace piece: from the arena when one is given, else the context's buffer
+template <typename T>
+static T* take(km_ctx* ctx, dbuf km_ctx::*buf, arena* A, size_t count) {
+    return A ? A->get<T>(count) : (ctx->*buf).get<T>(count);
+}
+
 template <typename F>
 struct work {
     // sizes
@@ -179,6 +215,8 @@ struct work {
     uint16_t* Cb = nullptr;       // k x dpad bf16(-2 c)
     float* cnb = nullptr;         // ||c||^2 + B (rounded up), 32 * ceil(k / 32) entries
     bf16_params* params = nullptr;
+    // workspace source: a fixed arena, or (null) the context's buffers
+    arena* A = nullptr;
 };
 
 static int pad_dim(int d) {
@@ -290,12 +328,12 @@ static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
         seg_lists S;
         S.cap = (tiles + grid - 1) / grid * (32 * NB);
         const int64_t ents = S.cap * nseg;
-        S.cand_rows = ctx->seg_cand_rows.get<int32_t>(ents);
-        S.cand_count = ctx->seg_cand_cnt.get<int32_t>(ents);
-        S.cand_ids = ctx->seg_cand_ids.get<int32_t>(ents * MAXC);
-        S.coll_rows = ctx->seg_coll_rows.get<int32_t>(ents);
-        S.coll_T = ctx->seg_coll_T.get<double>(ents);
-        S.counts = ctx->seg_counts.get<int32_t>(2 * int64_t(nseg));
+        S.cand_rows = take<int32_t>(ctx, &km_ctx::seg_cand_rows, w.A, ents);
+        S.cand_count = take<int32_t>(ctx, &km_ctx::seg_cand_cnt, w.A, ents);
+        S.cand_ids = take<int32_t>(ctx, &km_ctx::seg_cand_ids, w.A, ents * MAXC);
+        S.coll_rows = take<int32_t>(ctx, &km_ctx::seg_coll_rows, w.A, ents);
+        S.coll_T = take<double>(ctx, &km_ctx::seg_coll_T, w.A, ents);
+        S.counts = take<int32_t>(ctx, &km_ctx::seg_counts, w.A, 2 * int64_t(nseg));
         auto kern = assign_bf16_kernel<DPB, NB, false>;
         KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    int(lds)));
@@ -577,29 +615,30 @@ static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m
 
 template <typename F>
 static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, int k,
-                       int path) {
+                       int path, arena* A = nullptr, int32_t* labels_into = nullptr) {
     const bool vec_view = path == 2;
     w.n = n;
     w.d = d;
     w.k = k;
     w.dpad = pad_dim(d);
     w.X = Xd;
-    w.xnorm = ctx->xnorm.get<float>(n);
-    w.labels = ctx->labels.get<int32_t>(n);
-    w.L.cand_rows = ctx->lists_rows.get<int32_t>(n);
-    w.L.cand_count = ctx->lists_cnt.get<int32_t>(n);
-    w.L.cand_ids = ctx->lists_ids.get<int32_t>(n * int64_t(MAXC));
-    w.L.full_rows = ctx->lists_full.get<int32_t>(n);
-    w.L.counters = ctx->lists_counters.get<int32_t>(3);
-    w.cn = ctx->cn.get<F>(k);
-    w.cpad = ctx->cpad.get<F>(int64_t(k) * w.dpad);
-    w.cmax_bits = ctx->cmax.get<unsigned long long>(4);
+    w.A = A;
+    w.xnorm = take<float>(ctx, &km_ctx::xnorm, A, n);
+    w.labels = labels_into ? labels_into : take<int32_t>(ctx, &km_ctx::labels, A, n);
+    w.L.cand_rows = take<int32_t>(ctx, &km_ctx::lists_rows, A, n);
+    w.L.cand_count = take<int32_t>(ctx, &km_ctx::lists_cnt, A, n);
+    w.L.cand_ids = take<int32_t>(ctx, &km_ctx::lists_ids, A, n * int64_t(MAXC));
+    w.L.full_rows = take<int32_t>(ctx, &km_ctx::lists_full, A, n);
+    w.L.counters = take<int32_t>(ctx, &km_ctx::lists_counters, A, 3);
+    w.cn = take<F>(ctx, &km_ctx::cn, A, k);
+    w.cpad = take<F>(ctx, &km_ctx::cpad, A, int64_t(k) * w.dpad);
+    w.cmax_bits = take<unsigned long long>(ctx, &km_ctx::cmax, A, 4);
     // float4-readable view of X for the matrix-core kernels
     w.Xv = Xd;
     w.ldx = d;
     if (vec_view && ((d % 4) != 0 || (reinterpret_cast<uintptr_t>(Xd) % 16) != 0)) {
         const int64_t ldx = (d + 3) / 4 * 4;
-        F* xp = ctx->x_pad.get<F>(n * ldx);
+        F* xp = take<F>(ctx, &km_ctx::x_pad, A, n * ldx);
         const int g = grid_for(ctx, n * ldx, 256, 8);
         hipLaunchKernelGGL(pad_rows_kernel<F>, dim3(g), dim3(256), 0, ctx->stream, Xd, n, d, ldx, xp);
         KM_LAUNCH_CHECK();
@@ -611,16 +650,16 @@ static void setup_work(km_ctx* ctx, work<F>& w, const F* Xd, int64_t n, int d, i
     const int grid = grid_for(ctx, n * 64, 256, 8);
     if constexpr (std::is_same<F, float>::value) {
         if (path == 3) {
-            w.Xb = ctx->x_bf16.get<uint16_t>(n * int64_t(w.dpad));
-            w.enorm = ctx->enorm.get<float>(n);
-            w.xmax_bits = ctx->xmax.get<unsigned long long>(2);
-            w.Cb = ctx->c_bf16.get<uint16_t>(int64_t(bf16_padded_k(k)) * w.dpad);
-            w.cnb = ctx->cnb.get<float>(bf16_padded_k(k));
-            w.params = ctx->params.get<bf16_params>(1);
-            w.L.coll_rows = ctx->coll_rows.get<int32_t>(n);
-            w.L.coll_T = ctx->coll_T.get<double>(n);
-            w.L.coll_count = ctx->coll_count.get<int32_t>(n);
-            w.L.coll_ids = ctx->coll_ids.get<int32_t>(n * int64_t(MAXC2));
+            w.Xb = take<uint16_t>(ctx, &km_ctx::x_bf16, A, n * int64_t(w.dpad));
+            w.enorm = take<float>(ctx, &km_ctx::enorm, A, n);
+            w.xmax_bits = take<unsigned long long>(ctx, &km_ctx::xmax, A, 2);
+            w.Cb = take<uint16_t>(ctx, &km_ctx::c_bf16, A, int64_t(bf16_padded_k(k)) * w.dpad);
+            w.cnb = take<float>(ctx, &km_ctx::cnb, A, bf16_padded_k(k));
+            w.params = take<bf16_params>(ctx, &km_ctx::params, A, 1);
+            w.L.coll_rows = take<int32_t>(ctx, &km_ctx::coll_rows, A, n);
+            w.L.coll_T = take<double>(ctx, &km_ctx::coll_T, A, n);
+            w.L.coll_count = take<int32_t>(ctx, &km_ctx::coll_count, A, n);
+            w.L.coll_ids = take<int32_t>(ctx, &km_ctx::coll_ids, A, n * int64_t(MAXC2));
             fill<unsigned long long>(ctx, w.xmax_bits, 2, 0ull);
             hipLaunchKernelGGL(convert_rows_bf16_kernel, dim3(grid), dim3(256), 0, ctx->stream, Xd, n,
                                d, w.dpad, w.Xb, w.xnorm, w.enorm, w.xmax_bits);
@@ -796,6 +835,109 @@ static void infer(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C, 
             tot += a;
         }
         ctx->stats.assign_kernel_ms = tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inference without allocation (km_lloyd_infer_f32_noalloc).  Every workspace
+// piece comes from the device's fixed scratch array; rows go through in blocks
+// that fit it, each block a whole number of objective chunks (OBJ_CHUNK rows),
+// so the chunk sums and their in-order addition are those of infer().  The
+// labels are the exact-form argmin on every path, so they do not depend on the
+// blocking either.  Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_SLACK = 16384;  // alignment of the arena's pieces
+
+// arena bytes of one block of R rows (mirrors setup_work, launch_bf16 and the
+// objective chunks); the arena itself refuses anything beyond its end
+template <typename F>
+static size_t noalloc_block_bytes(const km_ctx* ctx, int path, int64_t R, int d, int k, bool pad_x,
+                                  bool want_obj) {
+    const int64_t dpad = pad_dim(d);
+    int64_t b = R * (4 + 4 + 4 + 4 * int64_t(MAXC) + 4) + 16 + int64_t(sizeof(F)) * k +
+                int64_t(sizeof(F)) * k * dpad + 32;
+    if (path == 2 && pad_x) b += R * ((d + 3) / 4 * 4) * int64_t(sizeof(F));
+    if (path == 3) {
+        const int64_t kp = bf16_padded_k(k);
+        b += R * (2 * dpad + 4 + 4 + 8 + 4 + 4 * int64_t(MAXC2)) + 16 + 2 * kp * dpad + 4 * kp + 64;
+        const int64_t rows_per_wg = dpad >= 256 ? 128 : 256;  // launch_bf16: 4 waves x 32 x NB
+        const int64_t tiles = (R + rows_per_wg - 1) / rows_per_wg;
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2));
+        const int64_t ents = (tiles + grid - 1) / grid * rows_per_wg * grid;
+        b += ents * (4 + 4 + 4 * int64_t(MAXC) + 4 + 8) + 2 * 4 * grid * 4;  // + segment counts
+    }
+    if (want_obj) b += 8 * ((R + OBJ_CHUNK - 1) / OBJ_CHUNK);
+    return size_t(b + NOALLOC_SLACK);
+}
+
+static void infer_noalloc(km_ctx* ctx, const float* X, int64_t n, int64_t d64_, const float* C,
+                          int64_t k64, uint32_t flags, int32_t* labels_out, double* obj_out) {
+    using F = float;
+    const int d = int(d64_), k = int(k64);
+    if (!(flags & KM_FLAG_DEVICE_POINTERS))
+        throw status_error(KM_INVALID_ARGUMENT,
+                           "km_lloyd_infer_f32_noalloc takes device pointers only (KM_FLAG_DEVICE_POINTERS)");
+    if (!ctx->scratch)
+        throw status_error(KM_INTERNAL_ERROR,
+                           "the fixed scratch buffer is not available on this device: " + ctx->scratch_err);
+    ctx->stats = km_stats{};
+    const int path = choose_path<F>(ctx, d, k, flags);
+    const bool want_obj = flags & KM_FLAG_EXACT_OBJECTIVE;
+    const bool pad_x = (d % 4) != 0 || (reinterpret_cast<uintptr_t>(X) % 16) != 0;
+    const size_t cap = NOALLOC_BYTES - 256;  // the objective accumulator sits in front
+    // largest block: a multiple of OBJ_CHUNK rows (or all rows) that fits
+    int64_t R = std::min<int64_t>(n, int64_t(1) << 22);
+    R = R < n ? R / OBJ_CHUNK * OBJ_CHUNK : R;
+    while (R > OBJ_CHUNK && noalloc_block_bytes<F>(ctx, path, R, d, k, pad_x, want_obj) > cap)
+        R = (R - 1) / OBJ_CHUNK * OBJ_CHUNK;
+    if (noalloc_block_bytes<F>(ctx, path, R, d, k, pad_x, want_obj) > cap)
+        throw status_error(KM_INVALID_ARGUMENT,
+                           "k x d too large for km_lloyd_infer_f32_noalloc: one block of " +
+                               std::to_string(R) + " rows needs " +
+                               std::to_string(noalloc_block_bytes<F>(ctx, path, R, d, k, pad_x, want_obj)) +
+                               " bytes of the " + std::to_string(NOALLOC_BYTES) +
+                               "-byte scratch buffer");
+    // no timing events here: creating one may allocate
+    struct timing_off {
+        km_ctx* c;
+        bool was;
+        ~timing_off() { c->timing = was; }
+    } restore{ctx, ctx->timing};
+    ctx->timing = false;
+    ctx->stats.assign_path = path;
+    unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+    double* acc = reinterpret_cast<double*>(base);
+    if (want_obj) fill<double>(ctx, acc, 1, 0.0);
+    int32_t* pin = ctx->pinned;
+    for (int64_t r0 = 0; r0 < n; r0 += R) {
+        const int64_t rows = std::min<int64_t>(R, n - r0);
+        arena A{base + 256, cap, 0};
+        work<F> w;
+        setup_work<F>(ctx, w, X + r0 * d, rows, d, k, path, &A, labels_out + r0);
+        centroid_prep<F>(ctx, w, C, path);
+        assign<F>(ctx, w, C, path);
+        if (want_obj) {
+            const int64_t nchunks = (rows + OBJ_CHUNK - 1) / OBJ_CHUNK;
+            double* chunks = A.get<double>(nchunks);
+            const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
+            hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, w.X,
+                               w.n, w.d, C, w.labels, (double*)nullptr, chunks);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(sum_sequential_into_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks,
+                               nchunks, acc);
+            KM_LAUNCH_CHECK();
+        }
+        // the next block reuses the arena: its pieces are stream-ordered after this
+        // block's kernels, and the counters are read before they are cleared again
+        KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        collect_refine_stats(ctx, pin + 2);
+    }
+    if (want_obj) {
+        KM_HIP(hipMemcpyAsync(pin + 6, acc, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        if (obj_out) std::memcpy(obj_out, pin + 6, sizeof(double));
     }
 }
 
@@ -1687,6 +1829,33 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
     KM_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+// Device address of the scratch array on `device`, which must be the current
+// device.  Looked up once per device (each device holds its own copy at its own
+// address) and kept for the life of the process.
+static std::mutex scratch_mu;
+static void* scratch_dev[NOALLOC_MAX_DEVICES];
+static hipError_t scratch_rc[NOALLOC_MAX_DEVICES];
+static bool scratch_done[NOALLOC_MAX_DEVICES];
+
+static void* scratch_address(int device, std::string* err) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        *err = "device index " + std::to_string(device) + " is beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_done[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) (void)hipGetLastError();
+        scratch_rc[device] = e;
+        scratch_dev[device] = e == hipSuccess ? p : nullptr;
+        scratch_done[device] = true;
+    }
+    if (!scratch_dev[device])
+        *err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(scratch_rc[device]);
+    return scratch_dev[device];
+}
+
 template <typename Fn>
 static km_status guarded(km_ctx* ctx, Fn fn) {
     if (!ctx) return KM_INVALID_ARGUMENT;
@@ -1741,6 +1910,8 @@ km_status km_context_create(int device, km_ctx** out) {
         return KM_HOST_BAD_ALLOC;
     }
     ctx->pinned = static_cast<int32_t*>(pin);
+    // the device is current: resolve its copy of the no-allocation scratch array
+    ctx->scratch = km::scratch_address(device, &ctx->scratch_err);
     *out = ctx;
     return KM_OK;
 }
@@ -1864,6 +2035,15 @@ km_status km_lloyd_infer_f32(km_ctx* ctx, const float* X, int64_t n, int64_t d, 
     return km::guarded(ctx, [&] {
         km::check_infer_args(X, n, d, C, k, flags, labels_out, obj_out);
         km::infer<float>(ctx, X, n, d, C, k, flags, labels_out, obj_out);
+    });
+}
+
+km_status km_lloyd_infer_f32_noalloc(km_ctx* ctx, const float* X, int64_t n, int64_t d,
+                                     const float* C, int64_t k, uint32_t flags, int32_t* labels_out,
+                                     double* obj_out) {
+    return km::guarded(ctx, [&] {
+        km::check_infer_args(X, n, d, C, k, flags, labels_out, obj_out);
+        km::infer_noalloc(ctx, X, n, d, C, k, flags, labels_out, obj_out);
     });
 }
 

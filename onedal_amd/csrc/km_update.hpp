@@ -472,6 +472,18 @@ __global__ void sum_sequential_kernel(const double* __restrict__ v, int64_t coun
     }
 }
 
+// *acc += v[0] + ... + v[count-1], added one at a time in order: a sum over row
+// blocks processed one after another equals sum_sequential_kernel over all
+// chunks at once (acc starts at 0, and 0 + v0 = v0).
+__global__ void sum_sequential_into_kernel(const double* __restrict__ v, int64_t count,
+                                           double* __restrict__ acc) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = *acc;
+        for (int64_t i = 0; i < count; ++i) s = s + v[i];
+        *acc = s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // empty clusters: radix select of the m largest d64 (keys are the raw bits of
 // non-negative doubles, which order like the values).

@@ -39,7 +39,7 @@ PATH_AUTO, PATH_GENERIC, PATH_MFMA_F32 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
-    "km_lloyd_infer_f64", "km_device_alloc", "km_device_free", "km_memcpy",
+    "km_lloyd_infer_f64", "km_lloyd_infer_f32_noalloc", "km_device_alloc", "km_device_free", "km_memcpy",
     "km_init_centroids_f32", "km_init_centroids_f64",
     "km_lloyd_train_csr_f32", "km_lloyd_train_csr_f64", "km_lloyd_infer_csr_f32",
     "km_lloyd_infer_csr_f64", "km_measure_bf16_mfma_peak",
@@ -164,6 +164,8 @@ def load_library():
         f = getattr(L, "km_lloyd_infer_csr_" + sfx)
         f.argtypes = [P, P, P, P, I64, I64, I64, ctypes.c_int32, P, I64, U32, P, P]
         f.restype = I
+    L.km_lloyd_infer_f32_noalloc.argtypes = [P, P, I64, I64, P, I64, U32, P, P]
+    L.km_lloyd_infer_f32_noalloc.restype = I
     L.km_batch_train_f32.argtypes = [P, ctypes.POINTER(KmBatchInstance), I64, I64, U32,
                                      ctypes.POINTER(D), ctypes.POINTER(I64)]
     L.km_batch_train_f32.restype = I
@@ -555,6 +557,30 @@ class Engine:
             ctypes.byref(obj))
         self._check(st)
         return labels, (obj.value if objective else None)
+
+    def infer_device_noalloc(self, x, c, labels, objective=False, bf16=False):
+        """km_lloyd_infer_f32_noalloc: labels of the float32 device rows x against
+        the device centroids c, written into the caller's int32 device tensor
+        `labels` (n entries); the library allocates nothing during the call."""
+        import torch
+
+        _check_device_rows(x)
+        if x.dtype != torch.float32:
+            raise TypeError("km_lloyd_infer_f32_noalloc takes float32 rows")
+        n, d = x.shape
+        _check_centroids(c, d, "float32", True, x.device)
+        if (not isinstance(labels, torch.Tensor) or labels.dtype != torch.int32 or
+                labels.device != x.device or labels.numel() < n or not labels.is_contiguous()):
+            raise ValueError("labels must be a contiguous int32 tensor of at least n entries "
+                             "on the rows' device")
+        obj = ctypes.c_double()
+        flags = KM_FLAG_DEVICE_POINTERS | (KM_FLAG_EXACT_OBJECTIVE if objective else 0) | (
+            KM_FLAG_BF16_REFINE if bf16 else 0)
+        st = self.lib.km_lloyd_infer_f32_noalloc(
+            self.ctx, x.data_ptr(), n, d, c.data_ptr(), c.shape[0], flags, labels.data_ptr(),
+            ctypes.byref(obj))
+        self._check(st)
+        return obj.value if objective else None
 
 
 def _raise_server(st):
