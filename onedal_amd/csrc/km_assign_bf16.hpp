@@ -89,68 +89,73 @@ __device__ __forceinline__ double bf16_threshold(const bf16_consts& k, const bf1
 __host__ __device__ inline int bf16_padded_k(int k) { return (k + 127) / 128 * 128; }
 
 // ---------------------------------------------------------------------------
-// X -> padded BF16 rows (stride DPB), ||x|| and ||x - bf16(x)|| (rounded up),
-// and their maxima.  One lane per row (64 rows per wave); each step reads one
-// 128-byte line of the row and writes its 32 bf16 values as four 16-byte stores.
-// The sums are FP64; the 1e-12 factor covers their rounding, so their order is
-// free.  DPB is a multiple of 32.
+// X -> padded BF16 rows (stride dpb), ||x|| and ||x - bf16(x)|| (rounded up),
+// and their maxima.  A row is covered by L = dpb / 4 consecutive lanes, lane f
+// converting features 4f .. 4f + 3 (zero past d), so a wave reads 64 / L rows as
+// one contiguous run of 16-byte loads and writes their bf16 rows as 8-byte stores.
+// The per-row sums are FP64 over shuffles; the 1e-12 factor covers their
+// rounding, so their order is free.  dpb = pad_dim(d) <= 256 (a power of two,
+// at least 32).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void convert_rows_bf16_kernel(
     const float* __restrict__ X, int64_t n, int d, int dpb, uint16_t* __restrict__ Xb,
     float* __restrict__ xnorm_up, float* __restrict__ enorm_up,
     unsigned long long* __restrict__ max_bits /* [0] max ||x||, [1] max ex */) {
     const int lane = threadIdx.x & 63;
+    const int L = dpb >> 2;                 // lanes per row: 8 .. 64
+    const int R = 64 / L;                   // rows per wave step
+    const int fl = lane & (L - 1);          // this lane's 4-feature group
+    const int sub = lane / L;               // this lane's row within the step
     const int64_t wave = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
     const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
     const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0;
+    const int f0 = 4 * fl;
     double mx = 0.0, me = 0.0;
-    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
-        const int64_t row = base + lane;
-        if (row >= n) continue;
-        const float* x = X + row * d;
-        uint16_t* xb = Xb + row * dpb;
-        double s = 0.0, e = 0.0;
-        for (int f0 = 0; f0 < dpb; f0 += 32) {
-            float v[32];
-            if (vec && f0 + 32 <= d) {
+    for (int64_t base = wave * R; base < n; base += nwaves * R) {
+        const int64_t row = base + sub;
+        const bool ok = row < n;
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (ok) {
+            const float* x = X + row * d;
+            if (vec && f0 < d) {
+                const float4 t = *reinterpret_cast<const float4*>(x + f0);
+                v[0] = t.x;
+                v[1] = t.y;
+                v[2] = t.z;
+                v[3] = t.w;
+            } else if (!vec) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float4 t = *reinterpret_cast<const float4*>(x + f0 + 4 * q);
-                    v[4 * q] = t.x;
-                    v[4 * q + 1] = t.y;
-                    v[4 * q + 2] = t.z;
-                    v[4 * q + 3] = t.w;
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < 32; ++u) v[u] = f0 + u < d ? x[f0 + u] : 0.f;
+                for (int u = 0; u < 4; ++u) v[u] = f0 + u < d ? x[f0 + u] : 0.f;
             }
-            uint32_t w[16];
-#pragma unroll
-            for (int u = 0; u < 32; u += 2) {
-                const uint16_t b0 = f2bf(v[u]), b1 = f2bf(v[u + 1]);
-                w[u / 2] = uint32_t(b0) | (uint32_t(b1) << 16);
-                const double d0 = double(v[u]), d1 = double(v[u + 1]);
-                const double e0 = d0 - double(bf2f(b0)), e1 = d1 - double(bf2f(b1));
-                s = fma(d0, d0, s);
-                s = fma(d1, d1, s);
-                e = fma(e0, e0, e);
-                e = fma(e1, e1, e);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                *reinterpret_cast<uint4*>(xb + f0 + 8 * q) =
-                    make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
         }
-        const double r = sqrt(s) * (1.0 + 1e-12);
-        const double re = sqrt(e) * (1.0 + 1e-12) + 1e-30;
-        float fr = float(r), fe = float(re);
-        if (double(fr) < r) fr = __uint_as_float(__float_as_uint(fr) + 1u);
-        if (double(fe) < re) fe = __uint_as_float(__float_as_uint(fe) + 1u);
-        xnorm_up[row] = fr;
-        enorm_up[row] = fe;
-        mx = fmax(mx, double(fr));
-        me = fmax(me, double(fe));
+        double s = 0.0, e = 0.0;
+        uint16_t b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            b[u] = f2bf(v[u]);
+            const double dv = double(v[u]);
+            const double ev = dv - double(bf2f(b[u]));
+            s = fma(dv, dv, s);
+            e = fma(ev, ev, e);
+        }
+        if (ok)
+            *reinterpret_cast<uint2*>(Xb + row * dpb + f0) =
+                make_uint2(uint32_t(b[0]) | (uint32_t(b[1]) << 16), uint32_t(b[2]) | (uint32_t(b[3]) << 16));
+        for (int off = L >> 1; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off);
+            e += __shfl_xor(e, off);
+        }
+        if (ok && fl == 0) {
+            const double r = sqrt(s) * (1.0 + 1e-12);
+            const double re = sqrt(e) * (1.0 + 1e-12) + 1e-30;
+            float fr = float(r), fe = float(re);
+            if (double(fr) < r) fr = __uint_as_float(__float_as_uint(fr) + 1u);
+            if (double(fe) < re) fe = __uint_as_float(__float_as_uint(fe) + 1u);
+            xnorm_up[row] = fr;
+            enorm_up[row] = fe;
+            mx = fmax(mx, double(fr));
+            me = fmax(me, double(fe));
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

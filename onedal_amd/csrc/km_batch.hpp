@@ -372,7 +372,9 @@ __global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_i
             top2<float> st[2];
             st[0].init();
             st[1].init();
-            for (int mb = 0; mb < mblocks; ++mb) {
+            // block mb into slot sl (even blocks slot 0, odd blocks slot 1; the
+            // slot is a compile-time index, so the slots stay in registers)
+            auto block = [&](int mb, top2<float>& slot) {
                 floatx16 acc;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) acc[i] = 0.f;
@@ -385,7 +387,6 @@ __global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_i
                         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[q], xr[s4 + q], acc, 0, 0, 0);
                 }
                 const float* cnb = Cn + mb * 32;
-                top2<float>& slot = st[mb & 1];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const floatx4 c4 = *reinterpret_cast<const floatx4*>(cnb + 8 * q + 4 * h);
@@ -393,6 +394,10 @@ __global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_i
                     for (int t = 0; t < 4; ++t)
                         slot.push(__builtin_fmaf(-2.f, acc[4 * q + t], c4[t]), mb * 32 + 8 * q + 4 * h + t);
                 }
+            };
+            for (int mb = 0; mb < mblocks; mb += 2) {
+                block(mb, st[0]);
+                if (mb + 1 < mblocks) block(mb + 1, st[1]);
             }
             // lane r32 (h = 0) holds the row's four slots: its two and lane r32 + 32's
             top2<float> sl[4];
@@ -420,27 +425,31 @@ __global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_i
                 const double T = bi >= 0 ? band_threshold(bc, Xup, cmax, double(best)) : 0.0;
                 bool cert = !cbad && bi >= 0 && bi < k && T < double(finf);
                 int cnt = 0;
-                int cand[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (double(sl[q].m2) <= T) cert = false;
                     if (sl[q].i1 >= 0 && double(sl[q].m1) <= T) {
                         if (sl[q].i1 >= k) cert = false;
-                        cand[cnt++] = sl[q].i1;
+                        ++cnt;
                     }
                 }
                 if (cert && cnt == 1) {
                     lab = bi;
                 } else if (cert) {
-                    // exact form over the candidates, ties to the lowest index
+                    // exact form over the candidates (the slot minima inside T, in
+                    // slot order; slots are visited by a fixed unrolled loop, so no
+                    // candidate array is indexed at run time), ties to the lowest index
                     float bd = finf;
                     lab = k;
-                    for (int c = 0; c < cnt; ++c) {
-                        const int j = cand[c];
-                        const float D = canon_dist<float>(xrow, Cs + j * CST, d);
-                        if (D < bd || (D == bd && j < lab)) {
-                            bd = D;
-                            lab = j;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (sl[q].i1 >= 0 && double(sl[q].m1) <= T) {
+                            const int j = sl[q].i1;
+                            const float D = canon_dist<float>(xrow, Cs + j * CST, d);
+                            if (D < bd || (D == bd && j < lab)) {
+                                bd = D;
+                                lab = j;
+                            }
                         }
                     }
                     if (lab >= k) lab = bi;
@@ -677,77 +686,149 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
     const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
     const int64_t total = qpre[B];
-    for (int64_t q = gw; q < total; q += nw) {
-        const int ii = batch_find(qpre, B, q);
-        const batch_inst in = I[ii];
-        const int d = in.d, k = in.k;
-        const int32_t lq = int32_t(q - qpre[ii]);
-        const int64_t* off = offsets + in.k0 + ii;
-        const int32_t* cb = chunk_base + in.k0 + ii;
-        int lo = 0, hi = k;  // cb[k] = chunks > lq
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (cb[mid] <= lq) lo = mid; else hi = mid;
-        }
-        const int j = lo;
-        const int64_t m_begin = off[j] + int64_t(lq - cb[j]) * REDUCE_CHUNK;
-        const int64_t m_end = min(m_begin + REDUCE_CHUNK, off[j + 1]);
-        const int32_t* pm = perm + in.row0;
-        double* out = part + in.p0 + int64_t(lq) * d;
-        if (d <= 32) {
-            const int G = reduce_groups(d);
-            const int g = lane / d, f0 = lane % d;
-            const bool active = g < G;
-            double acc = 0.0;
-            int64_t m = m_begin;
-            // eight steps of G members: ids and values loaded first, added in order
-            for (; m + 8 * G <= m_end; m += 8 * G) {
-                if (active) {
-                    int64_t r[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) r[u] = int64_t(pm[m + u * G + g]) - in.row0;
-                    double v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = double(in.X[r[u] * d + f0]);
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = acc + v[u];
+    // waves take runs of RUN consecutive chunks, round robin: the owning instance
+    // and cluster are searched once per run and then advanced chunk by chunk
+    constexpr int RUN = 16;
+    for (int64_t q0 = gw * RUN; q0 < total; q0 += nw * RUN) {
+        const int64_t q1 = min<int64_t>(q0 + RUN, total);
+        int ii = batch_find(qpre, B, q0);
+        int j = -1;
+        for (int64_t q = q0; q < q1; ++q) {
+            while (q >= qpre[ii + 1]) {  // next instance (every instance has chunks)
+                ++ii;
+                j = -1;
+            }
+            const batch_inst in = I[ii];
+            const int d = in.d, k = in.k;
+            const int32_t lq = int32_t(q - qpre[ii]);
+            const int64_t* off = offsets + in.k0 + ii;
+            const int32_t* cb = chunk_base + in.k0 + ii;
+            if (j < 0) {
+                int lo = 0, hi = k;  // cb[k] = chunks > lq
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cb[mid] <= lq) lo = mid; else hi = mid;
                 }
+                j = lo;
             }
-            for (; m < m_end; m += G) {
-                const int64_t mm = m + g;
-                if (active && mm < m_end) {
-                    const int64_t row = int64_t(pm[mm]) - in.row0;
-                    acc = acc + double(in.X[row * d + f0]);
-                }
-            }
-            double tot = acc;
-            for (int gg = 1; gg < G; ++gg) {
-                const double o = __shfl(acc, gg * d + f0);
-                tot = tot + o;
-            }
-            if (g == 0) out[f0] = tot;
-        } else {
-            for (int fc = 0; fc < d; fc += 64) {
-                const int f = fc + lane;
-                double acc = 0.0;
+            while (cb[j + 1] <= lq) ++j;  // clusters without chunks are passed over
+            const int64_t m_begin = off[j] + int64_t(lq - cb[j]) * REDUCE_CHUNK;
+            const int64_t m_end = min(m_begin + REDUCE_CHUNK, off[j + 1]);
+            const int32_t* pm = perm + in.row0;
+            double* out = part + in.p0 + int64_t(lq) * d;
+            if (in.vec && d <= 256) {
+                // 16-byte row reads: lane (g, f4) owns features 4 f4 .. 4 f4 + 3 of
+                // group g's members (d <= 32: the G interleaved partials of the scalar
+                // path, G * d / 4 <= 16 lanes), or of every member (d > 32: d / 4
+                // lanes); each feature is summed over the same members in the same
+                // order as the scalar path, so the partials are identical.  The member
+                // ids of the next step are loaded one step ahead of the row reads.
+                typedef float F4 __attribute__((ext_vector_type(4)));
+                const int d4 = d >> 2;
+                const int G = reduce_groups(d);
+                const int g = d <= 32 ? lane / d4 : 0, f4 = d <= 32 ? lane - g * d4 : lane;
+                const bool active = d <= 32 ? g < G : lane < d4;
+                const int S = d <= 32 ? G : 1;  // member stride of a lane
+                constexpr int U = 8;
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
                 int64_t m = m_begin;
-                for (; m + 16 <= m_end; m += 16) {
-                    int64_t r[16];
+                int32_t r[U];
+                if (active && m + U * S <= m_end) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) r[u] = int64_t(pm[m + u]) - in.row0;
-                    if (f < d) {
-                        double v[16];
+                    for (int u = 0; u < U; ++u) r[u] = pm[m + u * S + g];
+                }
+                for (; m + U * S <= m_end; m += U * S) {
+                    if (active) {
+                        const bool more = m + 2 * U * S <= m_end;
+                        int32_t rn[U];
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) v[u] = double(in.X[r[u] * d + f]);
+                        for (int u = 0; u < U; ++u) rn[u] = more ? pm[m + U * S + u * S + g] : 0;
+                        F4 xv[U];
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) acc = acc + v[u];
+                        for (int u = 0; u < U; ++u)
+                            xv[u] = *reinterpret_cast<const F4*>(in.X + (int64_t(r[u]) - in.row0) * d + 4 * f4);
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(xv[u][e]);
+#pragma unroll
+                        for (int u = 0; u < U; ++u) r[u] = rn[u];
                     }
                 }
-                for (; m < m_end; ++m) {
-                    const int64_t r1 = int64_t(pm[m]) - in.row0;
-                    if (f < d) acc = acc + double(in.X[r1 * d + f]);
+                for (; m < m_end; m += S) {
+                    const int64_t mm = m + g;
+                    if (active && mm < m_end) {
+                        const int64_t row = int64_t(pm[mm]) - in.row0;
+                        const F4 x4 = *reinterpret_cast<const F4*>(in.X + row * d + 4 * f4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(x4[e]);
+                    }
                 }
-                if (f < d) out[f] = acc;
+                double tot[4] = {acc[0], acc[1], acc[2], acc[3]};
+                if (d <= 32) {
+                    // group partials into group 0, in group order
+                    for (int gg = 1; gg < G; ++gg)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) tot[e] = tot[e] + __shfl(acc[e], gg * d4 + f4);
+                }
+                if (active && g == 0)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) out[4 * f4 + e] = tot[e];
+            } else if (d <= 32) {
+                const int G = reduce_groups(d);
+                const int g = lane / d, f0 = lane % d;
+                const bool active = g < G;
+                double acc = 0.0;
+                int64_t m = m_begin;
+                // eight steps of G members: ids and values loaded first, added in order
+                for (; m + 8 * G <= m_end; m += 8 * G) {
+                    if (active) {
+                        int64_t r[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) r[u] = int64_t(pm[m + u * G + g]) - in.row0;
+                        double v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) v[u] = double(in.X[r[u] * d + f0]);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+                    }
+                }
+                for (; m < m_end; m += G) {
+                    const int64_t mm = m + g;
+                    if (active && mm < m_end) {
+                        const int64_t row = int64_t(pm[mm]) - in.row0;
+                        acc = acc + double(in.X[row * d + f0]);
+                    }
+                }
+                double tot = acc;
+                for (int gg = 1; gg < G; ++gg) {
+                    const double o = __shfl(acc, gg * d + f0);
+                    tot = tot + o;
+                }
+                if (g == 0) out[f0] = tot;
+            } else {
+                for (int fc = 0; fc < d; fc += 64) {
+                    const int f = fc + lane;
+                    double acc = 0.0;
+                    int64_t m = m_begin;
+                    for (; m + 16 <= m_end; m += 16) {
+                        int64_t r[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) r[u] = int64_t(pm[m + u]) - in.row0;
+                        if (f < d) {
+                            double v[16];
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) v[u] = double(in.X[r[u] * d + f]);
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) acc = acc + v[u];
+                        }
+                    }
+                    for (; m < m_end; ++m) {
+                        const int64_t r1 = int64_t(pm[m]) - in.row0;
+                        if (f < d) acc = acc + double(in.X[r1 * d + f]);
+                    }
+                    if (f < d) out[f] = acc;
+                }
             }
         }
     }

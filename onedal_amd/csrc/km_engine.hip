@@ -434,6 +434,17 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
     KM_LAUNCH_CHECK();
 }
 
+// objective chunk sums (objective_chunks_kernel, staged through dynamic LDS)
+template <typename F>
+static void launch_objective(km_ctx* ctx, int grid, const F* X, int64_t n, int d, const F* C,
+                             const int32_t* labels, double* d64_out, double* chunks) {
+    const size_t lds = objective_lds_bytes<F>();
+    auto kern = objective_chunks_kernel<F>;
+    KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, X, n, d, C, labels, d64_out, chunks);
+    KM_LAUNCH_CHECK();
+}
+
 // objective of the current labels against C; optional per-row d64 output
 template <typename F>
 static double* objective(km_ctx* ctx, work<F>& w, const F* C, double* d64_out) {
@@ -441,8 +452,7 @@ static double* objective(km_ctx* ctx, work<F>& w, const F* C, double* d64_out) {
     double* chunks = ctx->obj_chunks.get<double>(nchunks);
     double* out = ctx->obj_dev.get<double>(1);
     const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
-    hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, w.X, w.n,
-                       w.d, C, w.labels, d64_out, chunks);
+    launch_objective<F>(ctx, grid, w.X, w.n, w.d, C, w.labels, d64_out, chunks);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(sum_sequential_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks, nchunks, out);
     KM_LAUNCH_CHECK();
@@ -920,9 +930,7 @@ static void infer_noalloc(km_ctx* ctx, const float* X, int64_t n, int64_t d64_, 
             const int64_t nchunks = (rows + OBJ_CHUNK - 1) / OBJ_CHUNK;
             double* chunks = A.get<double>(nchunks);
             const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
-            hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, w.X,
-                               w.n, w.d, C, w.labels, (double*)nullptr, chunks);
-            KM_LAUNCH_CHECK();
+            launch_objective<F>(ctx, grid, w.X, w.n, w.d, C, w.labels, nullptr, chunks);
             hipLaunchKernelGGL(sum_sequential_into_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks,
                                nchunks, acc);
             KM_LAUNCH_CHECK();
@@ -1561,11 +1569,14 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
         KM_HIP(hipEventRecord(e0, ctx->stream));
     }
     if ((flags & KM_FLAG_GRAPH) && max_iter > 0) {
+        // the whole job -- every iteration and the final pass -- is one graph,
+        // launched once
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         KM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            plan_iteration(ctx, p);
+            for (int64_t it = 0; it < max_iter; ++it) plan_iteration(ctx, p);
+            plan_finish(ctx, p);
         } catch (...) {
             hipGraph_t broken = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &broken);
@@ -1578,11 +1589,21 @@ static void batch_train(km_ctx* ctx, const km_batch_instance_f32* inst, int64_t 
             (void)hipGraphDestroy(graph);
             KM_HIP(ie);
         }
-        hipError_t le = hipSuccess;
-        for (int64_t it = 0; it < max_iter && le == hipSuccess; ++it) le = hipGraphLaunch(exec, ctx->stream);
+        const hipError_t le = hipGraphLaunch(exec, ctx->stream);
+        // the executable graph may be destroyed while its launch is in flight
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         KM_HIP(le);
+        if (ctx->timing) KM_HIP(hipEventRecord(e1, ctx->stream));
+        KM_HIP(hipMemcpyAsync(obj_out, p.objd, sizeof(double) * p.B, hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < p.B; ++i) iters_out[i] = max_iter;
+        if (ctx->timing) {
+            float ms = 0.f;
+            KM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ctx->stats.iterations_ms = ms;
+        }
+        return;
     } else {
         // eager, timed: events around every iteration's assignment stage
         std::vector<hipEvent_t> asg;

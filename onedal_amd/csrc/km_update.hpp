@@ -303,23 +303,34 @@ __global__ __launch_bounds__(256) void reduce_chunks_vec_kernel(
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[v][e] = 0.0;
         int64_t m = m_begin;
-        for (; m + 8 <= m_end; m += 8) {
-            int64_t r[8];
+        // member ids one batch ahead of the row loads (the row loads of a batch
+        // then wait on one memory latency, not two)
+        int32_t r[8];
+        if (m + 8 <= m_end) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) r[u] = perm[m + u];
+        }
+        for (; m + 8 <= m_end; m += 8) {
+            int32_t rn[8];
+            const bool more = m + 16 <= m_end;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rn[u] = more ? perm[m + 8 + u] : 0;
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 const int c4 = sl + lpc * v;
                 if (c4 < d4) {
                     F4 xv[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) xv[u] = *reinterpret_cast<const F4*>(X + r[u] * d + 4 * c4);
+                    for (int u = 0; u < 8; ++u)
+                        xv[u] = *reinterpret_cast<const F4*>(X + int64_t(r[u]) * d + 4 * c4);
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
 #pragma unroll
                         for (int e = 0; e < 4; ++e) acc[v][e] = acc[v][e] + double(xv[u][e]);
                 }
             }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r[u] = rn[u];
         }
         for (; m < m_end; ++m) {
             const int64_t r0 = perm[m];
@@ -382,67 +393,89 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
 // order; the 256 partials are combined by the tree p[t] += p[t+s], s = 128..1;
 // chunk sums are added in chunk order by sum_sequential_kernel.
 // ---------------------------------------------------------------------------
+// Rows reach the threads through LDS: for every 256-row group of a chunk, the
+// rows' features and their centroids' features are staged one 128-byte slice
+// (OBJ_SLICE<F> elements) at a time with contiguous loads, and thread t adds its
+// row's squared differences slice after slice, so d64 keeps the feature order.
+template <typename F>
+struct obj_slice {
+    static constexpr int FT = 128 / int(sizeof(F));  // elements per slice
+    static constexpr int ST = FT + 1;                 // LDS row stride (bank spread)
+};
+template <typename F>
+__host__ inline size_t objective_lds_bytes() {
+    return size_t(2) * 256 * obj_slice<F>::ST * sizeof(F);
+}
+
 template <typename F>
 __global__ __launch_bounds__(256) void objective_chunks_kernel(
     const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C,
     const int32_t* __restrict__ labels, double* __restrict__ d64_out,
     double* __restrict__ chunk_sums) {
+    constexpr int FT = obj_slice<F>::FT, ST = obj_slice<F>::ST;
+    constexpr int V = 16 / int(sizeof(F));            // elements per 16-byte load
     __shared__ double part[256];
+    __shared__ int32_t slab[256];
+    extern __shared__ __align__(16) unsigned char obj_smem[];
+    F* xs = reinterpret_cast<F*>(obj_smem);         // [256][ST]
+    F* cs = xs + 256 * ST;                          // [256][ST]
     const int t = threadIdx.x;
-    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(F))) == 0 &&
-                     (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0;
-    typedef F F4 __attribute__((ext_vector_type(4)));
+    const bool vec = (d % V) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0 &&
+                     (reinterpret_cast<uintptr_t>(C) % 16) == 0;
+    typedef F FV __attribute__((ext_vector_type(V)));
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     for (int64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
         const int64_t r0 = q * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
         double p = 0.0;
         // thread t owns chunk rows t, t+256, ... in increasing order
-        for (int r = t; r < rows; r += 256) {
-            const int64_t row = r0 + r;
-            const F* x = X + row * d;
-            const F* c = C + int64_t(labels[row]) * d;
+        for (int g0 = 0; g0 < rows; g0 += 256) {
+            const int nr = min(256, rows - g0);
+            const int64_t gb = r0 + g0;
+            __syncthreads();  // the previous group's readers are done with slab, xs, cs
+            slab[t] = t < nr ? labels[gb + t] : 0;
+            __syncthreads();
             double acc = 0.0;
-            int f = 0;
-            if (vec) {
-                // whole 128-byte lines of the row per step (8 loads issued together),
-                // summed in feature order
-                constexpr int LV = 128 / (4 * int(sizeof(F)));  // 4-vectors per line
-                for (; f + 4 * LV <= d; f += 4 * LV) {
-                    F4 xv[LV], cv[LV];
-#pragma unroll
-                    for (int u = 0; u < LV; ++u) {
-                        xv[u] = *reinterpret_cast<const F4*>(x + f + 4 * u);
-                        cv[u] = *reinterpret_cast<const F4*>(c + f + 4 * u);
-                    }
-#pragma unroll
-                    for (int u = 0; u < LV; ++u)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const double diff = double(xv[u][e]) - double(cv[u][e]);
-                            const double sq = diff * diff;
-                            acc = acc + sq;
+            for (int f0 = 0; f0 < d; f0 += FT) {
+                const int ft = min(FT, d - f0);
+                if (vec) {
+                    // V-element vectors: (row, part) = (e / (FT / V), e % (FT / V))
+                    constexpr int PPR = FT / V;
+                    for (int e = t; e < nr * PPR; e += 256) {
+                        const int r = e / PPR, pp = e - r * PPR;
+                        const int f = pp * V;
+                        FV xv = {}, cv = {};
+                        if (f < ft) {
+                            xv = *reinterpret_cast<const FV*>(X + (gb + r) * d + f0 + f);
+                            cv = *reinterpret_cast<const FV*>(C + int64_t(slab[r]) * d + f0 + f);
                         }
-                }
-                for (; f < d; f += 4) {
-                    const F4 x4 = *reinterpret_cast<const F4*>(x + f);
-                    const F4 c4 = *reinterpret_cast<const F4*>(c + f);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const double diff = double(x4[u]) - double(c4[u]);
+                        for (int u = 0; u < V; ++u) {
+                            xs[r * ST + f + u] = xv[u];
+                            cs[r * ST + f + u] = cv[u];
+                        }
+                    }
+                } else {
+                    for (int e = t; e < nr * ft; e += 256) {
+                        const int r = e / ft, f = e - r * ft;
+                        xs[r * ST + f] = X[(gb + r) * d + f0 + f];
+                        cs[r * ST + f] = C[int64_t(slab[r]) * d + f0 + f];
+                    }
+                }
+                __syncthreads();
+                if (t < nr) {
+                    for (int f = 0; f < ft; ++f) {
+                        const double diff = double(xs[t * ST + f]) - double(cs[t * ST + f]);
                         const double sq = diff * diff;
                         acc = acc + sq;
                     }
                 }
-            } else {
-                for (; f < d; ++f) {
-                    const double diff = double(x[f]) - double(c[f]);
-                    const double sq = diff * diff;
-                    acc = acc + sq;
-                }
+                __syncthreads();
             }
-            if (d64_out) d64_out[row] = acc;
-            p = p + acc;
+            if (t < nr) {
+                if (d64_out) d64_out[gb + t] = acc;
+                p = p + acc;
+            }
         }
         part[t] = p;
         __syncthreads();
