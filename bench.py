@@ -222,6 +222,18 @@ def run_batched(torch, args, wl):
     st = eng.stats()
     asg_ms = st["assign_kernel_ms"] / max(1, st["assign_launches"])
     achieved = flops_pit / (asg_ms * 1e-3) / 1e12
+    # the same stage against HBM: one read of every row (4 d bytes) and its label
+    # write (4 bytes) per iteration; small-k buckets are bound by this, not by flops
+    bytes_pit = sum(x.shape[0] * (4.0 * x.shape[1] + 4.0) for x in xs)
+    hbm_gbs = bytes_pit / (asg_ms * 1e-3) / 1e9
+    if args.copy_calibration:
+        log("copy calibration: 3 x 1 GiB device-to-device copies ...")
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -268,7 +280,9 @@ def run_batched(torch, args, wl):
                      "kernel": "assignment stage of one iteration (all tile kinds; 2kd flops per "
                                "row; HIP events, eager run)",
                      "avg_stage_ms": round(asg_ms, 3),
-                     "whole_iteration_tflops": round(whole, 2)},
+                     "whole_iteration_tflops": round(whole, 2),
+                     "hbm": {"algorithmic_bytes_per_stage": bytes_pit, "achieved_gbs": round(hbm_gbs, 1),
+                             "peak_gbs": PEAK["hbm_gbs"], "frac": round(hbm_gbs / PEAK["hbm_gbs"], 4)}},
         "cpu_baseline": cpu,
         "loop_ms_per_step": sum(loop_ms) / len(loop_ms),
     }

@@ -110,6 +110,33 @@ def test_train_bit_exact_vs_canonical(engine, oracle, dtype, path, shape):
     assert_bit_exact(got, oracle.canonical_lloyd(x, c0, iters, 0.0))
 
 
+BF16_EDGE_CASES = [
+    # name, n, d, k, iters, spread: rows nearly identical (spread 1e-3) put every
+    # centroid inside every row's BF16 error band (collect and full re-score
+    # paths); k = 1500 spans 48 centroid blocks; k = 130 leaves padding blocks
+    ("all_centroids_in_band", 6000, 64, 300, 3, 1e-3),
+    ("k1500_48_blocks", 30000, 32, 1500, 3, None),
+    ("k_not_multiple_of_128", 20000, 128, 130, 4, None),
+]
+
+
+@pytest.mark.parametrize("case", BF16_EDGE_CASES, ids=lambda c: c[0])
+def test_bf16_edge_cases_bit_exact(engine, oracle, case):
+    """BF16 matrix-core distances with exact re-score at the band's extremes: the
+    result stays the canonical one."""
+    _, n, d, k, iters, spread = case
+    bf16 = set_path(engine, "bf16")
+    if spread is None:
+        x = gaussian_mixture(n, d, k, seed=n + k, center_range=10.0, sigma=(0.5, 2.0))
+    else:
+        rng = np.random.default_rng(3)
+        x = (1.0 + spread * rng.standard_normal((n, d))).astype(np.float32)
+    rng = np.random.default_rng(k + 1)
+    c0 = x[rng.choice(n, size=k, replace=False)].copy()
+    got = engine.train(x, c0, iters, 0.0, bf16=bf16)
+    assert_bit_exact(got, oracle.canonical_lloyd(x, c0, iters, 0.0))
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_train_vs_daal_restatement(engine, oracle, dtype):
     """The reference's CPU workload shape (n=1e4, d=20, k=10, first-k init,
