@@ -671,6 +671,118 @@ __global__ __launch_bounds__(1024) void batch_chunk_prefix_kernel(const int32_t*
     if (t == 0) qpre[B] = carry;
 }
 
+// Chunk sums of the instances whose rows allow 16-byte reads (d % 4 == 0,
+// d <= 256, X 16-byte aligned).  A wave packs S = 64 / lpc consecutive chunks
+// of one instance into sub-waves of lpc lanes; within a sub-wave lane (g, f4)
+// owns features 4 f4 .. 4 f4 + 3 of group g's members (d <= 32: the
+// reduce_groups(d) interleaved partials of the scalar kernel, G * d / 4 <= 16
+// lanes) or of every member (d > 32: d / 4 lanes), so each feature is summed over
+// the same members in the same order as batch_reduce_kernel's and the partials
+// are identical.  Waves take runs of RUN consecutive chunks, round robin; member
+// ids are loaded one step ahead of the row reads.
+__global__ __launch_bounds__(256) void batch_reduce_vec_kernel(const batch_inst* __restrict__ I, int B,
+                                                               const int64_t* __restrict__ qpre,
+                                                               const int32_t* __restrict__ perm,
+                                                               const int64_t* __restrict__ offsets,
+                                                               const int32_t* __restrict__ chunk_base,
+                                                               double* __restrict__ part) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    constexpr int RUN = 16;
+    constexpr int U = 8;
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
+    const int64_t total = qpre[B];
+    for (int64_t q0 = gw * RUN; q0 < total; q0 += nw * RUN) {
+        const int64_t q1 = min<int64_t>(q0 + RUN, total);
+        int ii = batch_find(qpre, B, q0);
+        int64_t q = q0;
+        while (q < q1) {
+            while (q >= qpre[ii + 1]) ++ii;  // every instance has chunks
+            const batch_inst in = I[ii];
+            const int64_t inst_end = min<int64_t>(q1, qpre[ii + 1]);
+            const int d = in.d;
+            if (!(in.vec && d <= 256)) {  // batch_reduce_kernel's chunks
+                q = inst_end;
+                continue;
+            }
+            const int d4 = d >> 2;
+            const int lpc = d <= 64 ? 16 : (d <= 128 ? 32 : 64);  // lanes per chunk
+            const int S = 64 / lpc;
+            const int64_t qe = min<int64_t>(inst_end, q + S);
+            const int sub = lane / lpc, sl = lane - sub * lpc;
+            const int64_t myq = q + sub;
+            const bool has = myq < qe;
+            const int G = reduce_groups(d);
+            const int g = d <= 32 ? sl / d4 : 0, f4 = d <= 32 ? sl - g * d4 : sl;
+            const bool active = has && (d <= 32 ? g < G : sl < d4);
+            const int Sm = d <= 32 ? G : 1;  // member stride of a lane
+            int64_t m_begin = 0, m_end = 0;
+            double* out = nullptr;
+            if (has) {
+                const int32_t lq = int32_t(myq - qpre[ii]);
+                const int64_t* off = offsets + in.k0 + ii;
+                const int32_t* cb = chunk_base + in.k0 + ii;
+                int lo = 0, hi = in.k;  // cb[k] = chunks > lq
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cb[mid] <= lq) lo = mid; else hi = mid;
+                }
+                m_begin = off[lo] + int64_t(lq - cb[lo]) * REDUCE_CHUNK;
+                m_end = min(m_begin + REDUCE_CHUNK, off[lo + 1]);
+                out = part + in.p0 + int64_t(lq) * d;
+            }
+            const int32_t* pm = perm + in.row0;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            if (active) {
+                int64_t m = m_begin;
+                int32_t r[U];
+                if (m + U * Sm <= m_end) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) r[u] = pm[m + u * Sm + g];
+                }
+                for (; m + U * Sm <= m_end; m += U * Sm) {
+                    const bool more = m + 2 * U * Sm <= m_end;
+                    int32_t rn[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) rn[u] = more ? pm[m + U * Sm + u * Sm + g] : 0;
+                    F4 xv[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        xv[u] = *reinterpret_cast<const F4*>(in.X + (int64_t(r[u]) - in.row0) * d + 4 * f4);
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(xv[u][e]);
+#pragma unroll
+                    for (int u = 0; u < U; ++u) r[u] = rn[u];
+                }
+                for (; m < m_end; m += Sm) {
+                    const int64_t mm = m + g;
+                    if (mm < m_end) {
+                        const int64_t row = int64_t(pm[mm]) - in.row0;
+                        const F4 x4 = *reinterpret_cast<const F4*>(in.X + row * d + 4 * f4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(x4[e]);
+                    }
+                }
+            }
+            // every lane reaches the shuffles: group partials into group 0, in
+            // group order, within the sub-wave
+            double tot[4] = {acc[0], acc[1], acc[2], acc[3]};
+            if (d <= 32) {
+                for (int gg = 1; gg < G; ++gg)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) tot[e] = tot[e] + __shfl(acc[e], sub * lpc + gg * d4 + f4);
+            }
+            if (active && g == 0)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) out[4 * f4 + e] = tot[e];
+            q = qe;
+        }
+    }
+}
+
 // Chunk sums, one wave per chunk of REDUCE_CHUNK members of one cluster of one
 // instance, in the order of reduce_chunks_kernel: d <= 32 uses reduce_groups(d)
 // interleaved partials added in group order; otherwise each feature is summed
@@ -700,6 +812,11 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
             }
             const batch_inst in = I[ii];
             const int d = in.d, k = in.k;
+            if (in.vec && d <= 256) {
+                // batch_reduce_vec_kernel sums this instance's chunks
+                q = min<int64_t>(q1, qpre[ii + 1]) - 1;
+                continue;
+            }
             const int32_t lq = int32_t(q - qpre[ii]);
             const int64_t* off = offsets + in.k0 + ii;
             const int32_t* cb = chunk_base + in.k0 + ii;
@@ -716,65 +833,7 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
             const int64_t m_end = min(m_begin + REDUCE_CHUNK, off[j + 1]);
             const int32_t* pm = perm + in.row0;
             double* out = part + in.p0 + int64_t(lq) * d;
-            if (in.vec && d <= 256) {
-                // 16-byte row reads: lane (g, f4) owns features 4 f4 .. 4 f4 + 3 of
-                // group g's members (d <= 32: the G interleaved partials of the scalar
-                // path, G * d / 4 <= 16 lanes), or of every member (d > 32: d / 4
-                // lanes); each feature is summed over the same members in the same
-                // order as the scalar path, so the partials are identical.  The member
-                // ids of the next step are loaded one step ahead of the row reads.
-                typedef float F4 __attribute__((ext_vector_type(4)));
-                const int d4 = d >> 2;
-                const int G = reduce_groups(d);
-                const int g = d <= 32 ? lane / d4 : 0, f4 = d <= 32 ? lane - g * d4 : lane;
-                const bool active = d <= 32 ? g < G : lane < d4;
-                const int S = d <= 32 ? G : 1;  // member stride of a lane
-                constexpr int U = 8;
-                double acc[4] = {0.0, 0.0, 0.0, 0.0};
-                int64_t m = m_begin;
-                int32_t r[U];
-                if (active && m + U * S <= m_end) {
-#pragma unroll
-                    for (int u = 0; u < U; ++u) r[u] = pm[m + u * S + g];
-                }
-                for (; m + U * S <= m_end; m += U * S) {
-                    if (active) {
-                        const bool more = m + 2 * U * S <= m_end;
-                        int32_t rn[U];
-#pragma unroll
-                        for (int u = 0; u < U; ++u) rn[u] = more ? pm[m + U * S + u * S + g] : 0;
-                        F4 xv[U];
-#pragma unroll
-                        for (int u = 0; u < U; ++u)
-                            xv[u] = *reinterpret_cast<const F4*>(in.X + (int64_t(r[u]) - in.row0) * d + 4 * f4);
-#pragma unroll
-                        for (int u = 0; u < U; ++u)
-#pragma unroll
-                            for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(xv[u][e]);
-#pragma unroll
-                        for (int u = 0; u < U; ++u) r[u] = rn[u];
-                    }
-                }
-                for (; m < m_end; m += S) {
-                    const int64_t mm = m + g;
-                    if (active && mm < m_end) {
-                        const int64_t row = int64_t(pm[mm]) - in.row0;
-                        const F4 x4 = *reinterpret_cast<const F4*>(in.X + row * d + 4 * f4);
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) acc[e] = acc[e] + double(x4[e]);
-                    }
-                }
-                double tot[4] = {acc[0], acc[1], acc[2], acc[3]};
-                if (d <= 32) {
-                    // group partials into group 0, in group order
-                    for (int gg = 1; gg < G; ++gg)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) tot[e] = tot[e] + __shfl(acc[e], gg * d4 + f4);
-                }
-                if (active && g == 0)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) out[4 * f4 + e] = tot[e];
-            } else if (d <= 32) {
+            if (d <= 32) {
                 const int G = reduce_groups(d);
                 const int g = lane / d, f0 = lane % d;
                 const bool active = g < G;
@@ -871,9 +930,14 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const batch_inst* _
 
 // Row staging of the FP64 distance kernels: 256 rows of one instance, 32
 // features at a time (row stride 33 floats: a thread reading its own row hits its
-// own bank), so the X reads are coalesced whatever d is.
+// own bank), so the X reads are coalesced whatever d is; the same 32 features of
+// every centroid sit beside them when the instance has at most 128 centroids.
 constexpr int BATCH_D64_FT = 32;
-typedef float batch_d64_tile[256][BATCH_D64_FT + 1];
+constexpr int BATCH_D64_MAXK = 128;
+struct batch_d64_tile {
+    float x[256][BATCH_D64_FT + 1];
+    float c[BATCH_D64_MAXK][BATCH_D64_FT + 1];
+};
 
 // FP64 squared distance of row r0 + threadIdx.x (valid when threadIdx.x < nr) to
 // its labelled centroid: sum over the features in order of (double(x) - double(c))^2,
@@ -881,27 +945,46 @@ typedef float batch_d64_tile[256][BATCH_D64_FT + 1];
 // order); every thread of the workgroup must call it.  lab_base[r] is the label of row r.
 __device__ __forceinline__ double batch_block_d64(const batch_inst& in, int64_t r0, int nr,
                                                   const int32_t* __restrict__ lab_base,
-                                                  const float* __restrict__ Ci, batch_d64_tile& Xs) {
+                                                  const float* __restrict__ Ci, batch_d64_tile& T) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
     const int t = threadIdx.x;
-    const int d = in.d;
+    const int d = in.d, k = in.k;
     int lab = 0;
     if (t < nr) {
         lab = lab_base[r0 + t];
-        lab = (lab >= 0 && lab < in.k) ? lab : 0;
+        lab = (lab >= 0 && lab < k) ? lab : 0;
     }
+    const bool ctab = k <= BATCH_D64_MAXK;  // centroid slices staged beside the rows
     const float* c = Ci + int64_t(lab) * d;
     double acc = 0.0;
     for (int f0 = 0; f0 < d; f0 += BATCH_D64_FT) {
         const int ft = min(BATCH_D64_FT, d - f0);
         __syncthreads();
-        for (int e = t; e < nr * ft; e += 256) {
-            const int r = e / ft, q = e - r * ft;
-            Xs[r][q] = in.X[(r0 + r) * d + f0 + q];
+        if (in.vec) {
+            // d % 4 == 0, so ft % 4 == 0: 16-byte reads, four LDS writes each
+            const int v4 = ft >> 2;
+            for (int e = t; e < nr * v4; e += 256) {
+                const int r = e / v4, q = (e - r * v4) * 4;
+                const F4 v = *reinterpret_cast<const F4*>(in.X + (r0 + r) * d + f0 + q);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) T.x[r][q + u] = v[u];
+            }
+        } else {
+            for (int e = t; e < nr * ft; e += 256) {
+                const int r = e / ft, q = e - r * ft;
+                T.x[r][q] = in.X[(r0 + r) * d + f0 + q];
+            }
         }
+        if (ctab)
+            for (int e = t; e < k * ft; e += 256) {
+                const int j = e / ft, q = e - j * ft;
+                T.c[j][q] = Ci[int64_t(j) * d + f0 + q];
+            }
         __syncthreads();
         if (t < nr) {
             for (int q = 0; q < ft; ++q) {
-                const double diff = double(Xs[t][q]) - double(c[f0 + q]);
+                const float cv = ctab ? T.c[lab][q] : c[f0 + q];
+                const double diff = double(T.x[t][q]) - double(cv);
                 const double sq = diff * diff;
                 acc = acc + sq;
             }

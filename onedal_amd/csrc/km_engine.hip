@@ -1515,6 +1515,9 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_star
     size_t tmp_bytes = p.tmp_bytes;
     KM_HIP(rocprim::radix_sort_pairs(p.tmp, tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
                                      unsigned(p.rows), 0, p.kbits, ctx->stream));
+    hipLaunchKernelGGL(batch_reduce_vec_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre,
+                       p.perm, p.offsets, p.cb, p.part);
+    KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre, p.perm,
                        p.offsets, p.cb, p.part);
     KM_LAUNCH_CHECK();
