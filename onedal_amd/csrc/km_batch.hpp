@@ -39,6 +39,7 @@ struct batch_inst {
     int64_t k0;
     int64_t p0;
     int64_t o0;
+    int64_t h0;   // first of the instance's per-block label histograms (counting sort)
     int32_t d;
     int32_t k;
     int32_t index;
@@ -73,7 +74,7 @@ __device__ __forceinline__ int batch_find(const int64_t* __restrict__ pre, int B
 // The launch covers the instances sel[0..nsel) with pre the prefix of their
 // workgroup counts.
 // labels: concatenated labels (row0) or, with to_user, each instance's output;
-// skeys (when labels are concatenated): the row's sort key (index << lbits) |
+// skeys (when labels are concatenated and skeys is not null): the row's sort key (index << lbits) |
 // label, so one stable radix sort over the batch groups every instance's rows
 // by label in row order.
 // ---------------------------------------------------------------------------
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256) void batch_assign_kernel(const batch_inst* __r
                     in.labels_out[r0 + rr] = lab;
                 } else {
                     labels[in.row0 + r0 + rr] = lab;
-                    skeys[in.row0 + r0 + rr] = (uint32_t(in.index) << lbits) | uint32_t(lab);
+                    if (skeys) skeys[in.row0 + r0 + rr] = (uint32_t(in.index) << lbits) | uint32_t(lab);
                 }
             }
         }
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(256, 2) void batch_assign_mfma_kernel(const batch_i
                     in.labels_out[row] = lab;
                 } else {
                     labels[in.row0 + row] = lab;
-                    skeys[in.row0 + row] = (uint32_t(in.index) << lbits) | uint32_t(lab);
+                    if (skeys) skeys[in.row0 + row] = (uint32_t(in.index) << lbits) | uint32_t(lab);
                 }
             }
         }
@@ -544,7 +545,8 @@ __global__ void batch_tune_fill_kernel(float* __restrict__ p, int64_t count, uin
 __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __restrict__ I, int B,
                                                           const int64_t* __restrict__ pre,
                                                           const int32_t* __restrict__ labels,
-                                                          int32_t* __restrict__ counts) {
+                                                          int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ bhist) {
     __shared__ int hist[BATCH_HIST_MAX];
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -566,8 +568,154 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
         }
         __syncthreads();
         if (lds)
-            for (int j = threadIdx.x; j < in.k; j += 256)
+            for (int j = threadIdx.x; j < in.k; j += 256) {
                 if (hist[j]) atomicAdd(&counts[in.k0 + j], hist[j]);
+                // the block's histogram, for the counting sort (bhist: every k <= BATCH_HIST_MAX)
+                if (bhist) bhist[in.h0 + (b - pre[ii]) * in.k + j] = hist[j];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort of the row ids by label within each instance (the order
+// rocPRIM's radix sort of (instance, label) keys gives: rows of a cluster in
+// ascending row order), for batches whose k are all at most BATCH_HIST_MAX.
+// 1. batch_count_kernel writes each histogram block's label counts (bhist).
+// 2. batch_sortbase_kernel turns them into each block's first destination per
+//    label: the label's member offset plus the counts of the instance's earlier
+//    blocks (one thread per (instance, label), walking the blocks in order).
+// 3. batch_scatter_kernel: one workgroup per block.  Each wave ranks the rows of
+//    its quarter of the block within their label (one ballot per label bit per
+//    64 rows, in row order; a per-wave LDS counter per label carries the runs
+//    from one group of 64 rows to the next), the waves' counts give every row a
+//    slot in the block's label-ordered LDS staging array, and the staging array
+//    is written out in slot order, so each label's rows of the block leave as
+//    one contiguous run.  Labels outside [0, k) are skipped, as the histogram
+//    skips them.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void batch_sortbase_kernel(const batch_inst* __restrict__ I, int B,
+                                                             const int64_t* __restrict__ prek,
+                                                             const int64_t* __restrict__ offsets,
+                                                             int32_t* __restrict__ bhist) {
+    const int64_t total = prek[B];
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += int64_t(gridDim.x) * blockDim.x) {
+        const int ii = batch_find(prek, B, e);
+        const batch_inst in = I[ii];
+        const int j = int(e - prek[ii]);
+        const int64_t nb = (in.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
+        int32_t base = int32_t(offsets[in.k0 + ii + j]);
+        int32_t* h = bhist + in.h0 + j;
+        for (int64_t lb = 0; lb < nb; ++lb) {
+            const int32_t c = h[lb * in.k];
+            h[lb * in.k] = base;
+            base += c;
+        }
+    }
+}
+
+// LDS bytes of batch_scatter_kernel for labels below kcap
+__host__ __device__ constexpr size_t batch_scatter_lds_bytes(int kcap) {
+    return (size_t(6) * kcap + BATCH_CNT_ROWS) * sizeof(int32_t);
+}
+
+__global__ __launch_bounds__(256) void batch_scatter_kernel(const batch_inst* __restrict__ I, int B,
+                                                            const int64_t* __restrict__ pre,
+                                                            const int32_t* __restrict__ labels,
+                                                            const int32_t* __restrict__ bhist,
+                                                            int32_t* __restrict__ perm, int kcap) {
+    // LDS: wcnt[4][kcap] (each wave's label counts, then its first local slot per
+    // label), ls[kcap] (label's first local slot), gb[kcap] (label's destination),
+    // stage[BATCH_CNT_ROWS] (local slot -> label << 16 | row in block)
+    extern __shared__ int32_t slds[];
+    int32_t* ls = slds + 4 * kcap;
+    int32_t* gb = ls + kcap;
+    int32_t* stage = gb + kcap;
+    __shared__ int32_t wsum[4];
+    constexpr int G = BATCH_CNT_ROWS / 256;  // groups of 64 rows per wave
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    int32_t* wc = slds + wave * kcap;
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    const int64_t total = pre[B];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int ii = batch_find(pre, B, b);
+        const batch_inst in = I[ii];
+        const int k = in.k;
+        const int64_t lb = b - pre[ii];
+        const int64_t r0 = lb * BATCH_CNT_ROWS;
+        const int nr = int(min<int64_t>(BATCH_CNT_ROWS, in.n - r0));
+        int lbits = 0;
+        while ((1 << lbits) < k) ++lbits;
+        const int32_t* lab_in = labels + in.row0 + r0;
+        const int wr0 = wave * (BATCH_CNT_ROWS / 4);  // the wave's quarter of the block
+        int32_t lab[G], rk[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int r = wr0 + q * 64 + lane;
+            lab[q] = r < nr ? lab_in[r] : -1;
+        }
+        for (int j = lane; j < k; j += 64) wc[j] = 0;
+        // 1. rank of each row among the earlier rows of the wave's quarter with its label
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const bool ok = lab[q] >= 0 && lab[q] < k;
+            uint64_t same = __ballot(ok);
+            for (int bit = 0; bit < lbits; ++bit) {
+                const bool on = ok && ((lab[q] >> bit) & 1);
+                const uint64_t m = __ballot(on);
+                same &= on ? m : ~m;
+            }
+            const int rank = __popcll(same & below);
+            rk[q] = -1;
+            if (ok) {
+                rk[q] = wc[lab[q]] + rank;
+                if (rank == 0) wc[lab[q]] = rk[q] + __popcll(same);
+            }
+        }
+        __syncthreads();
+        // 2. local slots: labels in order, within a label the waves in order
+        const int m = (k + 255) / 256;
+        const int j0 = min(k, t * m), j1 = min(k, j0 + m);
+        int sum = 0;
+        for (int j = j0; j < j1; ++j) sum += slds[j] + slds[kcap + j] + slds[2 * kcap + j] + slds[3 * kcap + j];
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int excl = incl - sum;
+        for (int w = 0; w < wave; ++w) excl += wsum[w];
+        const int nvalid = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        const int32_t* base = bhist + in.h0 + lb * k;
+        for (int j = j0; j < j1; ++j) {
+            ls[j] = excl;
+            gb[j] = base[j];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int c = slds[w * kcap + j];
+                slds[w * kcap + j] = excl;
+                excl += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+            if (rk[q] >= 0) stage[wc[lab[q]] + rk[q]] = (lab[q] << 16) | (wr0 + q * 64 + lane);
+        __syncthreads();
+        // 3. the block's rows of each label to their destination run, slot order
+        int32_t* out = perm + in.row0;
+        const int32_t id0 = int32_t(in.row0 + r0);
+        for (int p = t; p < nvalid; p += 256) {
+            const int v = stage[p];
+            const int l = v >> 16;
+            out[gb[l] + (p - ls[l])] = id0 + (v & 0xffff);
+        }
+        __syncthreads();
     }
 }
 

@@ -94,10 +94,10 @@ struct dbuf {
 // per-row, per-cluster and per-chunk arrays, sort scratch
 struct batch_ws {
     dbuf inst, pre, labels, keys, perm, d64, ccur, cnext, counts, offsets, cb, empty, meta, qpre,
-        part, objc, obj, sort_tmp, asel, apre;
+        part, objc, obj, sort_tmp, asel, apre, bhist;
     void release() {
         dbuf* all[] = {&inst,  &pre,  &labels, &keys, &perm, &d64, &ccur, &cnext, &counts, &offsets, &cb,
-                       &empty, &meta, &qpre,   &part, &objc, &obj, &sort_tmp, &asel, &apre};
+                       &empty, &meta, &qpre,   &part, &objc, &obj, &sort_tmp, &asel, &apre, &bhist};
         for (dbuf* b : all) b->release();
     }
 };
@@ -1225,7 +1225,7 @@ static void infer_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
 struct batch_plan {
     // prefix tables: assignment batches, histogram chunks, centroid elements,
     // d64 batches, objective chunks
-    enum { P_CNT = 0, P_EL, P_D64, P_OBJ, P_NUM };
+    enum { P_CNT = 0, P_EL, P_D64, P_OBJ, P_K, P_NUM };
     // one assignment launch per tile variant in use: its instances
     // (asel[sel_off ..]) and the prefix of their workgroup counts (apre[pre_off ..])
     struct asg_launch {
@@ -1235,8 +1235,12 @@ struct batch_plan {
     };
     int B = 0;
     bool train = false;
-    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
+    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0, hh = 0;
     unsigned kbits = 1, lbits = 0;
+    int maxk = 0;
+    // rows grouped by label with the counting sort (every k <= BATCH_HIST_MAX), or
+    // with one radix sort of (instance, label) keys
+    bool csort = false;
     std::vector<batch_inst> hi;
     std::vector<int64_t> pre;
     std::vector<int32_t> asel;
@@ -1258,6 +1262,7 @@ struct batch_plan {
     int32_t* cb = nullptr;
     int32_t* empty = nullptr;
     int32_t* meta = nullptr;
+    int32_t* bhist = nullptr;
     int64_t* qpre = nullptr;
     double* part = nullptr;
     double* objc = nullptr;
@@ -1309,7 +1314,7 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
     p.hi.assign(B, batch_inst{});
     p.pre.assign(size_t(batch_plan::P_NUM) * (B + 1), 0);
     auto P = [&](int t, int i) -> int64_t& { return p.pre[size_t(t) * (B + 1) + i]; };
-    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0;
+    int64_t rows = 0, cfl = 0, kk = 0, pp = 0, oo = 0, hh = 0;
     int maxk = 1;
     for (int i = 0; i < B; ++i) {
         const km_batch_instance_f32& q = inst[i];
@@ -1340,16 +1345,19 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
         b.k0 = kk;
         b.p0 = pp;
         b.o0 = oo;
+        b.h0 = hh;
         rows += q.n;
         cfl += (q.k * q.d + 3) / 4 * 4;
         kk += q.k;
         pp += (q.n / REDUCE_CHUNK + q.k + 1) * q.d;
         oo += (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        hh += (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS * q.k;
         maxk = std::max<int>(maxk, int(q.k));
         P(batch_plan::P_CNT, i + 1) = P(batch_plan::P_CNT, i) + (q.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
         P(batch_plan::P_EL, i + 1) = P(batch_plan::P_EL, i) + q.k * q.d;
         P(batch_plan::P_D64, i + 1) = P(batch_plan::P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
         P(batch_plan::P_OBJ, i + 1) = P(batch_plan::P_OBJ, i) + (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        P(batch_plan::P_K, i + 1) = P(batch_plan::P_K, i) + q.k;
     }
     if (rows >= (int64_t(1) << 31))
         throw status_error(KM_UNIMPLEMENTED, "a batch holds at most 2^31 - 1 rows in total");
@@ -1358,6 +1366,9 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
     p.kk = kk;
     p.pp = pp;
     p.oo = oo;
+    p.hh = hh;
+    p.maxk = maxk;
+    p.csort = train && maxk <= BATCH_HIST_MAX;
     // rows are grouped by (instance, label) with one stable radix sort of the keys
     // (index << lbits) | label; instances are contiguous, so each instance's rows
     // come out by label in row order
@@ -1412,7 +1423,7 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
     p.d_apre = ws.apre.get<int64_t>(p.apre.size());
     if (p.train) {
         p.labels = ws.labels.get<int32_t>(p.rows);
-        p.skeys = ws.keys.get<uint32_t>(2 * p.rows);  // keys, then their sorted copy
+        if (!p.csort) p.skeys = ws.keys.get<uint32_t>(2 * p.rows);  // keys, then their sorted copy
         p.perm = ws.perm.get<int32_t>(p.rows);
         p.d64 = ws.d64.get<double>(p.rows);
         p.Cnext = ws.cnext.get<float>(p.cfl);
@@ -1422,12 +1433,19 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
         p.empty = ws.empty.get<int32_t>(p.kk);
         p.meta = ws.meta.get<int32_t>(2 * int64_t(B));
         p.qpre = ws.qpre.get<int64_t>(B + 1);
+        if (p.csort) {
+            p.bhist = ws.bhist.get<int32_t>(p.hh);
+            KM_HIP(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(batch_scatter_lds_bytes(p.maxk))));
+        }
         p.part = ws.part.get<double>(p.pp);
-        p.tmp_bytes = 0;
-        rocprim::counting_iterator<int32_t> ids(0);
-        KM_HIP(rocprim::radix_sort_pairs(nullptr, p.tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
-                                         unsigned(p.rows), 0, p.kbits, ctx->stream));
-        p.tmp = ws.sort_tmp.get<char>(p.tmp_bytes);
+        if (!p.csort) {
+            p.tmp_bytes = 0;
+            rocprim::counting_iterator<int32_t> ids(0);
+            KM_HIP(rocprim::radix_sort_pairs(nullptr, p.tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
+                                             unsigned(p.rows), 0, p.kbits, ctx->stream));
+            p.tmp = ws.sort_tmp.get<char>(p.tmp_bytes);
+        }
     }
     KM_HIP(hipMemcpyAsync(p.dI, p.hi.data(), sizeof(batch_inst) * B, hipMemcpyHostToDevice, ctx->stream));
     KM_HIP(hipMemcpyAsync(p.dpre, p.pre.data(), sizeof(int64_t) * p.pre.size(), hipMemcpyHostToDevice,
@@ -1500,21 +1518,31 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_star
     const int cap = ctx->cu_count * 8;
     auto g = [&](int64_t items) { return batch_grid(ctx, items); };
     if (asg_start) KM_HIP(hipEventRecord(asg_start, ctx->stream));
-    plan_assign(ctx, p, p.labels, p.skeys, 0);
+    plan_assign(ctx, p, p.labels, p.csort ? nullptr : p.skeys, 0);
     if (asg_end) KM_HIP(hipEventRecord(asg_end, ctx->stream));
     fill<int32_t>(ctx, p.counts, p.kk, 0);
     hipLaunchKernelGGL(batch_count_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256), 0, ctx->stream, p.dI, B,
-                       p.DP(BP::P_CNT), p.labels, p.counts);
+                       p.DP(BP::P_CNT), p.labels, p.counts, p.csort ? p.bhist : (int32_t*)nullptr);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, p.dI, B, p.counts,
                        p.offsets, p.cb, p.empty, p.meta);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, p.meta, B, p.qpre);
     KM_LAUNCH_CHECK();
-    rocprim::counting_iterator<int32_t> ids(0);
-    size_t tmp_bytes = p.tmp_bytes;
-    KM_HIP(rocprim::radix_sort_pairs(p.tmp, tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
-                                     unsigned(p.rows), 0, p.kbits, ctx->stream));
+    if (p.csort) {
+        hipLaunchKernelGGL(batch_sortbase_kernel, dim3(g((p.P(BP::P_K, B) + 255) / 256)), dim3(256), 0,
+                           ctx->stream, p.dI, B, p.DP(BP::P_K), p.offsets, p.bhist);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batch_scatter_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256),
+                           batch_scatter_lds_bytes(p.maxk),
+                           ctx->stream, p.dI, B, p.DP(BP::P_CNT), p.labels, p.bhist, p.perm, p.maxk);
+        KM_LAUNCH_CHECK();
+    } else {
+        rocprim::counting_iterator<int32_t> ids(0);
+        size_t tmp_bytes = p.tmp_bytes;
+        KM_HIP(rocprim::radix_sort_pairs(p.tmp, tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
+                                         unsigned(p.rows), 0, p.kbits, ctx->stream));
+    }
     hipLaunchKernelGGL(batch_reduce_vec_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre,
                        p.perm, p.offsets, p.cb, p.part);
     KM_LAUNCH_CHECK();
