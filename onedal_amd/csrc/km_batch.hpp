@@ -541,7 +541,8 @@ __global__ void batch_tune_fill_kernel(float* __restrict__ p, int64_t count, uin
     }
 }
 
-// label histogram: BATCH_CNT_ROWS rows of one instance per workgroup
+// label histogram: BATCH_CNT_ROWS rows of one instance per workgroup, added to
+// counts (when not null) and written per block to bhist (when not null)
 __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __restrict__ I, int B,
                                                           const int64_t* __restrict__ pre,
                                                           const int32_t* __restrict__ labels,
@@ -569,7 +570,7 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
         __syncthreads();
         if (lds)
             for (int j = threadIdx.x; j < in.k; j += 256) {
-                if (hist[j]) atomicAdd(&counts[in.k0 + j], hist[j]);
+                if (counts && hist[j]) atomicAdd(&counts[in.k0 + j], hist[j]);
                 // the block's histogram, for the counting sort (bhist: every k <= BATCH_HIST_MAX)
                 if (bhist) bhist[in.h0 + (b - pre[ii]) * in.k + j] = hist[j];
             }
@@ -581,9 +582,9 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
 // rocPRIM's radix sort of (instance, label) keys gives: rows of a cluster in
 // ascending row order), for batches whose k are all at most BATCH_HIST_MAX.
 // 1. batch_count_kernel writes each histogram block's label counts (bhist).
-// 2. batch_sortbase_kernel turns them into each block's first destination per
-//    label: the label's member offset plus the counts of the instance's earlier
-//    blocks (one thread per (instance, label), walking the blocks in order).
+// 2. batch_scan_kernel sums them into the counts and, after the scan, turns them
+//    into each block's first destination per label: the label's member offset
+//    plus the counts of the instance's earlier blocks.
 // 3. batch_scatter_kernel: one workgroup per block.  Each wave ranks the rows of
 //    its quarter of the block within their label (one ballot per label bit per
 //    64 rows, in row order; a per-wave LDS counter per label carries the runs
@@ -593,27 +594,6 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
 //    one contiguous run.  Labels outside [0, k) are skipped, as the histogram
 //    skips them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void batch_sortbase_kernel(const batch_inst* __restrict__ I, int B,
-                                                             const int64_t* __restrict__ prek,
-                                                             const int64_t* __restrict__ offsets,
-                                                             int32_t* __restrict__ bhist) {
-    const int64_t total = prek[B];
-    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-         e += int64_t(gridDim.x) * blockDim.x) {
-        const int ii = batch_find(prek, B, e);
-        const batch_inst in = I[ii];
-        const int j = int(e - prek[ii]);
-        const int64_t nb = (in.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
-        int32_t base = int32_t(offsets[in.k0 + ii + j]);
-        int32_t* h = bhist + in.h0 + j;
-        for (int64_t lb = 0; lb < nb; ++lb) {
-            const int32_t c = h[lb * in.k];
-            h[lb * in.k] = base;
-            base += c;
-        }
-    }
-}
-
 // LDS bytes of batch_scatter_kernel for labels below kcap
 __host__ __device__ constexpr size_t batch_scatter_lds_bytes(int kcap) {
     return (size_t(6) * kcap + BATCH_CNT_ROWS) * sizeof(int32_t);
@@ -721,26 +701,37 @@ __global__ __launch_bounds__(256) void batch_scatter_kernel(const batch_inst* __
 
 // Per instance (one workgroup each): member offsets, chunk bases and the list of
 // empty clusters from the counts, as scan_clusters_kernel; meta[2i] = #empty,
-// meta[2i+1] = #chunks.
+// meta[2i+1] = #chunks.  With bhist (the counting sort), the counts are first
+// summed from the per-block histograms and written to counts, and after the scan
+// each label's per-block counts become that block's first destination (the
+// label's member offset plus the counts of the instance's earlier blocks).
+// The last workgroup to finish (a ticket counter, reset by that workgroup) writes
+// the exclusive prefix over instances of their chunk counts: qpre[i] = first
+// global chunk of instance i, qpre[B] = total.
 __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __restrict__ I, int B,
-                                                          const int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ bhist,
                                                           int64_t* __restrict__ offsets,
                                                           int32_t* __restrict__ chunk_base,
                                                           int32_t* __restrict__ empty_list,
-                                                          int32_t* __restrict__ meta) {
+                                                          int32_t* __restrict__ meta,
+                                                          int64_t* __restrict__ qpre,
+                                                          uint32_t* __restrict__ ticket) {
     __shared__ int64_t s_off[1024];
     __shared__ int32_t s_chk[1024];
     __shared__ int32_t s_emp[1024];
     __shared__ int64_t carry_off;
     __shared__ int32_t carry_chk, carry_emp;
+    __shared__ bool s_last;
     const int t = threadIdx.x;
     for (int ii = blockIdx.x; ii < B; ii += gridDim.x) {
         const batch_inst in = I[ii];
         const int k = in.k;
-        const int32_t* cnt = counts + in.k0;
+        int32_t* cnt = counts + in.k0;
         int64_t* off = offsets + in.k0 + ii;
         int32_t* cb = chunk_base + in.k0 + ii;
         int32_t* emp = empty_list + in.k0;
+        const int64_t nb = (in.n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
         __syncthreads();
         if (t == 0) {
             carry_off = 0;
@@ -750,7 +741,22 @@ __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __re
         __syncthreads();
         for (int base = 0; base < k; base += 1024) {
             const int j = base + t;
-            const int32_t c = j < k ? cnt[j] : 0;
+            int32_t c = 0;
+            if (j < k) {
+                if (bhist) {
+                    const int32_t* h = bhist + in.h0 + j;
+                    int64_t lb = 0;
+                    for (; lb + 4 <= nb; lb += 4) {
+                        const int32_t h0 = h[lb * k], h1 = h[(lb + 1) * k], h2 = h[(lb + 2) * k],
+                                      h3 = h[(lb + 3) * k];
+                        c += h0 + h1 + h2 + h3;
+                    }
+                    for (; lb < nb; ++lb) c += h[lb * k];
+                    cnt[j] = c;
+                } else {
+                    c = cnt[j];
+                }
+            }
             s_off[t] = c;
             s_chk[t] = j < k ? (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK : 0;
             s_emp[t] = (j < k && c == 0) ? 1 : 0;
@@ -770,9 +776,19 @@ __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __re
                 __syncthreads();
             }
             if (j < k) {
-                off[j] = carry_off + s_off[t] - c;
+                const int64_t first = carry_off + s_off[t] - c;
+                off[j] = first;
                 cb[j] = carry_chk + s_chk[t] - (c + REDUCE_CHUNK - 1) / REDUCE_CHUNK;
                 if (c == 0) emp[carry_emp + s_emp[t] - 1] = j;
+                if (bhist) {
+                    int32_t* h = bhist + in.h0 + j;
+                    int32_t dst = int32_t(first);
+                    for (int64_t lb = 0; lb < nb; ++lb) {
+                        const int32_t hc = h[lb * k];
+                        h[lb * k] = dst;
+                        dst += hc;
+                    }
+                }
             }
             __syncthreads();
             if (t == 1023) {
@@ -789,34 +805,35 @@ __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __re
             meta[2 * ii + 1] = carry_chk;
         }
     }
-}
-
-// exclusive prefix over instances of their chunk counts (one workgroup):
-// qpre[i] = first global chunk of instance i, qpre[B] = total
-__global__ __launch_bounds__(1024) void batch_chunk_prefix_kernel(const int32_t* __restrict__ meta,
-                                                                  int B, int64_t* __restrict__ qpre) {
-    __shared__ int64_t s[1024];
-    __shared__ int64_t carry;
-    const int t = threadIdx.x;
-    if (t == 0) carry = 0;
+    // the last workgroup to finish: chunk prefix over the instances
     __syncthreads();
+    if (t == 0) {
+        __threadfence();  // this workgroup's meta before its ticket
+        s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // every workgroup's meta after the last ticket
+    int64_t carry = 0;
     for (int base = 0; base < B; base += 1024) {
         const int i = base + t;
         const int64_t v = i < B ? int64_t(meta[2 * i + 1]) : 0;
-        s[t] = v;
+        s_off[t] = v;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
-            const int64_t a = t >= o ? s[t - o] : 0;
+            const int64_t a = t >= o ? s_off[t - o] : 0;
             __syncthreads();
-            s[t] += a;
+            s_off[t] += a;
             __syncthreads();
         }
-        if (i < B) qpre[i] = carry + s[t] - v;
-        __syncthreads();
-        if (t == 1023) carry += s[1023];
+        if (i < B) qpre[i] = carry + s_off[t] - v;
+        carry += s_off[1023];
         __syncthreads();
     }
-    if (t == 0) qpre[B] = carry;
+    if (t == 0) {
+        qpre[B] = carry;
+        *ticket = 0u;  // ready for the next launch
+    }
 }
 
 // Chunk sums of the instances whose rows allow 16-byte reads (d % 4 == 0,
@@ -826,8 +843,8 @@ __global__ __launch_bounds__(1024) void batch_chunk_prefix_kernel(const int32_t*
 // reduce_groups(d) interleaved partials of the scalar kernel, G * d / 4 <= 16
 // lanes) or of every member (d > 32: d / 4 lanes), so each feature is summed over
 // the same members in the same order as batch_reduce_kernel's and the partials
-// are identical.  Waves take runs of RUN consecutive chunks, round robin; member
-// ids are loaded one step ahead of the row reads.
+// are identical.  Waves take runs of consecutive chunks (up to RUN), round
+// robin; member ids are loaded one step ahead of the row reads.
 __global__ __launch_bounds__(256) void batch_reduce_vec_kernel(const batch_inst* __restrict__ I, int B,
                                                                const int64_t* __restrict__ qpre,
                                                                const int32_t* __restrict__ perm,
@@ -841,8 +858,11 @@ __global__ __launch_bounds__(256) void batch_reduce_vec_kernel(const batch_inst*
     const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
     const int64_t total = qpre[B];
-    for (int64_t q0 = gw * RUN; q0 < total; q0 += nw * RUN) {
-        const int64_t q1 = min<int64_t>(q0 + RUN, total);
+    // runs of RUN chunks when there is work for every wave, shorter runs (down to
+    // one chunk per wave) when there is not
+    const int64_t run = max<int64_t>(1, min<int64_t>(RUN, total / nw));
+    for (int64_t q0 = gw * run; q0 < total; q0 += nw * run) {
+        const int64_t q1 = min<int64_t>(q0 + run, total);
         int ii = batch_find(qpre, B, q0);
         int64_t q = q0;
         while (q < q1) {
@@ -946,11 +966,13 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const batch_inst* __r
     const int64_t gw = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = (int64_t(gridDim.x) * blockDim.x) >> 6;
     const int64_t total = qpre[B];
-    // waves take runs of RUN consecutive chunks, round robin: the owning instance
-    // and cluster are searched once per run and then advanced chunk by chunk
+    // waves take runs of consecutive chunks, round robin: the owning instance
+    // and cluster are searched once per run and then advanced chunk by chunk;
+    // runs are RUN chunks long when there is work for every wave, shorter when not
     constexpr int RUN = 16;
-    for (int64_t q0 = gw * RUN; q0 < total; q0 += nw * RUN) {
-        const int64_t q1 = min<int64_t>(q0 + RUN, total);
+    const int64_t run = max<int64_t>(1, min<int64_t>(RUN, total / nw));
+    for (int64_t q0 = gw * run; q0 < total; q0 += nw * run) {
+        const int64_t q1 = min<int64_t>(q0 + run, total);
         int ii = batch_find(qpre, B, q0);
         int j = -1;
         for (int64_t q = q0; q < q1; ++q) {
@@ -1141,8 +1163,9 @@ __device__ __forceinline__ double batch_block_d64(const batch_inst& in, int64_t 
     return acc;
 }
 
-// d64 of every row of the instances that have empty clusters (meta[2i] > 0);
-// BATCH_D64_ROWS rows per workgroup
+// d64 of every row of the instances that have empty clusters (meta[2i] > 0), or
+// of every instance when meta is null; labels: concatenated (row0), or each
+// instance's labels_out when labels is null; BATCH_D64_ROWS rows per workgroup
 __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __restrict__ I, int B,
                                                         const int64_t* __restrict__ pre,
                                                         const int32_t* __restrict__ meta,
@@ -1154,11 +1177,12 @@ __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __rest
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
         const int ii = batch_find(pre, B, b);
-        if (meta[2 * ii] == 0) continue;  // uniform over the workgroup
+        if (meta && meta[2 * ii] == 0) continue;  // uniform over the workgroup
         const batch_inst in = I[ii];
         const int64_t r0 = (b - pre[ii]) * BATCH_D64_ROWS;
         const int nr = int(min<int64_t>(BATCH_D64_ROWS, in.n - r0));
-        const double v = batch_block_d64(in, r0, nr, labels + in.row0, C + in.c0, Xs);
+        const int32_t* lab = labels ? labels + in.row0 : in.labels_out;
+        const double v = batch_block_d64(in, r0, nr, lab, C + in.c0, Xs);
         if (int(threadIdx.x) < nr) d64[in.row0 + r0 + threadIdx.x] = v;
     }
 }
@@ -1166,11 +1190,13 @@ __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __rest
 // Empty clusters of one instance per workgroup: pick e = 0 .. m-1 takes the row
 // with the largest d64 (ties: lowest row) after pick e-1 in that order, and the
 // e-th empty cluster (ascending index) receives that row -- the policy of
-// empty_cluster_handling_dpc.cpp:26-114 with its tie order pinned.
+// empty_cluster_handling_dpc.cpp:26-114 with its tie order pinned.  An empty
+// cluster left without a row (every remaining d64 is NaN) keeps its centroid.
 __global__ __launch_bounds__(256) void batch_fill_empty_kernel(const batch_inst* __restrict__ I, int B,
                                                                const int32_t* __restrict__ meta,
                                                                const int32_t* __restrict__ empty_list,
                                                                const double* __restrict__ d64,
+                                                               const float* __restrict__ Ccur,
                                                                float* __restrict__ Cnext) {
     __shared__ double sd[256];
     __shared__ int64_t sr[256];
@@ -1210,26 +1236,24 @@ __global__ __launch_bounds__(256) void batch_fill_empty_kernel(const batch_inst*
             const double wd = sd[0];
             const int64_t wr = sr[0];
             __syncthreads();
-            if (wr >= 0) {
-                const int j = empty_list[in.k0 + e];
-                for (int f = t; f < in.d; f += 256)
-                    Cnext[in.c0 + int64_t(j) * in.d + f] = in.X[wr * in.d + f];
-            }
+            const int j = empty_list[in.k0 + e];
+            for (int f = t; f < in.d; f += 256)
+                Cnext[in.c0 + int64_t(j) * in.d + f] =
+                    wr >= 0 ? in.X[wr * in.d + f] : Ccur[in.c0 + int64_t(j) * in.d + f];
             pd = wd;
             pr = wr;
         }
     }
 }
 
-// Objective chunk sums: OBJ_CHUNK rows of one instance per workgroup, in the
-// order of objective_chunks_kernel (thread t sums rows t, t+256, ...; tree of
-// the 256 partials).
+// Objective chunk sums from the rows' d64 (batch_d64_kernel): OBJ_CHUNK rows of
+// one instance per workgroup, in the order of objective_chunks_kernel (thread t
+// sums rows t, t+256, ...; tree of the 256 partials).
 __global__ __launch_bounds__(256) void batch_objective_kernel(const batch_inst* __restrict__ I, int B,
                                                               const int64_t* __restrict__ pre,
-                                                              const float* __restrict__ C,
+                                                              const double* __restrict__ d64,
                                                               double* __restrict__ chunk_sums) {
     __shared__ double part[256];
-    __shared__ batch_d64_tile Xs;
     const int t = threadIdx.x;
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -1238,12 +1262,10 @@ __global__ __launch_bounds__(256) void batch_objective_kernel(const batch_inst* 
         const int64_t lq = b - pre[ii];
         const int64_t r0 = lq * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, in.n - r0));
+        const double* dv = d64 + in.row0 + r0;
         double p = 0.0;
-        for (int s0 = 0; s0 < rows; s0 += 256) {
-            const int nr = min(256, rows - s0);
-            const double v = batch_block_d64(in, r0 + s0, nr, in.labels_out, C + in.c0, Xs);
-            if (t < nr) p = p + v;
-        }
+        for (int s0 = 0; s0 < rows; s0 += 256)
+            if (s0 + t < rows) p = p + dv[s0 + t];
         __syncthreads();
         part[t] = p;
         __syncthreads();

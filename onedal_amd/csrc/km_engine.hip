@@ -94,10 +94,10 @@ struct dbuf {
 // per-row, per-cluster and per-chunk arrays, sort scratch
 struct batch_ws {
     dbuf inst, pre, labels, keys, perm, d64, ccur, cnext, counts, offsets, cb, empty, meta, qpre,
-        part, objc, obj, sort_tmp, asel, apre, bhist;
+        part, objc, obj, sort_tmp, asel, apre, bhist, ticket;
     void release() {
-        dbuf* all[] = {&inst,  &pre,  &labels, &keys, &perm, &d64, &ccur, &cnext, &counts, &offsets, &cb,
-                       &empty, &meta, &qpre,   &part, &objc, &obj, &sort_tmp, &asel, &apre, &bhist};
+        dbuf* all[] = {&inst, &pre,  &labels, &keys, &perm,     &d64,  &ccur, &cnext, &counts, &offsets, &cb,
+                       &empty, &meta, &qpre,  &part, &objc, &obj, &sort_tmp, &asel, &apre, &bhist, &ticket};
         for (dbuf* b : all) b->release();
     }
 };
@@ -1225,7 +1225,7 @@ static void infer_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
 struct batch_plan {
     // prefix tables: assignment batches, histogram chunks, centroid elements,
     // d64 batches, objective chunks
-    enum { P_CNT = 0, P_EL, P_D64, P_OBJ, P_K, P_NUM };
+    enum { P_CNT = 0, P_EL, P_D64, P_OBJ, P_NUM };
     // one assignment launch per tile variant in use: its instances
     // (asel[sel_off ..]) and the prefix of their workgroup counts (apre[pre_off ..])
     struct asg_launch {
@@ -1241,6 +1241,8 @@ struct batch_plan {
     // rows grouped by label with the counting sort (every k <= BATCH_HIST_MAX), or
     // with one radix sort of (instance, label) keys
     bool csort = false;
+    // instances summed by batch_reduce_vec_kernel / batch_reduce_kernel
+    bool any_vec = false, any_scalar = false;
     std::vector<batch_inst> hi;
     std::vector<int64_t> pre;
     std::vector<int32_t> asel;
@@ -1263,6 +1265,7 @@ struct batch_plan {
     int32_t* empty = nullptr;
     int32_t* meta = nullptr;
     int32_t* bhist = nullptr;
+    uint32_t* ticket = nullptr;
     int64_t* qpre = nullptr;
     double* part = nullptr;
     double* objc = nullptr;
@@ -1340,6 +1343,10 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
         b.k = int32_t(q.k);
         b.index = i;
         b.vec = (q.d % 4 == 0 && reinterpret_cast<uintptr_t>(q.X) % 16 == 0) ? 1 : 0;
+        if (b.vec && q.d <= 256)
+            p.any_vec = true;
+        else
+            p.any_scalar = true;
         b.row0 = rows;
         b.c0 = cfl;
         b.k0 = kk;
@@ -1357,7 +1364,6 @@ static void plan_build(batch_plan& p, const km_ctx* ctx, const km_batch_instance
         P(batch_plan::P_EL, i + 1) = P(batch_plan::P_EL, i) + q.k * q.d;
         P(batch_plan::P_D64, i + 1) = P(batch_plan::P_D64, i) + (q.n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
         P(batch_plan::P_OBJ, i + 1) = P(batch_plan::P_OBJ, i) + (q.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
-        P(batch_plan::P_K, i + 1) = P(batch_plan::P_K, i) + q.k;
     }
     if (rows >= (int64_t(1) << 31))
         throw status_error(KM_UNIMPLEMENTED, "a batch holds at most 2^31 - 1 rows in total");
@@ -1421,11 +1427,11 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
     p.objd = ws.obj.get<double>(B);
     p.d_asel = ws.asel.get<int32_t>(p.asel.size());
     p.d_apre = ws.apre.get<int64_t>(p.apre.size());
+    p.d64 = ws.d64.get<double>(p.rows);  // the final pass's objective, and empty clusters
     if (p.train) {
         p.labels = ws.labels.get<int32_t>(p.rows);
         if (!p.csort) p.skeys = ws.keys.get<uint32_t>(2 * p.rows);  // keys, then their sorted copy
         p.perm = ws.perm.get<int32_t>(p.rows);
-        p.d64 = ws.d64.get<double>(p.rows);
         p.Cnext = ws.cnext.get<float>(p.cfl);
         p.counts = ws.counts.get<int32_t>(p.kk);
         p.offsets = ws.offsets.get<int64_t>(p.kk + B);
@@ -1433,6 +1439,8 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
         p.empty = ws.empty.get<int32_t>(p.kk);
         p.meta = ws.meta.get<int32_t>(2 * int64_t(B));
         p.qpre = ws.qpre.get<int64_t>(B + 1);
+        p.ticket = ws.ticket.get<uint32_t>(1);
+        fill<uint32_t>(ctx, p.ticket, 1, 0u);
         if (p.csort) {
             p.bhist = ws.bhist.get<int32_t>(p.hh);
             KM_HIP(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1511,7 +1519,7 @@ static void plan_copy_centroids(km_ctx* ctx, const batch_plan& p, int dir) {
 
 // one Lloyd iteration of every instance of a training plan; with events, the
 // assignment stage is bracketed by them (eager runs only)
-static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_start = nullptr,
+static void plan_iteration(km_ctx* ctx, batch_plan& p, hipEvent_t asg_start = nullptr,
                            hipEvent_t asg_end = nullptr) {
     using BP = batch_plan;
     const int B = p.B;
@@ -1520,19 +1528,17 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_star
     if (asg_start) KM_HIP(hipEventRecord(asg_start, ctx->stream));
     plan_assign(ctx, p, p.labels, p.csort ? nullptr : p.skeys, 0);
     if (asg_end) KM_HIP(hipEventRecord(asg_end, ctx->stream));
-    fill<int32_t>(ctx, p.counts, p.kk, 0);
+    if (!p.csort) fill<int32_t>(ctx, p.counts, p.kk, 0);
+    // counting sort: per-block histograms only (the scan sums them into the counts)
     hipLaunchKernelGGL(batch_count_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256), 0, ctx->stream, p.dI, B,
-                       p.DP(BP::P_CNT), p.labels, p.counts, p.csort ? p.bhist : (int32_t*)nullptr);
+                       p.DP(BP::P_CNT), p.labels, p.csort ? (int32_t*)nullptr : p.counts,
+                       p.csort ? p.bhist : (int32_t*)nullptr);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, p.dI, B, p.counts,
-                       p.offsets, p.cb, p.empty, p.meta);
-    KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(batch_chunk_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, p.meta, B, p.qpre);
+                       p.csort ? p.bhist : (int32_t*)nullptr, p.offsets, p.cb, p.empty, p.meta, p.qpre,
+                       p.ticket);
     KM_LAUNCH_CHECK();
     if (p.csort) {
-        hipLaunchKernelGGL(batch_sortbase_kernel, dim3(g((p.P(BP::P_K, B) + 255) / 256)), dim3(256), 0,
-                           ctx->stream, p.dI, B, p.DP(BP::P_K), p.offsets, p.bhist);
-        KM_LAUNCH_CHECK();
         hipLaunchKernelGGL(batch_scatter_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256),
                            batch_scatter_lds_bytes(p.maxk),
                            ctx->stream, p.dI, B, p.DP(BP::P_CNT), p.labels, p.bhist, p.perm, p.maxk);
@@ -1543,12 +1549,16 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_star
         KM_HIP(rocprim::radix_sort_pairs(p.tmp, tmp_bytes, p.skeys, p.skeys + p.rows, ids, p.perm,
                                          unsigned(p.rows), 0, p.kbits, ctx->stream));
     }
-    hipLaunchKernelGGL(batch_reduce_vec_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre,
-                       p.perm, p.offsets, p.cb, p.part);
-    KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre, p.perm,
-                       p.offsets, p.cb, p.part);
-    KM_LAUNCH_CHECK();
+    if (p.any_vec) {
+        hipLaunchKernelGGL(batch_reduce_vec_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre,
+                           p.perm, p.offsets, p.cb, p.part);
+        KM_LAUNCH_CHECK();
+    }
+    if (p.any_scalar) {
+        hipLaunchKernelGGL(batch_reduce_kernel, dim3(cap), dim3(256), 0, ctx->stream, p.dI, B, p.qpre,
+                           p.perm, p.offsets, p.cb, p.part);
+        KM_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(g((p.P(BP::P_EL, B) + 255) / 256)), dim3(256), 0,
                        ctx->stream, p.dI, B, p.DP(BP::P_EL), p.counts, p.cb, p.part, p.Cnext);
     KM_LAUNCH_CHECK();
@@ -1556,11 +1566,12 @@ static void plan_iteration(km_ctx* ctx, const batch_plan& p, hipEvent_t asg_star
                        p.DP(BP::P_D64), p.meta, p.Ccur, p.labels, p.d64);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_fill_empty_kernel, dim3(g(B)), dim3(256), 0, ctx->stream, p.dI, B, p.meta,
-                       p.empty, p.d64, p.Cnext);
+                       p.empty, p.d64, p.Ccur, p.Cnext);
     KM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(copy_kernel<float>, dim3(g((p.cfl + 255) / 256)), dim3(256), 0, ctx->stream,
-                       p.Cnext, p.Ccur, p.cfl);
-    KM_LAUNCH_CHECK();
+    // every centroid of Cnext is written (finalize: counted clusters; fill:
+    // empty ones): the next iteration reads it, and this one's input becomes
+    // the next output buffer
+    std::swap(p.Ccur, p.Cnext);
 }
 
 // final pass: labels into every instance's labels_out, objectives into p.objd,
@@ -1570,8 +1581,12 @@ static void plan_finish(km_ctx* ctx, const batch_plan& p) {
     const int B = p.B;
     auto g = [&](int64_t items) { return batch_grid(ctx, items); };
     plan_assign(ctx, p, nullptr, nullptr, 1);
+    // every row's d64 (against its final label), then the chunk sums in order
+    hipLaunchKernelGGL(batch_d64_kernel, dim3(g(p.P(BP::P_D64, B))), dim3(256), 0, ctx->stream, p.dI, B,
+                       p.DP(BP::P_D64), (const int32_t*)nullptr, p.Ccur, (const int32_t*)nullptr, p.d64);
+    KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_objective_kernel, dim3(g(p.P(BP::P_OBJ, B))), dim3(256), 0, ctx->stream, p.dI,
-                       B, p.DP(BP::P_OBJ), p.Ccur, p.objc);
+                       B, p.DP(BP::P_OBJ), p.d64, p.objc);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_objective_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, p.dI,
                        B, p.objc, p.objd);

@@ -168,3 +168,30 @@ def test_batch_matrix_core_tile_exact_on_ties(engine, oracle):
         assert np.array_equal(l_out[i], wl), i
         assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), i
         assert obj[i] == wo, i
+
+
+@pytest.mark.parametrize("scale", [1e-12, 1.0, 1e17], ids=["tiny", "unit", "beyond_norm_range"])
+def test_batch_matrix_core_tile_magnitudes(engine, oracle, scale):
+    """The certified matrix-core variant at small, unit and large magnitudes (rows
+    and centroids around 1e-12, 1 and 1e17 per feature, so squared norms from 1e-22
+    to 1e38) equals the canonical result."""
+    from onedal_amd.engine import KM_OPT_BATCH_TILE
+
+    rng = np.random.default_rng(43)
+    inst = []
+    for i, (n, d, k) in enumerate([(3000, 16, 20), (2000, 64, 50), (2500, 128, 100), (1500, 32, 7),
+                                   (2000, 100, 64)]):
+        x = (gaussian_mixture(n, d, max(1, k // 2), seed=4300 + i, center_range=10.0,
+                              sigma=(0.5, 2.0)) * np.float32(scale)).astype(np.float32)
+        c0 = x[rng.choice(n, size=k, replace=False)].copy()
+        inst.append((x, c0))
+    try:
+        engine.set_option(KM_OPT_BATCH_TILE, 8)  # variant 7
+        c_out, l_out, obj, its = run_batch(engine, inst, 3, graph=False)
+    finally:
+        engine.set_option(KM_OPT_BATCH_TILE, 0)
+    for i, (x, c0) in enumerate(inst):
+        wc, wl, wo = oracle.canonical_lloyd(x, c0, 3, 0.0)[:3]
+        assert np.array_equal(l_out[i], wl), i
+        assert np.array_equal(c_out[i].view(np.uint8), wc.view(np.uint8)), i
+        assert obj[i] == wo, i
