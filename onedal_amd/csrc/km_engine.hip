@@ -434,14 +434,13 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
     KM_LAUNCH_CHECK();
 }
 
-// objective chunk sums (objective_chunks_kernel, staged through dynamic LDS)
+// objective chunk sums (objective_chunks_kernel, rows staged through dynamic LDS)
 template <typename F>
 static void launch_objective(km_ctx* ctx, int grid, const F* X, int64_t n, int d, const F* C,
                              const int32_t* labels, double* d64_out, double* chunks) {
     const size_t lds = objective_lds_bytes<F>();
-    auto kern = objective_chunks_kernel<F>;
-    KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, X, n, d, C, labels, d64_out, chunks);
+    hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), lds, ctx->stream, X, n, d, C, labels,
+                       d64_out, chunks);
     KM_LAUNCH_CHECK();
 }
 

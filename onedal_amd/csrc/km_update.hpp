@@ -393,10 +393,13 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
 // order; the 256 partials are combined by the tree p[t] += p[t+s], s = 128..1;
 // chunk sums are added in chunk order by sum_sequential_kernel.
 // ---------------------------------------------------------------------------
-// Rows reach the threads through LDS: for every 256-row group of a chunk, the
-// rows' features and their centroids' features are staged one 128-byte slice
-// (OBJ_SLICE<F> elements) at a time with contiguous loads, and thread t adds its
-// row's squared differences slice after slice, so d64 keeps the feature order.
+// Rows reach the threads through LDS, one wave at a time: wave w of a 256-row
+// group owns rows w*64 .. w*64+63 (thread t = row t of the group); per 128-byte
+// slice of the features (OBJ_SLICE<F> elements) the wave stages its 64 rows with
+// contiguous loads into its own LDS rows, and each lane adds its row's squared
+// differences slice after slice, reading its centroid's slice directly (16-byte
+// loads; the centroids stay in cache), so d64 keeps the feature order.  The next
+// slice's row loads are issued before the current slice is summed.
 template <typename F>
 struct obj_slice {
     static constexpr int FT = 128 / int(sizeof(F));  // elements per slice
@@ -404,7 +407,7 @@ struct obj_slice {
 };
 template <typename F>
 __host__ inline size_t objective_lds_bytes() {
-    return size_t(2) * 256 * obj_slice<F>::ST * sizeof(F);
+    return size_t(256) * obj_slice<F>::ST * sizeof(F);
 }
 
 template <typename F>
@@ -414,12 +417,12 @@ __global__ __launch_bounds__(256) void objective_chunks_kernel(
     double* __restrict__ chunk_sums) {
     constexpr int FT = obj_slice<F>::FT, ST = obj_slice<F>::ST;
     constexpr int V = 16 / int(sizeof(F));            // elements per 16-byte load
+    constexpr int PPR = FT / V;                       // 16-byte parts per row slice
     __shared__ double part[256];
-    __shared__ int32_t slab[256];
     extern __shared__ __align__(16) unsigned char obj_smem[];
-    F* xs = reinterpret_cast<F*>(obj_smem);         // [256][ST]
-    F* cs = xs + 256 * ST;                          // [256][ST]
     const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    F* xs = reinterpret_cast<F*>(obj_smem) + wave * 64 * ST;  // this wave's [64][ST]
     const bool vec = (d % V) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0 &&
                      (reinterpret_cast<uintptr_t>(C) % 16) == 0;
     typedef F FV __attribute__((ext_vector_type(V)));
@@ -431,49 +434,66 @@ __global__ __launch_bounds__(256) void objective_chunks_kernel(
         // thread t owns chunk rows t, t+256, ... in increasing order
         for (int g0 = 0; g0 < rows; g0 += 256) {
             const int nr = min(256, rows - g0);
-            const int64_t gb = r0 + g0;
-            __syncthreads();  // the previous group's readers are done with slab, xs, cs
-            slab[t] = t < nr ? labels[gb + t] : 0;
-            __syncthreads();
+            const int64_t wb = r0 + g0 + wave * 64;        // the wave's first row
+            const int wn = max(0, min(64, nr - wave * 64));  // its rows in this group
+            const bool mine = lane < wn;
+            const F* c = C + int64_t(mine ? labels[wb + lane] : 0) * d;
             double acc = 0.0;
-            for (int f0 = 0; f0 < d; f0 += FT) {
-                const int ft = min(FT, d - f0);
-                if (vec) {
-                    // V-element vectors: (row, part) = (e / (FT / V), e % (FT / V))
-                    constexpr int PPR = FT / V;
-                    for (int e = t; e < nr * PPR; e += 256) {
-                        const int r = e / PPR, pp = e - r * PPR;
-                        const int f = pp * V;
-                        FV xv = {}, cv = {};
-                        if (f < ft) {
-                            xv = *reinterpret_cast<const FV*>(X + (gb + r) * d + f0 + f);
-                            cv = *reinterpret_cast<const FV*>(C + int64_t(slab[r]) * d + f0 + f);
-                        }
+            if (vec) {
+                // the wave's row slice f0: parts e = lane + 64 u, (row, part) = (e / PPR, e % PPR)
+                FV xv[PPR];
+                auto load = [&](int f0) {
 #pragma unroll
-                        for (int u = 0; u < V; ++u) {
-                            xs[r * ST + f + u] = xv[u];
-                            cs[r * ST + f + u] = cv[u];
+                    for (int u = 0; u < PPR; ++u) {
+                        const int e = lane + 64 * u, r = e / PPR, f = (e - r * PPR) * V;
+                        xv[u] = (r < wn && f0 + f < d) ? *reinterpret_cast<const FV*>(X + (wb + r) * d + f0 + f)
+                                                       : FV{};
+                    }
+                };
+                load(0);
+                for (int f0 = 0; f0 < d; f0 += FT) {
+                    const int ft = min(FT, d - f0);
+                    __builtin_amdgcn_wave_barrier();  // the previous slice's readers are done
+#pragma unroll
+                    for (int u = 0; u < PPR; ++u) {
+                        const int e = lane + 64 * u, r = e / PPR, f = (e - r * PPR) * V;
+#pragma unroll
+                        for (int v = 0; v < V; ++v) xs[r * ST + f + v] = xv[u][v];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (f0 + FT < d) load(f0 + FT);
+                    if (mine) {
+                        for (int f = 0; f < ft; f += V) {
+                            const FV cv = *reinterpret_cast<const FV*>(c + f0 + f);
+#pragma unroll
+                            for (int v = 0; v < V; ++v) {
+                                const double diff = double(xs[lane * ST + f + v]) - double(cv[v]);
+                                const double sq = diff * diff;
+                                acc = acc + sq;
+                            }
                         }
                     }
-                } else {
-                    for (int e = t; e < nr * ft; e += 256) {
+                }
+            } else {
+                for (int f0 = 0; f0 < d; f0 += FT) {
+                    const int ft = min(FT, d - f0);
+                    __builtin_amdgcn_wave_barrier();
+                    for (int e = lane; e < wn * ft; e += 64) {
                         const int r = e / ft, f = e - r * ft;
-                        xs[r * ST + f] = X[(gb + r) * d + f0 + f];
-                        cs[r * ST + f] = C[int64_t(slab[r]) * d + f0 + f];
+                        xs[r * ST + f] = X[(wb + r) * d + f0 + f];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (mine) {
+                        for (int f = 0; f < ft; ++f) {
+                            const double diff = double(xs[lane * ST + f]) - double(c[f0 + f]);
+                            const double sq = diff * diff;
+                            acc = acc + sq;
+                        }
                     }
                 }
-                __syncthreads();
-                if (t < nr) {
-                    for (int f = 0; f < ft; ++f) {
-                        const double diff = double(xs[t * ST + f]) - double(cs[t * ST + f]);
-                        const double sq = diff * diff;
-                        acc = acc + sq;
-                    }
-                }
-                __syncthreads();
             }
-            if (t < nr) {
-                if (d64_out) d64_out[gb + t] = acc;
+            if (mine) {
+                if (d64_out) d64_out[wb + lane] = acc;
                 p = p + acc;
             }
         }
