@@ -135,6 +135,7 @@ struct km_ctx {
     km::dbuf tune_x, tune_c, tune_l;  // autotuner inputs (released after tuning)
     km::dbuf init_d2, init_chunks, init_chosen, init_ids;  // initialisation
     km::dbuf gt_d;  // empty-cluster candidates' d64
+    km::dbuf stat_acc, coop_wcnt;  // device-side statistics; per-workgroup counts of empty_clusters_kernel
     // CSR path: values, column indices, row offsets (when uploaded), row squares,
     // transposed centroids, centroid squares, closest distances, structure flag
     km::dbuf csr_vals, csr_cols, csr_offs, csr_xs, csr_ct, csr_cs, csr_closest, csr_flag;
@@ -437,10 +438,11 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
 // objective chunk sums (objective_chunks_kernel, rows staged through dynamic LDS)
 template <typename F>
 static void launch_objective(km_ctx* ctx, int grid, const F* X, int64_t n, int d, const F* C,
-                             const int32_t* labels, double* d64_out, double* chunks) {
+                             const int32_t* labels, double* d64_out, double* chunks,
+                             const int32_t* gate = nullptr) {
     const size_t lds = objective_lds_bytes<F>();
     hipLaunchKernelGGL(objective_chunks_kernel<F>, dim3(grid), dim3(256), lds, ctx->stream, X, n, d, C, labels,
-                       d64_out, chunks);
+                       d64_out, chunks, gate);
     KM_LAUNCH_CHECK();
 }
 
@@ -600,26 +602,40 @@ static int32_t* select_far_rows(km_ctx* ctx, const double* d64, int64_t n, int m
     return cand_dev;
 }
 
-// Refill empty clusters: the m rows with the largest d64 (ties: lowest row) go,
-// in that order, to the empty clusters in ascending index order; the objective
-// drops by their d64, subtracted in that order.
+// Empty clusters of one training iteration, on the device: the rows' d64 (the
+// objective pass, gated on meta[0] > 0 unless objp already ran with d64 out),
+// then empty_clusters_kernel, one cooperative launch that ends at once when
+// meta[0] = 0: the m rows with the largest d64 (ties: lowest row) go, in that
+// order, to the empty clusters in ascending index order; the objective at objp
+// (when not null) drops by their d64, subtracted in that order.  The launch also
+// adds the iteration's re-score counters into acc.
 template <typename F>
-static void handle_empty(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, int m,
-                         int32_t* empty_list, double* obj_host, bool have_obj) {
-    double* d64 = ctx->d64.get<double>(w.n);
-    double* objp = objective<F>(ctx, w, Ccur, d64);
-    std::vector<double> cand_d;
-    int32_t* cand_dev = select_far_rows(ctx, d64, w.n, m, cand_d);
-    const int fgrid = grid_for(ctx, int64_t(m) * w.d, 256, 4);
-    hipLaunchKernelGGL(fill_empty_kernel<F>, dim3(fgrid), dim3(256), 0, ctx->stream, w.X, w.d,
-                       empty_list, cand_dev, m, Cnext);
-    KM_LAUNCH_CHECK();
-    if (!have_obj) {
-        KM_HIP(hipMemcpyAsync(obj_host, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+static void empty_clusters(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, const int32_t* empty_list,
+                           const int32_t* meta, double* objp, double* d64, unsigned long long* acc) {
+    if (!objp) {
+        const int64_t nchunks = (w.n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        const int ogrid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
+        launch_objective<F>(ctx, ogrid, w.X, w.n, w.d, Ccur, w.labels, d64,
+                            ctx->obj_chunks.get<double>(nchunks), meta);
     }
-    KM_HIP(hipStreamSynchronize(ctx->stream));
-    for (int e = 0; e < m; ++e) *obj_host = *obj_host - cand_d[e];
-    ctx->stats.empty_cluster_events++;
+    auto kern = empty_clusters_kernel<F>;
+    int per_cu = 0;
+    KM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 256, 0));
+    const int grid = std::max(1, std::min(per_cu, 4)) * ctx->cu_count;
+    unsigned int* hist = ctx->hist.get<unsigned int>(8 * 256);
+    int32_t* wcnt = ctx->coop_wcnt.get<int32_t>(2 * int64_t(grid));
+    int32_t* gt = ctx->cand_gt.get<int32_t>(w.k);
+    int32_t* eq = ctx->cand_eq.get<int32_t>(w.k);
+    int32_t* cand = ctx->cand_dev.get<int32_t>(w.k);
+    double* cand_d = ctx->gt_d.get<double>(w.k);
+    const F* X = w.X;
+    int64_t n = w.n;
+    int d = w.d;
+    const int32_t* counters = w.L.counters;
+    const double* d64c = d64;
+    void* args[] = {&X,    &n,    &d,    &meta, &empty_list, &counters, &Cnext, &d64c,
+                    &hist, &wcnt, &gt,   &eq,   &cand,       &cand_d,   &objp,  &acc};
+    KM_HIP(hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(256), args, 0, ctx->stream));
 }
 
 template <typename F>
@@ -716,6 +732,10 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
     double* part = ctx->part.get<double>((n / REDUCE_CHUNK + k + 1) * int64_t(d));
     int32_t* pin = ctx->pinned;
     const bool need_obj = thr > 0;
+    // re-scored rows (candidate, full, collect) and refill events, summed on the device
+    unsigned long long* acc = ctx->stat_acc.get<unsigned long long>(4);
+    fill<unsigned long long>(ctx, acc, 4, 0ull);
+    double* d64 = ctx->d64.get<double>(n);  // per-row d64 for empty clusters
 
     hipEvent_t loop0 = nullptr, loop1 = nullptr;
     ctx->ev_used = 0;
@@ -749,20 +769,16 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
             KM_LAUNCH_CHECK();
         }
         double* objp = nullptr;
-        if (need_obj) objp = objective<F>(ctx, w, Ccur, nullptr);
-        KM_HIP(hipMemcpyAsync(pin, meta, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        if (need_obj)
-            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(hipStreamSynchronize(ctx->stream));
-        collect_refine_stats(ctx, pin + 2);
-        const int m = pin[0];
-        double obj = 0.0;
-        if (need_obj) std::memcpy(&obj, pin + 6, sizeof(double));
-        if (m > 0) handle_empty<F>(ctx, w, Ccur, Cnext, m, empty_list, &obj, need_obj);
+        if (need_obj) objp = objective<F>(ctx, w, Ccur, d64);
+        // empty clusters and the statistics stay on the device: with no stop
+        // threshold an iteration needs no host round trip
+        empty_clusters<F>(ctx, w, Ccur, Cnext, empty_list, meta, objp, d64, acc);
         std::swap(Ccur, Cnext);
         if (need_obj) {
+            double obj = 0.0;
+            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KM_HIP(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&obj, pin + 6, sizeof(double));
             if (obj + thr > prev) {
                 iter++;
                 break;
@@ -778,11 +794,17 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
     KM_HIP(hipMemcpyAsync(pin + 2, w.L.counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long acc_h[4];
+    KM_HIP(hipMemcpyAsync(acc_h, acc, sizeof(acc_h), hipMemcpyDeviceToHost, ctx->stream));
     const hipMemcpyKind out_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     KM_HIP(hipMemcpyAsync(C_out, Ccur, sizeof(F) * k * d, out_kind, ctx->stream));
     KM_HIP(hipMemcpyAsync(labels_out, w.labels, sizeof(int32_t) * n, out_kind, ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));
     collect_refine_stats(ctx, pin + 2);
+    ctx->stats.refined_candidate_rows += int64_t(acc_h[0]);
+    ctx->stats.refined_full_rows += int64_t(acc_h[1]);
+    ctx->stats.refined_collect_rows += int64_t(acc_h[2]);
+    ctx->stats.empty_cluster_events += int64_t(acc_h[3]);
     double obj;
     std::memcpy(&obj, pin + 6, sizeof(double));
     if (obj_out) *obj_out = obj;
@@ -1997,7 +2019,7 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
                         &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l,
                         &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids,
-                        &ctx->gt_d, &ctx->csr_vals, &ctx->csr_cols, &ctx->csr_offs, &ctx->csr_xs,
+                        &ctx->gt_d, &ctx->stat_acc, &ctx->coop_wcnt, &ctx->csr_vals, &ctx->csr_cols, &ctx->csr_offs, &ctx->csr_xs,
                         &ctx->csr_ct, &ctx->csr_cs, &ctx->csr_closest, &ctx->csr_flag};
     for (auto* b : bufs) b->release();
     ctx->bws.release();

@@ -17,6 +17,8 @@
 
 #include "km_common.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 namespace km {
 
 template <typename T>
@@ -414,7 +416,8 @@ template <typename F>
 __global__ __launch_bounds__(256) void objective_chunks_kernel(
     const F* __restrict__ X, int64_t n, int d, const F* __restrict__ C,
     const int32_t* __restrict__ labels, double* __restrict__ d64_out,
-    double* __restrict__ chunk_sums) {
+    double* __restrict__ chunk_sums, const int32_t* __restrict__ gate) {
+    if (gate && gate[0] == 0) return;  // gated off (no empty clusters this iteration)
     constexpr int FT = obj_slice<F>::FT, ST = obj_slice<F>::ST;
     constexpr int V = 16 / int(sizeof(F));            // elements per 16-byte load
     constexpr int PPR = FT / V;                       // 16-byte parts per row slice
@@ -573,6 +576,187 @@ __global__ void gather_f64_kernel(const double* __restrict__ src, const int32_t*
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
          i += int64_t(gridDim.x) * blockDim.x)
         out[i] = src[idx[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Empty clusters on the device, one cooperative launch per iteration (grid-wide
+// barriers between the steps; every workgroup passes every barrier).  With
+// meta[0] = m > 0 empty clusters:
+//   1. d64 of every row against its label's centroid is given (the objective
+//      kernel's per-row output);
+//   2. radix select of the m-th largest d64, 8 bits a pass from the top (keys:
+//      the raw bits of the non-negative doubles, which order like the values);
+//      every workgroup reads the same global histogram and picks the same digit;
+//   3. rows above the threshold (gt) and the first `remaining` rows equal to it
+//      (eq), collected in row order (workgroup w owns a contiguous row range;
+//      block scans give in-order ranks);
+//   4. gt rows ordered by (d64 descending, row ascending), eq rows after them:
+//      pick e is the row with the e-th largest d64, ties to the lowest row;
+//   5. pick e fills the e-th empty cluster (ascending index) of Cnext;
+//   6. the objective (when given) drops by the picks' d64, in pick order.
+// Workgroup 0 first adds the iteration's re-score counters into acc[0..2];
+// acc[3] counts iterations that refilled clusters.  With m = 0 the launch ends
+// there.
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ __launch_bounds__(256) void empty_clusters_kernel(
+    const F* __restrict__ X, int64_t n, int d, const int32_t* __restrict__ meta,
+    const int32_t* __restrict__ empty_list, const int32_t* __restrict__ counters, F* __restrict__ Cnext,
+    const double* __restrict__ d64, unsigned int* __restrict__ hist, int32_t* __restrict__ wcnt,
+    int32_t* __restrict__ gt_list, int32_t* __restrict__ eq_list, int32_t* __restrict__ cand,
+    double* __restrict__ cand_d, double* __restrict__ obj, unsigned long long* __restrict__ acc) {
+    namespace cg = cooperative_groups;
+    __shared__ unsigned int h[256];
+    __shared__ int32_t sa[256], sb[256];
+    __shared__ unsigned long long s_prefix;
+    __shared__ long long s_rem;
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0 && t == 0 && counters) {
+        acc[0] += (unsigned long long)counters[0];
+        acc[1] += (unsigned long long)counters[1];
+        acc[2] += (unsigned long long)counters[2];
+    }
+    const int m = meta[0];
+    if (m <= 0) return;  // the same for every workgroup
+    cg::grid_group grid = cg::this_grid();
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + t, gsize = int64_t(gridDim.x) * 256;
+    for (int64_t i = gtid; i < 8 * 256; i += gsize) hist[i] = 0u;
+    grid.sync();
+    // 2. radix select
+    unsigned long long prefix = 0ull, mask = 0ull;
+    long long rem = m;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        h[t] = 0u;
+        __syncthreads();
+        for (int64_t i = gtid; i < n; i += gsize) {
+            const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+            if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 0xFF], 1u);
+        }
+        __syncthreads();
+        if (h[t]) atomicAdd(&hist[pass * 256 + t], h[t]);
+        grid.sync();
+        if (t == 0) {
+            long long cum = 0;
+            int digit = 0;
+            for (int dg = 255; dg >= 0; --dg) {
+                const long long c = hist[pass * 256 + dg];
+                if (cum + c >= rem) {
+                    digit = dg;
+                    rem -= cum;
+                    break;
+                }
+                cum += c;
+            }
+            s_prefix = prefix | ((unsigned long long)digit << shift);
+            s_rem = rem;
+        }
+        __syncthreads();
+        prefix = s_prefix;
+        rem = s_rem;
+        mask |= 0xFFull << shift;
+        __syncthreads();
+    }
+    // 3. in-order collection; workgroup w owns rows [lo, hi)
+    const int64_t span = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = min<int64_t>(n, int64_t(blockIdx.x) * span), hi = min<int64_t>(n, lo + span);
+    int cg_ = 0, ce = 0;
+    for (int64_t i = lo + t; i < hi; i += 256) {
+        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+        cg_ += key > prefix;
+        ce += key == prefix;
+    }
+    sa[t] = cg_;
+    sb[t] = ce;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (t < s2) {
+            sa[t] += sa[t + s2];
+            sb[t] += sb[t + s2];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        wcnt[2 * blockIdx.x] = sa[0];
+        wcnt[2 * blockIdx.x + 1] = sb[0];
+    }
+    grid.sync();
+    long long bg = 0, be = 0;  // this workgroup's first gt and eq positions
+    for (int w = t; w < int(blockIdx.x); w += 256) {
+        bg += wcnt[2 * w];
+        be += wcnt[2 * w + 1];
+    }
+    __syncthreads();
+    sa[t] = int(bg);
+    sb[t] = int(be);
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (t < s2) {
+            sa[t] += sa[t + s2];
+            sb[t] += sb[t + s2];
+        }
+        __syncthreads();
+    }
+    bg = sa[0];
+    be = sb[0];
+    __syncthreads();
+    for (int64_t tile = lo; tile < hi; tile += 256) {
+        const int64_t i = tile + t;
+        bool fg = false, fe = false;
+        if (i < hi) {
+            const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+            fg = key > prefix;
+            fe = key == prefix;
+        }
+        sa[t] = fg;
+        sb[t] = fe;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {  // inclusive scans
+            const int a = t >= o ? sa[t - o] : 0, b = t >= o ? sb[t - o] : 0;
+            __syncthreads();
+            sa[t] += a;
+            sb[t] += b;
+            __syncthreads();
+        }
+        if (fg) gt_list[bg + sa[t] - 1] = int32_t(i);
+        if (fe && be + sb[t] - 1 < rem) eq_list[be + sb[t] - 1] = int32_t(i);
+        bg += sa[255];
+        be += sb[255];
+        __syncthreads();
+    }
+    grid.sync();
+    // 4. pick order
+    const long long ngt = m - rem;
+    for (long long i = gtid; i < ngt; i += gsize) {
+        const int32_t ri = gt_list[i];
+        const unsigned long long ki = (unsigned long long)__double_as_longlong(d64[ri]);
+        long long rank = 0;
+        for (long long j = 0; j < ngt; ++j) {
+            const int32_t rj = gt_list[j];
+            const unsigned long long kj = (unsigned long long)__double_as_longlong(d64[rj]);
+            rank += (kj > ki) || (kj == ki && rj < ri);
+        }
+        cand[rank] = ri;
+        cand_d[rank] = d64[ri];
+    }
+    for (long long e = gtid; e < rem; e += gsize) {
+        cand[ngt + e] = eq_list[e];
+        cand_d[ngt + e] = d64[eq_list[e]];
+    }
+    grid.sync();
+    // 5. fill, 6. objective
+    for (int64_t e = gtid; e < int64_t(m) * d; e += gsize) {
+        const int q = int(e / d), f = int(e - int64_t(q) * d);
+        Cnext[int64_t(empty_list[q]) * d + f] = X[int64_t(cand[q]) * d + f];
+    }
+    if (gtid == 0) {
+        if (obj) {
+            double o = *obj;
+            for (int e = 0; e < m; ++e) o = o - cand_d[e];
+            *obj = o;
+        }
+        acc[3] += 1ull;
+    }
 }
 
 template <typename F>

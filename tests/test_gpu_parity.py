@@ -81,6 +81,25 @@ def test_empty_clusters_fixture(engine, oracle, dtype, path):
     assert_bit_exact((c, lab, obj, it), oracle.canonical_lloyd(x, c0, 1, 0.0))
 
 
+@pytest.mark.parametrize("path", ["generic", "mfma_f32", "bf16"])
+@pytest.mark.parametrize("thr", [0.0, 1e-12], ids=["no_stop_rule", "stop_rule"])
+def test_empty_clusters_many_with_ties(engine, oracle, path, thr):
+    """Many empty clusters at once, with tied d64 values (integer rows with many
+    duplicates; initial centroids repeated, so the repeats start empty) over rows
+    spread across many workgroups: the device-side selection (radix select,
+    in-order collection, pick order by d64 then row) equals the canonical
+    restatement, with and without the stop rule's objective."""
+    bf16 = set_path(engine, path)
+    rng = np.random.default_rng(77)
+    n, d, k = 60000, 4, 96
+    x = rng.integers(-6, 7, size=(n, d)).astype(np.float32)
+    idx = rng.choice(n, size=k // 3, replace=False)
+    c0 = x[np.resize(idx, k)].copy()  # every centroid three times
+    got = engine.train(x, c0, 4, thr, bf16=bf16)
+    assert engine.stats()["empty_cluster_events"] >= 1
+    assert_bit_exact(got, oracle.canonical_lloyd(x, c0, 4, thr))
+
+
 SHAPES = [
     # n, d, k, iters
     (1000, 1, 3, 5),
