@@ -707,7 +707,9 @@ __global__ __launch_bounds__(256) void batch_scatter_kernel(const batch_inst* __
 // label's member offset plus the counts of the instance's earlier blocks).
 // The last workgroup to finish (a ticket counter, reset by that workgroup) writes
 // the exclusive prefix over instances of their chunk counts: qpre[i] = first
-// global chunk of instance i, qpre[B] = total.
+// global chunk of instance i, qpre[B] = total; and, when dpre is not null, the
+// same prefix of the d64 blocks (BATCH_D64_ROWS rows) of the instances that have
+// empty clusters (0 blocks for the others), so the d64 pass visits only those.
 __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __restrict__ I, int B,
                                                           int32_t* __restrict__ counts,
                                                           int32_t* __restrict__ bhist,
@@ -716,6 +718,7 @@ __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __re
                                                           int32_t* __restrict__ empty_list,
                                                           int32_t* __restrict__ meta,
                                                           int64_t* __restrict__ qpre,
+                                                          int64_t* __restrict__ dpre,
                                                           uint32_t* __restrict__ ticket) {
     __shared__ int64_t s_off[1024];
     __shared__ int32_t s_chk[1024];
@@ -814,26 +817,34 @@ __global__ __launch_bounds__(1024) void batch_scan_kernel(const batch_inst* __re
     __syncthreads();
     if (!s_last) return;
     __threadfence();  // every workgroup's meta after the last ticket
-    int64_t carry = 0;
-    for (int base = 0; base < B; base += 1024) {
-        const int i = base + t;
-        const int64_t v = i < B ? int64_t(meta[2 * i + 1]) : 0;
-        s_off[t] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const int64_t a = t >= o ? s_off[t - o] : 0;
+    for (int which = 0; which < 2; ++which) {
+        int64_t* out = which == 0 ? qpre : dpre;
+        if (!out) continue;
+        int64_t carry = 0;
+        for (int base = 0; base < B; base += 1024) {
+            const int i = base + t;
+            int64_t v = 0;
+            if (i < B) {
+                if (which == 0)
+                    v = meta[2 * i + 1];
+                else if (meta[2 * i] > 0)
+                    v = (I[i].n + BATCH_D64_ROWS - 1) / BATCH_D64_ROWS;
+            }
+            s_off[t] = v;
             __syncthreads();
-            s_off[t] += a;
+            for (int o = 1; o < 1024; o <<= 1) {
+                const int64_t a = t >= o ? s_off[t - o] : 0;
+                __syncthreads();
+                s_off[t] += a;
+                __syncthreads();
+            }
+            if (i < B) out[i] = carry + s_off[t] - v;
+            carry += s_off[1023];
             __syncthreads();
         }
-        if (i < B) qpre[i] = carry + s_off[t] - v;
-        carry += s_off[1023];
-        __syncthreads();
+        if (t == 0) out[B] = carry;
     }
-    if (t == 0) {
-        qpre[B] = carry;
-        *ticket = 0u;  // ready for the next launch
-    }
+    if (t == 0) *ticket = 0u;  // ready for the next launch
 }
 
 // Chunk sums of the instances whose rows allow 16-byte reads (d % 4 == 0,
@@ -1164,8 +1175,10 @@ __device__ __forceinline__ double batch_block_d64(const batch_inst& in, int64_t 
 }
 
 // d64 of every row of the instances that have empty clusters (meta[2i] > 0), or
-// of every instance when meta is null; labels: concatenated (row0), or each
-// instance's labels_out when labels is null; BATCH_D64_ROWS rows per workgroup
+// of every instance when meta is null; pre: the instances' d64 block prefix (with
+// meta, the scan's prefix over the instances with empty clusters, so the others
+// take no blocks); labels: concatenated (row0), or each instance's labels_out
+// when labels is null; BATCH_D64_ROWS rows per workgroup
 __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __restrict__ I, int B,
                                                         const int64_t* __restrict__ pre,
                                                         const int32_t* __restrict__ meta,

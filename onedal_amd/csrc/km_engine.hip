@@ -1288,6 +1288,7 @@ struct batch_plan {
     int32_t* bhist = nullptr;
     uint32_t* ticket = nullptr;
     int64_t* qpre = nullptr;
+    int64_t* d64pre = nullptr;  // d64 blocks of the instances with empty clusters (device prefix)
     double* part = nullptr;
     double* objc = nullptr;
     double* objd = nullptr;
@@ -1459,7 +1460,8 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
         p.cb = ws.cb.get<int32_t>(p.kk + B);
         p.empty = ws.empty.get<int32_t>(p.kk);
         p.meta = ws.meta.get<int32_t>(2 * int64_t(B));
-        p.qpre = ws.qpre.get<int64_t>(B + 1);
+        p.qpre = ws.qpre.get<int64_t>(2 * (int64_t(B) + 1));
+        p.d64pre = p.qpre + (B + 1);
         p.ticket = ws.ticket.get<uint32_t>(1);
         fill<uint32_t>(ctx, p.ticket, 1, 0u);
         if (p.csort) {
@@ -1557,7 +1559,7 @@ static void plan_iteration(km_ctx* ctx, batch_plan& p, hipEvent_t asg_start = nu
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_scan_kernel, dim3(g(B)), dim3(1024), 0, ctx->stream, p.dI, B, p.counts,
                        p.csort ? p.bhist : (int32_t*)nullptr, p.offsets, p.cb, p.empty, p.meta, p.qpre,
-                       p.ticket);
+                       p.d64pre, p.ticket);
     KM_LAUNCH_CHECK();
     if (p.csort) {
         hipLaunchKernelGGL(batch_scatter_kernel, dim3(g(p.P(BP::P_CNT, B))), dim3(256),
@@ -1583,8 +1585,9 @@ static void plan_iteration(km_ctx* ctx, batch_plan& p, hipEvent_t asg_start = nu
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(g((p.P(BP::P_EL, B) + 255) / 256)), dim3(256), 0,
                        ctx->stream, p.dI, B, p.DP(BP::P_EL), p.counts, p.cb, p.part, p.Cnext);
     KM_LAUNCH_CHECK();
+    // blocks of the instances with empty clusters only (the scan's prefix)
     hipLaunchKernelGGL(batch_d64_kernel, dim3(g(p.P(BP::P_D64, B))), dim3(256), 0, ctx->stream, p.dI, B,
-                       p.DP(BP::P_D64), p.meta, p.Ccur, p.labels, p.d64);
+                       p.d64pre, p.meta, p.Ccur, p.labels, p.d64);
     KM_LAUNCH_CHECK();
     hipLaunchKernelGGL(batch_fill_empty_kernel, dim3(g(B)), dim3(256), 0, ctx->stream, p.dI, B, p.meta,
                        p.empty, p.d64, p.Ccur, p.Cnext);
