@@ -29,7 +29,7 @@ PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy of the same run
 # (profiles/traffic_precision_n20M_d128_k1024.md); hardware counters cannot be read
 # from inside this process.
-MEASURED_TRAFFIC = {"precision_c3": {"bytes": 5.537e9, "read": 5.299e9, "write": 0.238e9,
+MEASURED_TRAFFIC = {"precision_c3": {"bytes": 5.538e9, "read": 5.300e9, "write": 0.238e9,
                                      "source": "profiles/traffic_precision_n20M_d128_k1024.md"}}
 
 WORKLOADS = {
@@ -271,7 +271,7 @@ def run_batched(torch, args, wl):
         "data": "synthetic",
         "config": {"workload": args.workload, "instances": count, "rows": rows, "iterations": iters,
                    "path": "batched assignment (exact-form VALU tiles or certified FP32 matrix "
-                           "cores per tuned (d, k) bucket), HIP graph per iteration",
+                           "cores per tuned (d, k) bucket), the whole job one HIP graph",
                    "tiles": "autotuned per (d, k) bucket" if tune else "default by k"},
         "autotune": tune,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2),

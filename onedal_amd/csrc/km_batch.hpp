@@ -1109,67 +1109,48 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const batch_inst* _
     }
 }
 
-// Row staging of the FP64 distance kernels: 256 rows of one instance, 32
-// features at a time (row stride 33 floats: a thread reading its own row hits its
-// own bank), so the X reads are coalesced whatever d is; the same 32 features of
-// every centroid sit beside them when the instance has at most 128 centroids.
-constexpr int BATCH_D64_FT = 32;
-constexpr int BATCH_D64_MAXK = 128;
-struct batch_d64_tile {
-    float x[256][BATCH_D64_FT + 1];
-    float c[BATCH_D64_MAXK][BATCH_D64_FT + 1];
-};
-
-// FP64 squared distance of row r0 + threadIdx.x (valid when threadIdx.x < nr) to
-// its labelled centroid: sum over the features in order of (double(x) - double(c))^2,
-// each sub, mul and add rounded separately (the single-instance d64 and objective
-// order); every thread of the workgroup must call it.  lab_base[r] is the label of row r.
-__device__ __forceinline__ double batch_block_d64(const batch_inst& in, int64_t r0, int nr,
-                                                  const int32_t* __restrict__ lab_base,
-                                                  const float* __restrict__ Ci, batch_d64_tile& T) {
+// FP64 squared distance of one row x to its labelled centroid c: sum over the
+// features in order of (double(x) - double(c))^2, each sub, mul and add rounded
+// separately (the single-instance d64 and objective order).  The thread reads the
+// row and the centroid directly, a 128-byte line per step with 16-byte loads when
+// vec (d % 4 == 0, 16-byte aligned rows and centroids).
+__device__ __forceinline__ double batch_row_d64(const float* __restrict__ x, const float* __restrict__ c,
+                                                int d, bool vec) {
     typedef float F4 __attribute__((ext_vector_type(4)));
-    const int t = threadIdx.x;
-    const int d = in.d, k = in.k;
-    int lab = 0;
-    if (t < nr) {
-        lab = lab_base[r0 + t];
-        lab = (lab >= 0 && lab < k) ? lab : 0;
-    }
-    const bool ctab = k <= BATCH_D64_MAXK;  // centroid slices staged beside the rows
-    const float* c = Ci + int64_t(lab) * d;
     double acc = 0.0;
-    for (int f0 = 0; f0 < d; f0 += BATCH_D64_FT) {
-        const int ft = min(BATCH_D64_FT, d - f0);
-        __syncthreads();
-        if (in.vec) {
-            // d % 4 == 0, so ft % 4 == 0: 16-byte reads, four LDS writes each
-            const int v4 = ft >> 2;
-            for (int e = t; e < nr * v4; e += 256) {
-                const int r = e / v4, q = (e - r * v4) * 4;
-                const F4 v = *reinterpret_cast<const F4*>(in.X + (r0 + r) * d + f0 + q);
+    int f = 0;
+    if (vec) {
+        for (; f + 32 <= d; f += 32) {
+            F4 xv[8], cv[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) T.x[r][q + u] = v[u];
+            for (int v = 0; v < 8; ++v) {
+                xv[v] = *reinterpret_cast<const F4*>(x + f + 4 * v);
+                cv[v] = *reinterpret_cast<const F4*>(c + f + 4 * v);
             }
-        } else {
-            for (int e = t; e < nr * ft; e += 256) {
-                const int r = e / ft, q = e - r * ft;
-                T.x[r][q] = in.X[(r0 + r) * d + f0 + q];
-            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double diff = double(xv[v][e]) - double(cv[v][e]);
+                    const double sq = diff * diff;
+                    acc = acc + sq;
+                }
         }
-        if (ctab)
-            for (int e = t; e < k * ft; e += 256) {
-                const int j = e / ft, q = e - j * ft;
-                T.c[j][q] = Ci[int64_t(j) * d + f0 + q];
-            }
-        __syncthreads();
-        if (t < nr) {
-            for (int q = 0; q < ft; ++q) {
-                const float cv = ctab ? T.c[lab][q] : c[f0 + q];
-                const double diff = double(T.x[t][q]) - double(cv);
+        for (; f + 4 <= d; f += 4) {
+            const F4 x4 = *reinterpret_cast<const F4*>(x + f);
+            const F4 c4 = *reinterpret_cast<const F4*>(c + f);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double diff = double(x4[e]) - double(c4[e]);
                 const double sq = diff * diff;
                 acc = acc + sq;
             }
         }
+    }
+    for (; f < d; ++f) {
+        const double diff = double(x[f]) - double(c[f]);
+        const double sq = diff * diff;
+        acc = acc + sq;
     }
     return acc;
 }
@@ -1186,7 +1167,6 @@ __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __rest
                                                         const int32_t* __restrict__ labels,
                                                         double* __restrict__ d64) {
     static_assert(BATCH_D64_ROWS == 256, "one row per thread");
-    __shared__ batch_d64_tile Xs;
     const int64_t total = pre[B];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
         const int ii = batch_find(pre, B, b);
@@ -1195,8 +1175,14 @@ __global__ __launch_bounds__(256) void batch_d64_kernel(const batch_inst* __rest
         const int64_t r0 = (b - pre[ii]) * BATCH_D64_ROWS;
         const int nr = int(min<int64_t>(BATCH_D64_ROWS, in.n - r0));
         const int32_t* lab = labels ? labels + in.row0 : in.labels_out;
-        const double v = batch_block_d64(in, r0, nr, lab, C + in.c0, Xs);
-        if (int(threadIdx.x) < nr) d64[in.row0 + r0 + threadIdx.x] = v;
+        const int t = threadIdx.x;
+        if (t < nr) {
+            const int64_t r = r0 + t;
+            int j = lab[r];
+            j = (j >= 0 && j < in.k) ? j : 0;
+            // centroid rows are 16-byte aligned when d % 4 == 0 (c0 is a multiple of 4)
+            d64[in.row0 + r] = batch_row_d64(in.X + r * in.d, C + in.c0 + int64_t(j) * in.d, in.d, in.vec != 0);
+        }
     }
 }
 

@@ -435,7 +435,7 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
     KM_LAUNCH_CHECK();
 }
 
-// objective chunk sums (objective_chunks_kernel, rows staged through dynamic LDS)
+// objective chunk sums (objective_chunks_kernel)
 template <typename F>
 static void launch_objective(km_ctx* ctx, int grid, const F* X, int64_t n, int d, const F* C,
                              const int32_t* labels, double* d64_out, double* chunks,

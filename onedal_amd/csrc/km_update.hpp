@@ -395,21 +395,13 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
 // order; the 256 partials are combined by the tree p[t] += p[t+s], s = 128..1;
 // chunk sums are added in chunk order by sum_sequential_kernel.
 // ---------------------------------------------------------------------------
-// Rows reach the threads through LDS, one wave at a time: wave w of a 256-row
-// group owns rows w*64 .. w*64+63 (thread t = row t of the group); per 128-byte
-// slice of the features (OBJ_SLICE<F> elements) the wave stages its 64 rows with
-// contiguous loads into its own LDS rows, and each lane adds its row's squared
-// differences slice after slice, reading its centroid's slice directly (16-byte
-// loads; the centroids stay in cache), so d64 keeps the feature order.  The next
-// slice's row loads are issued before the current slice is summed.
-template <typename F>
-struct obj_slice {
-    static constexpr int FT = 128 / int(sizeof(F));  // elements per slice
-    static constexpr int ST = FT + 1;                 // LDS row stride (bank spread)
-};
+// Thread t reads its rows directly, two rows (t + 256 j and t + 256 (j + 1)) at a
+// time so their loads overlap: one 128-byte line of each row and of its
+// centroid per step (16-byte loads), each row's squares added in feature order;
+// the two rows' d64 then enter the thread's sum in row order.  No LDS.
 template <typename F>
 __host__ inline size_t objective_lds_bytes() {
-    return size_t(256) * obj_slice<F>::ST * sizeof(F);
+    return 0;
 }
 
 template <typename F>
@@ -418,87 +410,69 @@ __global__ __launch_bounds__(256) void objective_chunks_kernel(
     const int32_t* __restrict__ labels, double* __restrict__ d64_out,
     double* __restrict__ chunk_sums, const int32_t* __restrict__ gate) {
     if (gate && gate[0] == 0) return;  // gated off (no empty clusters this iteration)
-    constexpr int FT = obj_slice<F>::FT, ST = obj_slice<F>::ST;
-    constexpr int V = 16 / int(sizeof(F));            // elements per 16-byte load
-    constexpr int PPR = FT / V;                       // 16-byte parts per row slice
+    constexpr int RPT = 2;                           // rows in flight per thread
+    constexpr int LV = 128 / (4 * int(sizeof(F)));   // 4-vectors per 128-byte line
     __shared__ double part[256];
-    extern __shared__ __align__(16) unsigned char obj_smem[];
     const int t = threadIdx.x;
-    const int lane = t & 63, wave = t >> 6;
-    F* xs = reinterpret_cast<F*>(obj_smem) + wave * 64 * ST;  // this wave's [64][ST]
-    const bool vec = (d % V) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0 &&
-                     (reinterpret_cast<uintptr_t>(C) % 16) == 0;
-    typedef F FV __attribute__((ext_vector_type(V)));
+    const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(F))) == 0 &&
+                     (reinterpret_cast<uintptr_t>(C) % (4 * sizeof(F))) == 0;
+    typedef F F4 __attribute__((ext_vector_type(4)));
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     for (int64_t q = blockIdx.x; q < nchunks; q += gridDim.x) {
         const int64_t r0 = q * OBJ_CHUNK;
         const int rows = int(min<int64_t>(OBJ_CHUNK, n - r0));
         double p = 0.0;
-        // thread t owns chunk rows t, t+256, ... in increasing order
-        for (int g0 = 0; g0 < rows; g0 += 256) {
-            const int nr = min(256, rows - g0);
-            const int64_t wb = r0 + g0 + wave * 64;        // the wave's first row
-            const int wn = max(0, min(64, nr - wave * 64));  // its rows in this group
-            const bool mine = lane < wn;
-            const F* c = C + int64_t(mine ? labels[wb + lane] : 0) * d;
-            double acc = 0.0;
+        // thread t owns chunk rows t, t+256, ... and adds them in that order
+        for (int r = t; r < rows; r += 256 * RPT) {
+            double acc[RPT];
+            const F* xs[RPT];
+            const F* cs[RPT];
+            bool ok[RPT];
+#pragma unroll
+            for (int u = 0; u < RPT; ++u) {
+                const int rr = r + 256 * u;
+                ok[u] = rr < rows;
+                const int64_t row = r0 + (ok[u] ? rr : r);
+                xs[u] = X + row * d;
+                cs[u] = C + int64_t(labels[row]) * d;
+                acc[u] = 0.0;
+            }
+            int f = 0;
             if (vec) {
-                // the wave's row slice f0: parts e = lane + 64 u, (row, part) = (e / PPR, e % PPR)
-                FV xv[PPR];
-                auto load = [&](int f0) {
+                for (; f + 4 * LV <= d; f += 4 * LV) {
+                    F4 xv[RPT][LV], cv[RPT][LV];
 #pragma unroll
-                    for (int u = 0; u < PPR; ++u) {
-                        const int e = lane + 64 * u, r = e / PPR, f = (e - r * PPR) * V;
-                        xv[u] = (r < wn && f0 + f < d) ? *reinterpret_cast<const FV*>(X + (wb + r) * d + f0 + f)
-                                                       : FV{};
-                    }
-                };
-                load(0);
-                for (int f0 = 0; f0 < d; f0 += FT) {
-                    const int ft = min(FT, d - f0);
-                    __builtin_amdgcn_wave_barrier();  // the previous slice's readers are done
+                    for (int u = 0; u < RPT; ++u)
 #pragma unroll
-                    for (int u = 0; u < PPR; ++u) {
-                        const int e = lane + 64 * u, r = e / PPR, f = (e - r * PPR) * V;
+                        for (int v = 0; v < LV; ++v) {
+                            xv[u][v] = *reinterpret_cast<const F4*>(xs[u] + f + 4 * v);
+                            cv[u][v] = *reinterpret_cast<const F4*>(cs[u] + f + 4 * v);
+                        }
 #pragma unroll
-                        for (int v = 0; v < V; ++v) xs[r * ST + f + v] = xv[u][v];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (f0 + FT < d) load(f0 + FT);
-                    if (mine) {
-                        for (int f = 0; f < ft; f += V) {
-                            const FV cv = *reinterpret_cast<const FV*>(c + f0 + f);
+                    for (int u = 0; u < RPT; ++u)
 #pragma unroll
-                            for (int v = 0; v < V; ++v) {
-                                const double diff = double(xs[lane * ST + f + v]) - double(cv[v]);
+                        for (int v = 0; v < LV; ++v)
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const double diff = double(xv[u][v][e]) - double(cv[u][v][e]);
                                 const double sq = diff * diff;
-                                acc = acc + sq;
+                                acc[u] = acc[u] + sq;
                             }
-                        }
-                    }
-                }
-            } else {
-                for (int f0 = 0; f0 < d; f0 += FT) {
-                    const int ft = min(FT, d - f0);
-                    __builtin_amdgcn_wave_barrier();
-                    for (int e = lane; e < wn * ft; e += 64) {
-                        const int r = e / ft, f = e - r * ft;
-                        xs[r * ST + f] = X[(wb + r) * d + f0 + f];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (mine) {
-                        for (int f = 0; f < ft; ++f) {
-                            const double diff = double(xs[lane * ST + f]) - double(c[f0 + f]);
-                            const double sq = diff * diff;
-                            acc = acc + sq;
-                        }
-                    }
                 }
             }
-            if (mine) {
-                if (d64_out) d64_out[wb + lane] = acc;
-                p = p + acc;
-            }
+            for (; f < d; ++f)
+#pragma unroll
+                for (int u = 0; u < RPT; ++u) {
+                    const double diff = double(xs[u][f]) - double(cs[u][f]);
+                    const double sq = diff * diff;
+                    acc[u] = acc[u] + sq;
+                }
+#pragma unroll
+            for (int u = 0; u < RPT; ++u)
+                if (ok[u]) {
+                    if (d64_out) d64_out[r0 + r + 256 * u] = acc[u];
+                    p = p + acc[u];
+                }
         }
         part[t] = p;
         __syncthreads();
