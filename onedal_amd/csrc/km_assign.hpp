@@ -401,4 +401,130 @@ __global__ __launch_bounds__(256, DP >= 256 ? 1 : 2) void assign_mfma_f32_kernel
     }
 }
 
+// ---------------------------------------------------------------------------
+// Certified FP64 matrix-core assignment (double data, d <= DP <= 256).
+// x.c_j runs on v_mfma_f64_16x16x4f64: a wave takes NB blocks of 16 rows, lane
+// (r16, q) holds row r16's features [q DP/4, (q+1) DP/4) in registers and feeds
+// them in steps s (feature q DP/4 + s on the 4-deep inner dimension), the same
+// mapping on the centroid side from LDS.  Centroids stream through LDS in chunks
+// of CB = 64 (zero-padded to DP features and to whole blocks of 16).  Lane (r16,
+// q) ends a block with v_j = ||c_j||^2 - 2 x.c_j for centroids q, q+4, q+8 and
+// q+12 of the block (the instruction's output rows in that lane) and keeps two
+// top-2 slots (even and odd blocks); a row's eight slots
+// (its four lanes) are settled by decide_row with the FP64 error band, as the
+// FP32 matrix-core kernel does, so labels equal the exact form's.
+// ---------------------------------------------------------------------------
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+constexpr int MFMA_F64_CB = 64;  // centroids per LDS chunk
+
+__host__ __device__ constexpr int mfma_f64_cst(int dp) { return dp + 2; }  // LDS row stride (doubles)
+__host__ inline size_t mfma_f64_lds_bytes(int dp) {
+    return size_t(MFMA_F64_CB) * mfma_f64_cst(dp) * 8 + MFMA_F64_CB * 8;
+}
+
+template <int DP, int NB>
+__global__ __launch_bounds__(256, 2) void assign_mfma_f64_kernel(
+    const double* __restrict__ X, int64_t ldx, int64_t n, int d, const double* __restrict__ Cpad, int k,
+    const double* __restrict__ cn, const float* __restrict__ xnorm_up,
+    const double* __restrict__ cmax_p, band_consts bc, int32_t* __restrict__ labels,
+    refine_lists L) {
+    constexpr int Q = DP / 4;           // features per lane (and MFMA steps per block)
+    constexpr int CST = mfma_f64_cst(DP);
+    constexpr int ROWS_PER_WAVE = 16 * NB;
+    constexpr int ROWS_PER_WG = 4 * ROWS_PER_WAVE;
+    extern __shared__ __align__(16) double dsm[];
+    double* Cs = dsm;                          // [CB][CST]
+    double* Cn = dsm + MFMA_F64_CB * CST;      // [CB]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int r16 = lane & 15;
+    const int q = lane >> 4;
+    const double cmax = *cmax_p;
+    const int64_t ntiles = (n + ROWS_PER_WG - 1) / ROWS_PER_WG;
+    const int nchunks = (k + MFMA_F64_CB - 1) / MFMA_F64_CB;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t wrow0 = tile * ROWS_PER_WG + int64_t(wave) * ROWS_PER_WAVE;
+        double xr[NB][Q];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int64_t row = wrow0 + nb * 16 + r16;
+            const bool ok = row < n;
+            const double* src = X + (ok ? row : 0) * ldx + q * Q;
+#pragma unroll
+            for (int s = 0; s < Q; s += 2) {
+                doublex2 v = {0.0, 0.0};
+                if (ok && q * Q + s < d) v = *reinterpret_cast<const doublex2*>(src + s);
+                xr[nb][s] = v[0];
+                xr[nb][s + 1] = v[1];
+            }
+        }
+        top2<double> st[NB][2];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            st[nb][0].init();
+            st[nb][1].init();
+        }
+        for (int ch = 0; ch < nchunks; ++ch) {
+            const int c0 = ch * MFMA_F64_CB;
+            __syncthreads();  // the previous chunk's readers are done
+            for (int e = tid; e < MFMA_F64_CB * (DP / 2); e += 256) {
+                const int r = e / (DP / 2), f = (e - r * (DP / 2)) * 2;
+                doublex2 v = {0.0, 0.0};
+                if (c0 + r < k) v = *reinterpret_cast<const doublex2*>(Cpad + int64_t(c0 + r) * DP + f);
+                *reinterpret_cast<doublex2*>(Cs + r * CST + f) = v;
+            }
+            for (int r = tid; r < MFMA_F64_CB; r += 256) Cn[r] = c0 + r < k ? cn[c0 + r] : fp_traits<double>::inf;
+            __syncthreads();
+            const int blocks = min(MFMA_F64_CB, k - c0 + 15) / 16;
+            for (int b = 0; b < blocks; ++b) {
+                doublex4 acc[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[nb] = doublex4{0.0, 0.0, 0.0, 0.0};
+                const double* arow = Cs + (b * 16 + r16) * CST + q * Q;
+#pragma unroll
+                for (int s = 0; s < Q; s += 2) {
+                    const doublex2 a2 = *reinterpret_cast<const doublex2*>(arow + s);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[u], xr[nb][s + u], acc[nb], 0, 0, 0);
+                }
+                const int gb = (c0 >> 4) + b;  // global block index
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int lj = b * 16 + q + 4 * t;  // output row q + 4 t of the 16 x 16 block
+                    const double cnj = Cn[lj];
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (gb & 1)
+                            st[nb][1].push(fma(-2.0, acc[nb][t], cnj), c0 + lj);
+                        else
+                            st[nb][0].push(fma(-2.0, acc[nb][t], cnj), c0 + lj);
+                    }
+                }
+            }
+        }
+        // settle rows: lane (r16, 0) gathers the eight slots of its row
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            top2<double> sl[8];
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) {
+                    sl[2 * w + z].m1 = __shfl_xor(st[nb][z].m1, 16 * w);
+                    sl[2 * w + z].m2 = __shfl_xor(st[nb][z].m2, 16 * w);
+                    sl[2 * w + z].i1 = __shfl_xor(st[nb][z].i1, 16 * w);
+                    sl[2 * w + z].i2 = __shfl_xor(st[nb][z].i2, 16 * w);
+                }
+            const int64_t row = wrow0 + nb * 16 + r16;
+            if (q == 0 && row < n)
+                decide_row<double>(row, sl, 8, bc, double(xnorm_up[row]), cmax, labels, L);
+        }
+    }
+}
+
 }  // namespace km

@@ -226,11 +226,12 @@ static int pad_dim(int d) {
     return p;
 }
 
-// 1 generic tile kernel, 2 FP32 matrix cores, 3 BF16 matrix cores + exact re-score.
-// BF16 applies to float data with d <= 256; double data always takes path 1.
+// 1 generic tile kernel, 2 matrix cores of the data's precision (FP32 or FP64),
+// 3 BF16 matrix cores + exact re-score.  BF16 applies to float data with
+// d <= 256; double data takes path 2 for d <= 256, else path 1.
 template <typename F>
 static int choose_path(const km_ctx* ctx, int d, int k, uint32_t flags) {
-    if (std::is_same<F, double>::value) return 1;
+    if (std::is_same<F, double>::value) return (d <= 256 && ctx->assign_path_opt != 1) ? 2 : 1;
     if (d > 256) return 1;
     // keys carry (block, register) in at most 15 low bits: k <= 65536
     if ((flags & KM_FLAG_BF16_REFINE) && k <= 65536) return 3;
@@ -315,6 +316,19 @@ static void launch_mfma_f32(km_ctx* ctx, work<float>& w, const band_consts& bc) 
         launch_mfma_f32_mode<DP, NB, false>(ctx, w, bc, grid, mfma_f32_lds_bytes(DP, w.k, false));
 }
 
+template <int DP, int NB>
+static void launch_mfma_f64(km_ctx* ctx, work<double>& w, const band_consts& bc) {
+    constexpr int ROWS_PER_WG = 4 * 16 * NB;
+    const int64_t tiles = (w.n + ROWS_PER_WG - 1) / ROWS_PER_WG;
+    const int grid = int(std::max<int64_t>(1, std::min<int64_t>(tiles, int64_t(ctx->cu_count) * 2)));
+    const size_t lds = mfma_f64_lds_bytes(DP);
+    auto kern = assign_mfma_f64_kernel<DP, NB>;
+    KM_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, ctx->stream, w.Xv, w.ldx, w.n, w.d, w.cpad, w.k, w.cn,
+                       w.xnorm, (const double*)w.cmax_bits, bc, w.labels, w.L);
+    KM_LAUNCH_CHECK();
+}
+
 template <int DPB>
 static void launch_bf16(km_ctx* ctx, work<float>& w, bool collect) {
     constexpr int NB = DPB >= 256 ? 1 : 2;  // row registers of two blocks fit below 256 VGPRs
@@ -397,6 +411,16 @@ static void assign(km_ctx* ctx, work<F>& w, const F* C, int path) {
                 case 64: launch_bf16<64>(ctx, w, false); break;
                 case 128: launch_bf16<128>(ctx, w, false); break;
                 default: launch_bf16<256>(ctx, w, false); break;
+            }
+        }
+    }
+    if constexpr (std::is_same<F, double>::value) {
+        if (path == 2) {
+            switch (w.dpad) {
+                case 32: launch_mfma_f64<32, 2>(ctx, w, bc); break;
+                case 64: launch_mfma_f64<64, 2>(ctx, w, bc); break;
+                case 128: launch_mfma_f64<128, 2>(ctx, w, bc); break;
+                default: launch_mfma_f64<256, 1>(ctx, w, bc); break;
             }
         }
     }

@@ -188,11 +188,13 @@ def test_infer_bit_exact(engine, oracle, dtype, path):
     assert obj == oracle.canonical_objective(x, c, wl)
 
 
-def test_duplicate_centroids_tie_to_lowest_index(engine, oracle):
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_duplicate_centroids_tie_to_lowest_index(engine, oracle, dtype):
     """Exact ties (duplicate centroids) must go to the lowest index, as in
-    select_min_distance (kernels_fp_impl.hpp:198-202); exercises the re-score kernels."""
+    select_min_distance (kernels_fp_impl.hpp:198-202); exercises the re-score
+    kernels behind the FP32 and the FP64 matrix-core assignment."""
     set_path(engine, "mfma_f32")
-    x = gaussian_mixture(5000, 16, 4, seed=5)
+    x = gaussian_mixture(5000, 16, 4, seed=5, dtype=dtype)
     c = np.concatenate([x[:4], x[:4]]).copy()  # centroid j+4 duplicates j
     lab, _ = engine.infer(x, c)
     assert lab.max() < 4
