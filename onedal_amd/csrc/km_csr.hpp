@@ -253,7 +253,9 @@ template <typename F>
 __global__ __launch_bounds__(256) void csr_objective_chunks_kernel(const F* __restrict__ closest,
                                                                    int64_t n,
                                                                    double* __restrict__ d64_out,
-                                                                   double* __restrict__ chunk_sums) {
+                                                                   double* __restrict__ chunk_sums,
+                                                                   const int32_t* __restrict__ gate) {
+    if (gate && gate[0] == 0) return;  // gated off (no empty clusters this iteration)
     __shared__ double part[256];
     const int t = threadIdx.x;
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
@@ -409,7 +411,8 @@ template <typename F>
 __global__ __launch_bounds__(64) void csr_fill_empty_kernel(
     const F* __restrict__ vals, const int64_t* __restrict__ cols, const int64_t* __restrict__ offs,
     int64_t base, int d, const int32_t* __restrict__ empty_list, const int32_t* __restrict__ cand,
-    int m, F* __restrict__ Cnext) {
+    const int32_t* __restrict__ meta, F* __restrict__ Cnext) {
+    const int m = meta[0];  // empty clusters this iteration (picks in cand, ec_order_kernel)
     for (int e = blockIdx.x; e < m; e += gridDim.x) {
         F* dst = Cnext + int64_t(empty_list[e]) * d;
         for (int f = threadIdx.x; f < d; f += blockDim.x) dst[f] = F(0);

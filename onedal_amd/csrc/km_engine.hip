@@ -128,14 +128,14 @@ struct km_ctx {
     km::dbuf seg_cand_rows, seg_cand_cnt, seg_cand_ids, seg_coll_rows, seg_coll_T, seg_counts;
     km::dbuf x_dev, x_pad, c_cur, c_next, cn, cpad, cmax, xnorm, labels, lists_rows, lists_cnt, lists_ids,
         lists_full, lists_counters, counts, offsets, chunk_base, empty_list, meta, perm, keys_out,
-        sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_counters, cand_dev,
+        sort_tmp, part, d64, obj_chunks, obj_dev, hist, cand_gt, cand_eq, cand_dev,
         c_out_dev;
     // batched training and steps: the iterated set, and the finished set of a step
     km::batch_ws bws, bws_fin;
     km::dbuf tune_x, tune_c, tune_l;  // autotuner inputs (released after tuning)
     km::dbuf init_d2, init_chunks, init_chosen, init_ids;  // initialisation
     km::dbuf gt_d;  // empty-cluster candidates' d64
-    km::dbuf stat_acc, coop_wcnt;  // device-side statistics; per-workgroup counts of empty_clusters_kernel
+    km::dbuf stat_acc, coop_wcnt;  // device-side statistics; per-workgroup counts of ec_count_kernel
     // CSR path: values, column indices, row offsets (when uploaded), row squares,
     // transposed centroids, centroid squares, closest distances, structure flag
     km::dbuf csr_vals, csr_cols, csr_offs, csr_xs, csr_ct, csr_cs, csr_closest, csr_flag;
@@ -537,102 +537,47 @@ static void sort_by_label(km_ctx* ctx, const int32_t* labels, int64_t n, int k, 
                                      bits, ctx->stream));
 }
 
-// The m rows with the largest d64 (ties: lowest row), in that order, on the
-// device; their d64 in cand_d.
-static int32_t* select_far_rows(km_ctx* ctx, const double* d64, int64_t n, int m,
-                                std::vector<double>& cand_d) {
-    // radix select of the m-th largest key, 8 bits at a time from the top
-    unsigned int* hist = ctx->hist.get<unsigned int>(256);
-    unsigned long long prefix = 0, mask = 0;
-    int64_t remaining = m;
-    std::vector<unsigned int> h(256);
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        fill<unsigned int>(ctx, hist, 256, 0u);
-        const int grid = grid_for(ctx, n, 256, 4);
-        hipLaunchKernelGGL(digit_hist_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n,
-                           prefix, mask, shift, hist);
-        KM_LAUNCH_CHECK();
-        KM_HIP(hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned int), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        KM_HIP(hipStreamSynchronize(ctx->stream));
-        int64_t cum = 0;
-        int digit = 0;
-        for (int dg = 255; dg >= 0; --dg) {
-            if (cum + h[dg] >= remaining) {
-                digit = dg;
-                remaining -= cum;
-                break;
-            }
-            cum += h[dg];
-        }
-        prefix |= (unsigned long long)digit << shift;
-        mask |= 0xFFull << shift;
-    }
-    // prefix = key of the m-th largest; `remaining` rows equal to it are taken
-    int32_t* gt = ctx->cand_gt.get<int32_t>(m);
-    int32_t* eq = ctx->cand_eq.get<int32_t>(n);
-    int32_t* cc = ctx->cand_counters.get<int32_t>(2);
-    fill<int32_t>(ctx, cc, 2, 0);
-    const int grid = grid_for(ctx, n, 256, 4);
-    hipLaunchKernelGGL(collect_candidates_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n,
-                       prefix, gt, eq, cc);
+// Empty clusters of one training iteration, on the device (the ec_* kernels: a
+// fixed sequence of launches, each ending at once when meta[0] = 0): the m rows
+// with the largest d64 (ties: lowest row) go, in that order, to the empty
+// clusters in ascending index order (dense rows X; with X null the picks are
+// left in the context's candidate list for the sparse fill); the objective at
+// objp (when not null) drops by their d64, subtracted in that order.  The
+// sequence also adds the re-score counters (when not null) into acc.  Returns
+// the candidate list.
+template <typename F>
+static int32_t* launch_empty_clusters(km_ctx* ctx, const F* X, int64_t n, int d, int k, const int32_t* meta,
+                                      const int32_t* empty_list, const int32_t* counters, F* Cnext,
+                                      const double* d64, double* objp, unsigned long long* acc) {
+    const int grid = int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, int64_t(ctx->cu_count) * 4)));
+    unsigned int* hist = ctx->hist.get<unsigned int>(EC_PASSES * 256);
+    int32_t* wcnt = ctx->coop_wcnt.get<int32_t>(2 * int64_t(grid));
+    int32_t* gt = ctx->cand_gt.get<int32_t>(k);
+    int32_t* eq = ctx->cand_eq.get<int32_t>(k);
+    int32_t* cand = ctx->cand_dev.get<int32_t>(k);
+    double* cand_d = ctx->gt_d.get<double>(k);
+    hipLaunchKernelGGL(ec_begin_kernel, dim3(1), dim3(256), 0, ctx->stream, meta, counters, acc, hist);
     KM_LAUNCH_CHECK();
-    int32_t counts[2];
-    KM_HIP(hipMemcpyAsync(counts, cc, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(hipStreamSynchronize(ctx->stream));
-    const int64_t n_gt = counts[0], n_eq = counts[1];
-    if (n_gt + remaining != m || remaining > n_eq)
-        throw status_error(KM_INTERNAL_ERROR, "empty-cluster candidate selection is inconsistent");
-    std::vector<int32_t> vgt(n_gt), veq(n_eq);
-    if (n_gt) KM_HIP(hipMemcpyAsync(vgt.data(), gt, n_gt * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (n_eq) KM_HIP(hipMemcpyAsync(veq.data(), eq, n_eq * 4, hipMemcpyDeviceToHost, ctx->stream));
-    double* gt_d = ctx->gt_d.get<double>(std::max<int64_t>(n_gt, 1));
-    std::vector<double> dgt(n_gt);
-    if (n_gt) {
-        hipLaunchKernelGGL(gather_f64_kernel, dim3(grid_for(ctx, n_gt, 256, 1)), dim3(256), 0,
-                           ctx->stream, d64, gt, n_gt, gt_d);
+    for (int pass = 0; pass < EC_PASSES; ++pass) {
+        hipLaunchKernelGGL(ec_hist_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n, meta, hist, pass);
         KM_LAUNCH_CHECK();
-        KM_HIP(hipMemcpyAsync(dgt.data(), gt_d, n_gt * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
     }
-    KM_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<std::pair<double, int32_t>> order(n_gt);
-    for (int64_t i = 0; i < n_gt; ++i) order[i] = {dgt[i], vgt[i]};
-    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
-        if (a.first != b.first) return a.first > b.first;
-        return a.second < b.second;
-    });
-    std::sort(veq.begin(), veq.end());
-    double thr_val;
-    {
-        const unsigned long long pk = prefix;
-        std::memcpy(&thr_val, &pk, sizeof(double));
-    }
-    std::vector<int32_t> cand(m);
-    cand_d.assign(m, 0.0);
-    for (int64_t i = 0; i < n_gt; ++i) {
-        cand[i] = order[i].second;
-        cand_d[i] = order[i].first;
-    }
-    for (int64_t i = 0; i < remaining; ++i) {
-        cand[n_gt + i] = veq[i];
-        cand_d[n_gt + i] = thr_val;
-    }
-    int32_t* cand_dev = ctx->cand_dev.get<int32_t>(m);
-    KM_HIP(hipMemcpyAsync(cand_dev, cand.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
-    // cand is a stack vector: the copy must finish before it goes
-    KM_HIP(hipStreamSynchronize(ctx->stream));
-    return cand_dev;
+    hipLaunchKernelGGL(ec_count_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n, meta, hist, wcnt);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ec_collect_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, n, meta, hist, wcnt, gt,
+                       eq);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ec_order_kernel, dim3(grid), dim3(256), 0, ctx->stream, d64, meta, hist, gt, eq, cand,
+                       cand_d);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ec_finish_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream, X, d, meta, empty_list, cand,
+                       cand_d, Cnext, objp, acc);
+    KM_LAUNCH_CHECK();
+    return cand;
 }
 
-// Empty clusters of one training iteration, on the device: the rows' d64 (the
-// objective pass, gated on meta[0] > 0 unless objp already ran with d64 out),
-// then empty_clusters_kernel, one cooperative launch that ends at once when
-// meta[0] = 0: the m rows with the largest d64 (ties: lowest row) go, in that
-// order, to the empty clusters in ascending index order; the objective at objp
-// (when not null) drops by their d64, subtracted in that order.  The launch also
-// adds the iteration's re-score counters into acc.
+// Dense rows: the d64 pass first (the objective pass, gated on meta[0] > 0
+// unless objp already ran with d64 out), then launch_empty_clusters.
 template <typename F>
 static void empty_clusters(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, const int32_t* empty_list,
                            const int32_t* meta, double* objp, double* d64, unsigned long long* acc) {
@@ -642,24 +587,7 @@ static void empty_clusters(km_ctx* ctx, work<F>& w, const F* Ccur, F* Cnext, con
         launch_objective<F>(ctx, ogrid, w.X, w.n, w.d, Ccur, w.labels, d64,
                             ctx->obj_chunks.get<double>(nchunks), meta);
     }
-    auto kern = empty_clusters_kernel<F>;
-    int per_cu = 0;
-    KM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 256, 0));
-    const int grid = std::max(1, std::min(per_cu, 4)) * ctx->cu_count;
-    unsigned int* hist = ctx->hist.get<unsigned int>(8 * 256);
-    int32_t* wcnt = ctx->coop_wcnt.get<int32_t>(2 * int64_t(grid));
-    int32_t* gt = ctx->cand_gt.get<int32_t>(w.k);
-    int32_t* eq = ctx->cand_eq.get<int32_t>(w.k);
-    int32_t* cand = ctx->cand_dev.get<int32_t>(w.k);
-    double* cand_d = ctx->gt_d.get<double>(w.k);
-    const F* X = w.X;
-    int64_t n = w.n;
-    int d = w.d;
-    const int32_t* counters = w.L.counters;
-    const double* d64c = d64;
-    void* args[] = {&X,    &n,    &d,    &meta, &empty_list, &counters, &Cnext, &d64c,
-                    &hist, &wcnt, &gt,   &eq,   &cand,       &cand_d,   &objp,  &acc};
-    KM_HIP(hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(256), args, 0, ctx->stream));
+    launch_empty_clusters<F>(ctx, w.X, w.n, w.d, w.k, meta, empty_list, w.L.counters, Cnext, d64, objp, acc);
 }
 
 template <typename F>
@@ -1088,14 +1016,16 @@ static void csr_assign(km_ctx* ctx, const csr_view<F>& x, const F* C, int k, con
 
 // objective = sum of closest (as double) in the fixed order; d64 per row if asked
 template <typename F>
-static double* csr_objective(km_ctx* ctx, const F* closest, int64_t n, double* d64_out) {
+static double* csr_objective(km_ctx* ctx, const F* closest, int64_t n, double* d64_out,
+                             const int32_t* gate = nullptr) {
     const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
     double* chunks = ctx->obj_chunks.get<double>(nchunks);
     double* out = ctx->obj_dev.get<double>(1);
     const int grid = int(std::min<int64_t>(nchunks, int64_t(ctx->cu_count) * 8));
     hipLaunchKernelGGL(csr_objective_chunks_kernel<F>, dim3(grid), dim3(256), 0, ctx->stream,
-                       closest, n, d64_out, chunks);
+                       closest, n, d64_out, chunks, gate);
     KM_LAUNCH_CHECK();
+    if (gate) return nullptr;  // the d64 pass of the empty-cluster refill only
     hipLaunchKernelGGL(sum_sequential_kernel, dim3(1), dim3(64), 0, ctx->stream, chunks, nchunks, out);
     KM_LAUNCH_CHECK();
     return out;
@@ -1148,6 +1078,9 @@ static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
     const size_t lds = size_t(reduce_groups(d)) * d * sizeof(double);
     KM_HIP(hipFuncSetAttribute((const void*)csr_chunk_sums_kernel<F>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    double* d64 = ctx->d64.get<double>(n);  // refill candidates' keys
+    unsigned long long* acc = ctx->stat_acc.get<unsigned long long>(4);  // [3]: refill events
+    fill<unsigned long long>(ctx, acc, 4, 0ull);
     if (ctx->timing) KM_HIP(hipEventRecord(loop0, ctx->stream));
 
     double prev = double(std::numeric_limits<F>::max());
@@ -1174,33 +1107,25 @@ static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
         hipLaunchKernelGGL(finalize_centroids_kernel<F>, dim3(grid_for(ctx, int64_t(k) * d, 256, 4)),
                            dim3(256), 0, ctx->stream, part, counts, chunk_base, k, d, Cnext);
         KM_LAUNCH_CHECK();
+        // empty clusters on the device (d64 pass gated on the empty count, the
+        // selection kernels, the sparse fill): no host round trip without a
+        // stop threshold
         double* objp = nullptr;
-        if (need_obj) objp = csr_objective<F>(ctx, closest, n, nullptr);
-        KM_HIP(hipMemcpyAsync(pin, meta, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         if (need_obj)
-            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(hipStreamSynchronize(ctx->stream));
-        const int m = pin[0];
-        double obj = 0.0;
-        if (need_obj) std::memcpy(&obj, pin + 6, sizeof(double));
-        if (m > 0) {
-            double* d64 = ctx->d64.get<double>(n);
-            double* op = csr_objective<F>(ctx, closest, n, d64);
-            std::vector<double> cand_d;
-            int32_t* cand_dev = select_far_rows(ctx, d64, n, m, cand_d);
-            hipLaunchKernelGGL(csr_fill_empty_kernel<F>, dim3(std::min(m, ctx->cu_count * 4)),
-                               dim3(64), 0, ctx->stream, x.vals, x.cols, x.offs, x.base, d,
-                               empty_list, cand_dev, m, Cnext);
-            KM_LAUNCH_CHECK();
-            if (!need_obj)
-                KM_HIP(hipMemcpyAsync(pin + 8, op, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            KM_HIP(hipStreamSynchronize(ctx->stream));
-            if (!need_obj) std::memcpy(&obj, pin + 8, sizeof(double));
-            for (int e = 0; e < m; ++e) obj = obj - cand_d[e];
-            ctx->stats.empty_cluster_events++;
-        }
+            objp = csr_objective<F>(ctx, closest, n, d64);
+        else
+            csr_objective<F>(ctx, closest, n, d64, meta);
+        int32_t* cand = launch_empty_clusters<F>(ctx, (const F*)nullptr, n, d, k, meta, empty_list,
+                                                 (const int32_t*)nullptr, Cnext, d64, objp, acc);
+        hipLaunchKernelGGL(csr_fill_empty_kernel<F>, dim3(std::min(k, ctx->cu_count * 4)), dim3(64), 0,
+                           ctx->stream, x.vals, x.cols, x.offs, x.base, d, empty_list, cand, meta, Cnext);
+        KM_LAUNCH_CHECK();
         std::swap(Ccur, Cnext);
         if (need_obj) {
+            double obj = 0.0;
+            KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KM_HIP(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&obj, pin + 6, sizeof(double));
             if (obj + thr > prev) {
                 iter++;
                 break;
@@ -1212,10 +1137,13 @@ static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
     double* objp = csr_objective<F>(ctx, closest, n, nullptr);
     if (ctx->timing) KM_HIP(hipEventRecord(loop1, ctx->stream));
     KM_HIP(hipMemcpyAsync(pin + 6, objp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long acc_h[4];
+    KM_HIP(hipMemcpyAsync(acc_h, acc, sizeof(acc_h), hipMemcpyDeviceToHost, ctx->stream));
     const hipMemcpyKind out_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     KM_HIP(hipMemcpyAsync(C_out, Ccur, sizeof(F) * k * d, out_kind, ctx->stream));
     KM_HIP(hipMemcpyAsync(labels_out, labels, sizeof(int32_t) * n, out_kind, ctx->stream));
     KM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stats.empty_cluster_events += int64_t(acc_h[3]);
     double obj;
     std::memcpy(&obj, pin + 6, sizeof(double));
     if (obj_out) *obj_out = obj;
@@ -2042,7 +1970,7 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->offsets, &ctx->chunk_base, &ctx->empty_list, &ctx->meta, &ctx->perm,
                         &ctx->keys_out, &ctx->sort_tmp, &ctx->part, &ctx->d64, &ctx->obj_chunks,
                         &ctx->obj_dev, &ctx->hist, &ctx->cand_gt, &ctx->cand_eq,
-                        &ctx->cand_counters, &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
+                        &ctx->cand_dev, &ctx->c_out_dev, &ctx->seg_cand_rows,
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
                         &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l,
                         &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids,

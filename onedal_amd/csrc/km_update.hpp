@@ -17,8 +17,6 @@
 
 #include "km_common.hpp"
 
-#include <hip/hip_cooperative_groups.h>
-
 namespace km {
 
 template <typename T>
@@ -361,7 +359,7 @@ __global__ __launch_bounds__(256) void reduce_chunks_vec_kernel(
 
 // ---------------------------------------------------------------------------
 // merge chunk sums per cluster (chunk order) and divide: C_j = (F)(S1_j / n_j).
-// Empty clusters are left for fill_empty_kernel.
+// Empty clusters are left for the ec_* kernels (refilled there).
 // ---------------------------------------------------------------------------
 template <typename F>
 __global__ __launch_bounds__(256) void finalize_centroids_kernel(
@@ -515,128 +513,131 @@ __global__ void sum_sequential_into_kernel(const double* __restrict__ v, int64_t
 }
 
 // ---------------------------------------------------------------------------
-// empty clusters: radix select of the m largest d64 (keys are the raw bits of
-// non-negative doubles, which order like the values).
+// Empty clusters on the device, as a fixed sequence of ordinary launches per
+// iteration, each of which ends at once when meta[0] = 0 (no empty cluster):
+//   ec_begin_kernel     adds the iteration's re-score counters into acc[0..2];
+//                       with m = meta[0] > 0 empty clusters, zeroes the
+//                       histograms;
+//   ec_hist_kernel x 8  radix select of the m-th largest d64, 8 bits a pass from
+//                       the top (keys: the raw bits of the non-negative doubles,
+//                       which order like the values); every workgroup derives
+//                       the digits of the earlier passes from their histograms;
+//   ec_count_kernel,    rows above the threshold (gt) and the first `remaining`
+//   ec_collect_kernel   rows equal to it (eq), collected in row order (workgroup
+//                       w owns a contiguous row range; block scans give in-order
+//                       ranks);
+//   ec_order_kernel     gt rows ordered by (d64 descending, row ascending), eq
+//                       rows after them: pick e is the row with the e-th largest
+//                       d64, ties to the lowest row;
+//   ec_finish_kernel    pick e fills the e-th empty cluster (ascending index) of
+//                       Cnext (dense rows; sparse rows are filled by
+//                       csr_fill_empty_kernel from cand), the objective (when
+//                       given) drops by the picks' d64 in pick order, and acc[3]
+//                       counts the iteration.
+// The d64 of every row against its label's centroid comes in from the
+// objective pass (gated on the same count).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void digit_hist_kernel(const double* __restrict__ d64, int64_t n,
-                                                         unsigned long long prefix,
-                                                         unsigned long long mask, int shift,
-                                                         unsigned int* __restrict__ hist) {
-    __shared__ unsigned int h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x) {
-        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
-        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 0xFF], 1u);
+constexpr int EC_PASSES = 8;
+
+// threshold state after `passes` radix passes: key prefix, its mask, and how
+// many rows equal to the final key are still to be taken
+__device__ __forceinline__ void ec_state(const unsigned int* __restrict__ hist, int passes, int m,
+                                         unsigned long long& prefix, unsigned long long& mask,
+                                         long long& rem) {
+    prefix = 0ull;
+    mask = 0ull;
+    rem = m;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 56 - 8 * p;
+        long long cum = 0;
+        int digit = 0;
+        for (int dg = 255; dg >= 0; --dg) {
+            const long long c = hist[p * 256 + dg];
+            if (cum + c >= rem) {
+                digit = dg;
+                rem -= cum;
+                break;
+            }
+            cum += c;
+        }
+        prefix |= (unsigned long long)digit << shift;
+        mask |= 0xFFull << shift;
     }
-    __syncthreads();
-    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void collect_candidates_kernel(
-    const double* __restrict__ d64, int64_t n, unsigned long long thr_key,
-    int32_t* __restrict__ gt_list, int32_t* __restrict__ eq_list, int32_t* __restrict__ counters) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x) {
-        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
-        if (key > thr_key) gt_list[atomicAdd(&counters[0], 1)] = int32_t(i);
-        else if (key == thr_key) eq_list[atomicAdd(&counters[1], 1)] = int32_t(i);
-    }
-}
-
-__global__ void gather_f64_kernel(const double* __restrict__ src, const int32_t* __restrict__ idx,
-                                  int64_t count, double* __restrict__ out) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < count;
-         i += int64_t(gridDim.x) * blockDim.x)
-        out[i] = src[idx[i]];
-}
-
-// ---------------------------------------------------------------------------
-// Empty clusters on the device, one cooperative launch per iteration (grid-wide
-// barriers between the steps; every workgroup passes every barrier).  With
-// meta[0] = m > 0 empty clusters:
-//   1. d64 of every row against its label's centroid is given (the objective
-//      kernel's per-row output);
-//   2. radix select of the m-th largest d64, 8 bits a pass from the top (keys:
-//      the raw bits of the non-negative doubles, which order like the values);
-//      every workgroup reads the same global histogram and picks the same digit;
-//   3. rows above the threshold (gt) and the first `remaining` rows equal to it
-//      (eq), collected in row order (workgroup w owns a contiguous row range;
-//      block scans give in-order ranks);
-//   4. gt rows ordered by (d64 descending, row ascending), eq rows after them:
-//      pick e is the row with the e-th largest d64, ties to the lowest row;
-//   5. pick e fills the e-th empty cluster (ascending index) of Cnext;
-//   6. the objective (when given) drops by the picks' d64, in pick order.
-// Workgroup 0 first adds the iteration's re-score counters into acc[0..2];
-// acc[3] counts iterations that refilled clusters.  With m = 0 the launch ends
-// there.
-// ---------------------------------------------------------------------------
-template <typename F>
-__global__ __launch_bounds__(256) void empty_clusters_kernel(
-    const F* __restrict__ X, int64_t n, int d, const int32_t* __restrict__ meta,
-    const int32_t* __restrict__ empty_list, const int32_t* __restrict__ counters, F* __restrict__ Cnext,
-    const double* __restrict__ d64, unsigned int* __restrict__ hist, int32_t* __restrict__ wcnt,
-    int32_t* __restrict__ gt_list, int32_t* __restrict__ eq_list, int32_t* __restrict__ cand,
-    double* __restrict__ cand_d, double* __restrict__ obj, unsigned long long* __restrict__ acc) {
-    namespace cg = cooperative_groups;
-    __shared__ unsigned int h[256];
-    __shared__ int32_t sa[256], sb[256];
-    __shared__ unsigned long long s_prefix;
+// the workgroup's copy of the state (thread 0 computes it)
+__device__ __forceinline__ void ec_state_shared(const unsigned int* __restrict__ hist, int passes, int m,
+                                                unsigned long long& prefix, unsigned long long& mask,
+                                                long long& rem) {
+    __shared__ unsigned long long s_prefix, s_mask;
     __shared__ long long s_rem;
-    const int t = threadIdx.x;
-    if (blockIdx.x == 0 && t == 0 && counters) {
+    if (threadIdx.x == 0) ec_state(hist, passes, m, s_prefix, s_mask, s_rem);
+    __syncthreads();
+    prefix = s_prefix;
+    mask = s_mask;
+    rem = s_rem;
+}
+
+__device__ __forceinline__ unsigned long long ec_key(double v) {
+    return (unsigned long long)__double_as_longlong(v);
+}
+
+__global__ __launch_bounds__(256) void ec_begin_kernel(const int32_t* __restrict__ meta,
+                                                       const int32_t* __restrict__ counters,
+                                                       unsigned long long* __restrict__ acc,
+                                                       unsigned int* __restrict__ hist) {
+    if (threadIdx.x == 0 && counters) {
         acc[0] += (unsigned long long)counters[0];
         acc[1] += (unsigned long long)counters[1];
         acc[2] += (unsigned long long)counters[2];
     }
+    if (meta[0] <= 0) return;
+    for (int i = threadIdx.x; i < EC_PASSES * 256; i += blockDim.x) hist[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void ec_hist_kernel(const double* __restrict__ d64, int64_t n,
+                                                      const int32_t* __restrict__ meta,
+                                                      unsigned int* __restrict__ hist, int pass) {
     const int m = meta[0];
-    if (m <= 0) return;  // the same for every workgroup
-    cg::grid_group grid = cg::this_grid();
-    const int64_t gtid = int64_t(blockIdx.x) * 256 + t, gsize = int64_t(gridDim.x) * 256;
-    for (int64_t i = gtid; i < 8 * 256; i += gsize) hist[i] = 0u;
-    grid.sync();
-    // 2. radix select
-    unsigned long long prefix = 0ull, mask = 0ull;
-    long long rem = m;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        h[t] = 0u;
-        __syncthreads();
-        for (int64_t i = gtid; i < n; i += gsize) {
-            const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
-            if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 0xFF], 1u);
-        }
-        __syncthreads();
-        if (h[t]) atomicAdd(&hist[pass * 256 + t], h[t]);
-        grid.sync();
-        if (t == 0) {
-            long long cum = 0;
-            int digit = 0;
-            for (int dg = 255; dg >= 0; --dg) {
-                const long long c = hist[pass * 256 + dg];
-                if (cum + c >= rem) {
-                    digit = dg;
-                    rem -= cum;
-                    break;
-                }
-                cum += c;
-            }
-            s_prefix = prefix | ((unsigned long long)digit << shift);
-            s_rem = rem;
-        }
-        __syncthreads();
-        prefix = s_prefix;
-        rem = s_rem;
-        mask |= 0xFFull << shift;
-        __syncthreads();
+    if (m <= 0) return;
+    __shared__ unsigned int h[256];
+    unsigned long long prefix, mask;
+    long long rem;
+    ec_state_shared(hist, pass, m, prefix, mask, rem);
+    const int shift = 56 - 8 * pass;
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const unsigned long long key = ec_key(d64[i]);
+        if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 0xFF], 1u);
     }
-    // 3. in-order collection; workgroup w owns rows [lo, hi)
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[pass * 256 + threadIdx.x], h[threadIdx.x]);
+}
+
+// workgroup w's contiguous row range
+__device__ __forceinline__ void ec_range(int64_t n, int64_t& lo, int64_t& hi) {
     const int64_t span = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = min<int64_t>(n, int64_t(blockIdx.x) * span), hi = min<int64_t>(n, lo + span);
+    lo = min<int64_t>(n, int64_t(blockIdx.x) * span);
+    hi = min<int64_t>(n, lo + span);
+}
+
+__global__ __launch_bounds__(256) void ec_count_kernel(const double* __restrict__ d64, int64_t n,
+                                                       const int32_t* __restrict__ meta,
+                                                       const unsigned int* __restrict__ hist,
+                                                       int32_t* __restrict__ wcnt) {
+    const int m = meta[0];
+    if (m <= 0) return;
+    __shared__ int32_t sa[256], sb[256];
+    const int t = threadIdx.x;
+    unsigned long long prefix, mask;
+    long long rem;
+    ec_state_shared(hist, EC_PASSES, m, prefix, mask, rem);
+    int64_t lo, hi;
+    ec_range(n, lo, hi);
     int cg_ = 0, ce = 0;
     for (int64_t i = lo + t; i < hi; i += 256) {
-        const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+        const unsigned long long key = ec_key(d64[i]);
         cg_ += key > prefix;
         ce += key == prefix;
     }
@@ -654,13 +655,29 @@ __global__ __launch_bounds__(256) void empty_clusters_kernel(
         wcnt[2 * blockIdx.x] = sa[0];
         wcnt[2 * blockIdx.x + 1] = sb[0];
     }
-    grid.sync();
-    long long bg = 0, be = 0;  // this workgroup's first gt and eq positions
+}
+
+__global__ __launch_bounds__(256) void ec_collect_kernel(const double* __restrict__ d64, int64_t n,
+                                                         const int32_t* __restrict__ meta,
+                                                         const unsigned int* __restrict__ hist,
+                                                         const int32_t* __restrict__ wcnt,
+                                                         int32_t* __restrict__ gt_list,
+                                                         int32_t* __restrict__ eq_list) {
+    const int m = meta[0];
+    if (m <= 0) return;
+    __shared__ int32_t sa[256], sb[256];
+    const int t = threadIdx.x;
+    unsigned long long prefix, mask;
+    long long rem;
+    ec_state_shared(hist, EC_PASSES, m, prefix, mask, rem);
+    int64_t lo, hi;
+    ec_range(n, lo, hi);
+    // this workgroup's first gt and eq positions: the counts of the earlier ones
+    long long bg = 0, be = 0;
     for (int w = t; w < int(blockIdx.x); w += 256) {
         bg += wcnt[2 * w];
         be += wcnt[2 * w + 1];
     }
-    __syncthreads();
     sa[t] = int(bg);
     sb[t] = int(be);
     __syncthreads();
@@ -678,7 +695,7 @@ __global__ __launch_bounds__(256) void empty_clusters_kernel(
         const int64_t i = tile + t;
         bool fg = false, fe = false;
         if (i < hi) {
-            const unsigned long long key = (unsigned long long)__double_as_longlong(d64[i]);
+            const unsigned long long key = ec_key(d64[i]);
             fg = key > prefix;
             fe = key == prefix;
         }
@@ -698,16 +715,28 @@ __global__ __launch_bounds__(256) void empty_clusters_kernel(
         be += sb[255];
         __syncthreads();
     }
-    grid.sync();
-    // 4. pick order
+}
+
+__global__ __launch_bounds__(256) void ec_order_kernel(const double* __restrict__ d64,
+                                                       const int32_t* __restrict__ meta,
+                                                       const unsigned int* __restrict__ hist,
+                                                       const int32_t* __restrict__ gt_list,
+                                                       const int32_t* __restrict__ eq_list,
+                                                       int32_t* __restrict__ cand, double* __restrict__ cand_d) {
+    const int m = meta[0];
+    if (m <= 0) return;
+    unsigned long long prefix, mask;
+    long long rem;
+    ec_state_shared(hist, EC_PASSES, m, prefix, mask, rem);
     const long long ngt = m - rem;
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, gsize = int64_t(gridDim.x) * 256;
     for (long long i = gtid; i < ngt; i += gsize) {
         const int32_t ri = gt_list[i];
-        const unsigned long long ki = (unsigned long long)__double_as_longlong(d64[ri]);
+        const unsigned long long ki = ec_key(d64[ri]);
         long long rank = 0;
         for (long long j = 0; j < ngt; ++j) {
             const int32_t rj = gt_list[j];
-            const unsigned long long kj = (unsigned long long)__double_as_longlong(d64[rj]);
+            const unsigned long long kj = ec_key(d64[rj]);
             rank += (kj > ki) || (kj == ki && rj < ri);
         }
         cand[rank] = ri;
@@ -717,12 +746,24 @@ __global__ __launch_bounds__(256) void empty_clusters_kernel(
         cand[ngt + e] = eq_list[e];
         cand_d[ngt + e] = d64[eq_list[e]];
     }
-    grid.sync();
-    // 5. fill, 6. objective
-    for (int64_t e = gtid; e < int64_t(m) * d; e += gsize) {
-        const int q = int(e / d), f = int(e - int64_t(q) * d);
-        Cnext[int64_t(empty_list[q]) * d + f] = X[int64_t(cand[q]) * d + f];
-    }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void ec_finish_kernel(const F* __restrict__ X, int d,
+                                                        const int32_t* __restrict__ meta,
+                                                        const int32_t* __restrict__ empty_list,
+                                                        const int32_t* __restrict__ cand,
+                                                        const double* __restrict__ cand_d, F* __restrict__ Cnext,
+                                                        double* __restrict__ obj,
+                                                        unsigned long long* __restrict__ acc) {
+    const int m = meta[0];
+    if (m <= 0) return;
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, gsize = int64_t(gridDim.x) * 256;
+    if (X)
+        for (int64_t e = gtid; e < int64_t(m) * d; e += gsize) {
+            const int q = int(e / d), f = int(e - int64_t(q) * d);
+            Cnext[int64_t(empty_list[q]) * d + f] = X[int64_t(cand[q]) * d + f];
+        }
     if (gtid == 0) {
         if (obj) {
             double o = *obj;
@@ -730,19 +771,6 @@ __global__ __launch_bounds__(256) void empty_clusters_kernel(
             *obj = o;
         }
         acc[3] += 1ull;
-    }
-}
-
-template <typename F>
-__global__ __launch_bounds__(256) void fill_empty_kernel(const F* __restrict__ X, int d,
-                                                         const int32_t* __restrict__ empty_list,
-                                                         const int32_t* __restrict__ cand, int m,
-                                                         F* __restrict__ Cnext) {
-    const int64_t total = int64_t(m) * d;
-    for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
-         e += int64_t(gridDim.x) * blockDim.x) {
-        const int q = int(e / d), f = int(e % d);
-        Cnext[int64_t(empty_list[q]) * d + f] = X[int64_t(cand[q]) * d + f];
     }
 }
 
