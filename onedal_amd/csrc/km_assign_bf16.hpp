@@ -92,10 +92,10 @@ __host__ __device__ inline int bf16_padded_k(int k) { return (k + 127) / 128 * 1
 // X -> padded BF16 rows (stride dpb), ||x|| and ||x - bf16(x)|| (rounded up),
 // and their maxima.  A row is covered by L = dpb / 4 consecutive lanes, lane f
 // converting features 4f .. 4f + 3 (zero past d), so a wave reads 64 / L rows as
-// one contiguous run of 16-byte loads and writes their bf16 rows as 8-byte stores.
-// The per-row sums are FP64 over shuffles; the 1e-12 factor covers their
-// rounding, so their order is free.  dpb = pad_dim(d) <= 256 (a power of two,
-// at least 32).
+// one contiguous run of 16-byte loads and writes their bf16 rows as 8-byte stores;
+// the loads of four such steps are issued together.  The per-row sums are FP64
+// over shuffles; the 1e-12 factor covers their rounding, so their order is free.
+// dpb = pad_dim(d) <= 256 (a power of two, at least 32).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void convert_rows_bf16_kernel(
     const float* __restrict__ X, int64_t n, int d, int dpb, uint16_t* __restrict__ Xb,
@@ -111,50 +111,60 @@ __global__ __launch_bounds__(256) void convert_rows_bf16_kernel(
     const bool vec = (d % 4) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0;
     const int f0 = 4 * fl;
     double mx = 0.0, me = 0.0;
-    for (int64_t base = wave * R; base < n; base += nwaves * R) {
-        const int64_t row = base + sub;
-        const bool ok = row < n;
-        float v[4] = {0.f, 0.f, 0.f, 0.f};
-        if (ok) {
-            const float* x = X + row * d;
-            if (vec && f0 < d) {
-                const float4 t = *reinterpret_cast<const float4*>(x + f0);
-                v[0] = t.x;
-                v[1] = t.y;
-                v[2] = t.z;
-                v[3] = t.w;
-            } else if (!vec) {
+    constexpr int U = 4;  // wave steps whose loads are issued together
+    for (int64_t base = wave * R * U; base < n; base += nwaves * R * U) {
+        float v[U][4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = f0 + u < d ? x[f0 + u] : 0.f;
+        for (int st = 0; st < U; ++st) {
+            const int64_t row = base + st * R + sub;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[st][u] = 0.f;
+            if (row < n) {
+                const float* x = X + row * d;
+                if (vec && f0 < d) {
+                    const float4 t = *reinterpret_cast<const float4*>(x + f0);
+                    v[st][0] = t.x;
+                    v[st][1] = t.y;
+                    v[st][2] = t.z;
+                    v[st][3] = t.w;
+                } else if (!vec) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v[st][u] = f0 + u < d ? x[f0 + u] : 0.f;
+                }
             }
         }
-        double s = 0.0, e = 0.0;
-        uint16_t b[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            b[u] = f2bf(v[u]);
-            const double dv = double(v[u]);
-            const double ev = dv - double(bf2f(b[u]));
-            s = fma(dv, dv, s);
-            e = fma(ev, ev, e);
-        }
-        if (ok)
-            *reinterpret_cast<uint2*>(Xb + row * dpb + f0) =
-                make_uint2(uint32_t(b[0]) | (uint32_t(b[1]) << 16), uint32_t(b[2]) | (uint32_t(b[3]) << 16));
-        for (int off = L >> 1; off > 0; off >>= 1) {
-            s += __shfl_xor(s, off);
-            e += __shfl_xor(e, off);
-        }
-        if (ok && fl == 0) {
-            const double r = sqrt(s) * (1.0 + 1e-12);
-            const double re = sqrt(e) * (1.0 + 1e-12) + 1e-30;
-            float fr = float(r), fe = float(re);
-            if (double(fr) < r) fr = __uint_as_float(__float_as_uint(fr) + 1u);
-            if (double(fe) < re) fe = __uint_as_float(__float_as_uint(fe) + 1u);
-            xnorm_up[row] = fr;
-            enorm_up[row] = fe;
-            mx = fmax(mx, double(fr));
-            me = fmax(me, double(fe));
+        for (int st = 0; st < U; ++st) {
+            const int64_t row = base + st * R + sub;
+            const bool ok = row < n;
+            double s = 0.0, e = 0.0;
+            uint16_t b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                b[u] = f2bf(v[st][u]);
+                const double dv = double(v[st][u]);
+                const double ev = dv - double(bf2f(b[u]));
+                s = fma(dv, dv, s);
+                e = fma(ev, ev, e);
+            }
+            if (ok)
+                *reinterpret_cast<uint2*>(Xb + row * dpb + f0) =
+                    make_uint2(uint32_t(b[0]) | (uint32_t(b[1]) << 16), uint32_t(b[2]) | (uint32_t(b[3]) << 16));
+            for (int off = L >> 1; off > 0; off >>= 1) {
+                s += __shfl_xor(s, off);
+                e += __shfl_xor(e, off);
+            }
+            if (ok && fl == 0) {
+                const double r = sqrt(s) * (1.0 + 1e-12);
+                const double re = sqrt(e) * (1.0 + 1e-12) + 1e-30;
+                float fr = float(r), fe = float(re);
+                if (double(fr) < r) fr = __uint_as_float(__float_as_uint(fr) + 1u);
+                if (double(fe) < re) fe = __uint_as_float(__float_as_uint(fe) + 1u);
+                xnorm_up[row] = fr;
+                enorm_up[row] = fe;
+                mx = fmax(mx, double(fr));
+                me = fmax(me, double(fe));
+            }
         }
     }
 #pragma unroll
