@@ -594,6 +594,47 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
 //    one contiguous run.  Labels outside [0, k) are skipped, as the histogram
 //    skips them.
 // ---------------------------------------------------------------------------
+// One large instance (the single-instance engine): its block destinations in
+// two parallel steps instead of batch_scan_kernel's per-label walk over every
+// block.  The nb blocks form G groups of GB consecutive blocks.
+// sort_group_sums_kernel: gsum[g k + j] = label j's count over group g's blocks.
+// sort_block_bases_kernel: thread (g, j) starts at offsets[j] plus label j's
+// counts in the earlier groups and turns its blocks' counts into destinations,
+// in block order.  The result equals batch_scan_kernel's.
+__global__ __launch_bounds__(256) void sort_group_sums_kernel(const int32_t* __restrict__ bhist, int64_t nb,
+                                                              int k, int G, int GB,
+                                                              int32_t* __restrict__ gsum) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= int64_t(G) * k) return;
+    const int g = int(e / k), j = int(e - int64_t(g) * k);
+    const int64_t b0 = int64_t(g) * GB, b1 = min<int64_t>(nb, b0 + GB);
+    int32_t s = 0;
+    int64_t lb = b0;
+    for (; lb + 4 <= b1; lb += 4) {
+        const int32_t h0 = bhist[lb * k + j], h1 = bhist[(lb + 1) * k + j], h2 = bhist[(lb + 2) * k + j],
+                      h3 = bhist[(lb + 3) * k + j];
+        s += h0 + h1 + h2 + h3;
+    }
+    for (; lb < b1; ++lb) s += bhist[lb * k + j];
+    gsum[e] = s;
+}
+
+__global__ __launch_bounds__(256) void sort_block_bases_kernel(int32_t* __restrict__ bhist, int64_t nb, int k,
+                                                               int G, int GB, const int32_t* __restrict__ gsum,
+                                                               const int64_t* __restrict__ offsets) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= int64_t(G) * k) return;
+    const int g = int(e / k), j = int(e - int64_t(g) * k);
+    int32_t base = int32_t(offsets[j]);
+    for (int q = 0; q < g; ++q) base += gsum[int64_t(q) * k + j];
+    const int64_t b0 = int64_t(g) * GB, b1 = min<int64_t>(nb, b0 + GB);
+    for (int64_t lb = b0; lb < b1; ++lb) {
+        const int32_t c = bhist[lb * k + j];
+        bhist[lb * k + j] = base;
+        base += c;
+    }
+}
+
 // LDS bytes of batch_scatter_kernel for labels below kcap
 __host__ __device__ constexpr size_t batch_scatter_lds_bytes(int kcap) {
     return (size_t(6) * kcap + BATCH_CNT_ROWS) * sizeof(int32_t);

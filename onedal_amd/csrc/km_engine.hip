@@ -136,6 +136,7 @@ struct km_ctx {
     km::dbuf init_d2, init_chunks, init_chosen, init_ids;  // initialisation
     km::dbuf gt_d;  // empty-cluster candidates' d64
     km::dbuf stat_acc, coop_wcnt;  // device-side statistics; per-workgroup counts of ec_count_kernel
+    km::dbuf cs_inst, cs_pre, cs_hist, cs_gsum;  // single-instance counting sort (label_sort)
     // CSR path: values, column indices, row offsets (when uploaded), row squares,
     // transposed centroids, centroid squares, closest distances, structure flag
     km::dbuf csr_vals, csr_cols, csr_offs, csr_xs, csr_ct, csr_cs, csr_closest, csr_flag;
@@ -537,6 +538,89 @@ static void sort_by_label(km_ctx* ctx, const int32_t* labels, int64_t n, int k, 
                                      bits, ctx->stream));
 }
 
+// Label counts and the stable grouping of row ids by label for one instance.
+// k <= BATCH_HIST_MAX: the batched path's counting sort on a one-instance table
+// (per-block histograms that also add into counts, block destinations from
+// sort_group_sums_kernel / sort_block_bases_kernel, the LDS-staged scatter);
+// otherwise count_kernel and rocPRIM's radix sort.  Both leave the members of
+// each cluster in ascending row order.
+struct label_sort {
+    bool csort = false;
+    int64_t n = 0, nb = 0;
+    int k = 0, G = 0, GB = 0;
+    batch_inst* I = nullptr;
+    int64_t* pre = nullptr;
+    int32_t* bhist = nullptr;
+    int32_t* gsum = nullptr;
+};
+
+// the scatter kernel's LDS limit, set to the largest use (every caller sets the
+// same value, so concurrent contexts never lower it under each other)
+static void allow_scatter_lds() {
+    KM_HIP(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(batch_scatter_lds_bytes(BATCH_HIST_MAX))));
+}
+
+static label_sort label_sort_setup(km_ctx* ctx, int64_t n, int k) {
+    label_sort ls;
+    ls.n = n;
+    ls.k = k;
+    if (k > BATCH_HIST_MAX) return ls;
+    ls.csort = true;
+    ls.nb = (n + BATCH_CNT_ROWS - 1) / BATCH_CNT_ROWS;
+    ls.G = int(std::min<int64_t>(ls.nb, 64));
+    ls.GB = int((ls.nb + ls.G - 1) / ls.G);
+    ls.I = ctx->cs_inst.get<batch_inst>(1);
+    ls.pre = ctx->cs_pre.get<int64_t>(2);
+    ls.bhist = ctx->cs_hist.get<int32_t>(ls.nb * k);
+    ls.gsum = ctx->cs_gsum.get<int32_t>(int64_t(ls.G) * k);
+    batch_inst b{};
+    b.n = n;
+    b.k = k;
+    const int64_t pre[2] = {0, ls.nb};
+    KM_HIP(hipMemcpyAsync(ls.I, &b, sizeof(b), hipMemcpyHostToDevice, ctx->stream));
+    KM_HIP(hipMemcpyAsync(ls.pre, pre, sizeof(pre), hipMemcpyHostToDevice, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));  // b and pre are stack values
+    allow_scatter_lds();
+    return ls;
+}
+
+// counts (zeroed first) of the labels
+static void label_counts(km_ctx* ctx, const label_sort& ls, const int32_t* labels, int32_t* counts) {
+    fill<int32_t>(ctx, counts, ls.k, 0);
+    if (ls.csort) {
+        const int grid = int(std::min<int64_t>(ls.nb, int64_t(ctx->cu_count) * 8));
+        hipLaunchKernelGGL(batch_count_kernel, dim3(grid), dim3(256), 0, ctx->stream, ls.I, 1, ls.pre, labels,
+                           counts, ls.bhist);
+    } else {
+        const bool lds = ls.k <= 16384;
+        hipLaunchKernelGGL(count_kernel, dim3(grid_for(ctx, ls.n, 256, 4)), dim3(256),
+                           lds ? ls.k * sizeof(int32_t) : 0, ctx->stream, labels, ls.n, ls.k, counts, lds ? 1 : 0);
+    }
+    KM_LAUNCH_CHECK();
+}
+
+// row ids grouped by label (after the scan has set offsets)
+static void label_group(km_ctx* ctx, const label_sort& ls, const int32_t* labels, const int64_t* offsets,
+                        int32_t* perm) {
+    if (!ls.csort) {
+        sort_by_label(ctx, labels, ls.n, ls.k, perm);
+        return;
+    }
+    const int64_t gk = int64_t(ls.G) * ls.k;
+    const int g2 = int((gk + 255) / 256);
+    hipLaunchKernelGGL(sort_group_sums_kernel, dim3(g2), dim3(256), 0, ctx->stream, ls.bhist, ls.nb, ls.k, ls.G,
+                       ls.GB, ls.gsum);
+    KM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sort_block_bases_kernel, dim3(g2), dim3(256), 0, ctx->stream, ls.bhist, ls.nb, ls.k, ls.G,
+                       ls.GB, ls.gsum, offsets);
+    KM_LAUNCH_CHECK();
+    const int grid = int(std::min<int64_t>(ls.nb, int64_t(ctx->cu_count) * 8));
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid), dim3(256), batch_scatter_lds_bytes(ls.k), ctx->stream,
+                       ls.I, 1, ls.pre, labels, ls.bhist, perm, ls.k);
+    KM_LAUNCH_CHECK();
+}
+
 // Empty clusters of one training iteration, on the device (the ec_* kernels: a
 // fixed sequence of launches, each ending at once when meta[0] = 0): the m rows
 // with the largest d64 (ties: lowest row) go, in that order, to the empty
@@ -688,6 +772,7 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
     unsigned long long* acc = ctx->stat_acc.get<unsigned long long>(4);
     fill<unsigned long long>(ctx, acc, 4, 0ull);
     double* d64 = ctx->d64.get<double>(n);  // per-row d64 for empty clusters
+    const label_sort lsort = label_sort_setup(ctx, n, k);
 
     hipEvent_t loop0 = nullptr, loop1 = nullptr;
     ctx->ev_used = 0;
@@ -701,18 +786,11 @@ static void train(km_ctx* ctx, const F* X, int64_t n, int64_t d64_, const F* C0,
     for (iter = 0; iter < max_iter; ++iter) {
         centroid_prep<F>(ctx, w, Ccur, path);
         assign<F>(ctx, w, Ccur, path);
-        fill<int32_t>(ctx, counts, k, 0);
-        {
-            const bool lds = k <= 16384;
-            const int grid = grid_for(ctx, n, 256, 4);
-            hipLaunchKernelGGL(count_kernel, dim3(grid), dim3(256), lds ? k * sizeof(int32_t) : 0,
-                               ctx->stream, w.labels, n, k, counts, lds ? 1 : 0);
-            KM_LAUNCH_CHECK();
-        }
+        label_counts(ctx, lsort, w.labels, counts);
         hipLaunchKernelGGL(scan_clusters_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, k,
                            offsets, chunk_base, empty_list, meta);
         KM_LAUNCH_CHECK();
-        sort_by_label(ctx, w.labels, n, k, perm);
+        label_group(ctx, lsort, w.labels, offsets, perm);
         launch_reduce<F>(ctx, w, perm, offsets, chunk_base, meta, part);
         {
             const int grid = grid_for(ctx, int64_t(k) * d, 256, 4);
@@ -1079,6 +1157,7 @@ static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
     KM_HIP(hipFuncSetAttribute((const void*)csr_chunk_sums_kernel<F>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     double* d64 = ctx->d64.get<double>(n);  // refill candidates' keys
+    const label_sort lsort = label_sort_setup(ctx, n, k);
     unsigned long long* acc = ctx->stat_acc.get<unsigned long long>(4);  // [3]: refill events
     fill<unsigned long long>(ctx, acc, 4, 0ull);
     if (ctx->timing) KM_HIP(hipEventRecord(loop0, ctx->stream));
@@ -1087,18 +1166,11 @@ static void train_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int
     int64_t iter = 0;
     for (iter = 0; iter < max_iter; ++iter) {
         csr_assign<F>(ctx, x, Ccur, k, xs, labels, closest);
-        fill<int32_t>(ctx, counts, k, 0);
-        {
-            const bool in_lds = k <= 16384;
-            hipLaunchKernelGGL(count_kernel, dim3(grid_for(ctx, n, 256, 4)), dim3(256),
-                               in_lds ? k * sizeof(int32_t) : 0, ctx->stream, labels, n, k, counts,
-                               in_lds ? 1 : 0);
-            KM_LAUNCH_CHECK();
-        }
+        label_counts(ctx, lsort, labels, counts);
         hipLaunchKernelGGL(scan_clusters_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, k,
                            offsets, chunk_base, empty_list, meta);
         KM_LAUNCH_CHECK();
-        sort_by_label(ctx, labels, n, k, perm);
+        label_group(ctx, lsort, labels, offsets, perm);
         const int64_t max_chunks = n / REDUCE_CHUNK + k + 1;
         hipLaunchKernelGGL(csr_chunk_sums_kernel<F>, dim3(grid_for(ctx, max_chunks * 64, 64, 16)),
                            dim3(64), lds, ctx->stream, x.vals, x.cols, x.offs, x.base, d, k, perm,
@@ -1418,8 +1490,7 @@ static void plan_upload(km_ctx* ctx, batch_ws& ws, batch_plan& p) {
         fill<uint32_t>(ctx, p.ticket, 1, 0u);
         if (p.csort) {
             p.bhist = ws.bhist.get<int32_t>(p.hh);
-            KM_HIP(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(batch_scatter_lds_bytes(p.maxk))));
+            allow_scatter_lds();
         }
         p.part = ws.part.get<double>(p.pp);
         if (!p.csort) {
@@ -1974,7 +2045,8 @@ km_status km_context_destroy(km_ctx* ctx) {
                         &ctx->seg_cand_cnt, &ctx->seg_cand_ids, &ctx->seg_coll_rows, &ctx->seg_coll_T,
                         &ctx->seg_counts, &ctx->tune_x, &ctx->tune_c, &ctx->tune_l,
                         &ctx->init_d2, &ctx->init_chunks, &ctx->init_chosen, &ctx->init_ids,
-                        &ctx->gt_d, &ctx->stat_acc, &ctx->coop_wcnt, &ctx->csr_vals, &ctx->csr_cols, &ctx->csr_offs, &ctx->csr_xs,
+                        &ctx->gt_d, &ctx->stat_acc, &ctx->coop_wcnt, &ctx->cs_inst, &ctx->cs_pre,
+                        &ctx->cs_hist, &ctx->cs_gsum, &ctx->csr_vals, &ctx->csr_cols, &ctx->csr_offs, &ctx->csr_xs,
                         &ctx->csr_ct, &ctx->csr_cs, &ctx->csr_closest, &ctx->csr_flag};
     for (auto* b : bufs) b->release();
     ctx->bws.release();
