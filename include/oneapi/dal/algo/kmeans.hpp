@@ -582,50 +582,17 @@ void csr_of(const hip::queue& q, const table& t, csr_args<Float>& a) {
 
 // Default initial centroids of CSR data: the reference runs DAAL's plusPlusCSR
 // on the host (train_kernel_lloyd_csr_dpc.cpp:41-68, train_init_centroids.hpp:40-41);
-// here the rows are expanded on the host and k-means++ runs on the device with
-// the dense initialiser's seed.
-template <typename Float>
-array<Float> csr_plus_plus(hip::queue& q, const table& t, std::int64_t k) {
-    const csr_table c(t);
-    const std::int64_t n = c.get_row_count(), d = c.get_column_count();
-    if (double(n) * double(d) * sizeof(Float) > double(std::int64_t(1) << 32))
-        throw unimplemented("Default initial centroids of CSR data need the rows expanded on the host; "
-                            "above 4 GiB expanded, pass initial centroids");
-    const std::int64_t base = c.get_indexing() == sparse_indexing::one_based ? 1 : 0;
-    std::vector<std::int64_t> offs(std::size_t(n + 1));
-    std::vector<std::int64_t> cols(std::size_t(c.get_non_zero_count()));
-    std::vector<Float> vals(cols.size());
-    auto fetch = [&](void* dst, const void* src, std::int64_t bytes) {
-        if (!bytes) return;
-        if (c.is_device())
-            hip::check(km_memcpy(q.get_context(), dst, src, bytes, KM_COPY_DEVICE_TO_HOST), q.get_context());
-        else
-            std::memcpy(dst, src, std::size_t(bytes));
-    };
-    fetch(offs.data(), c.get_row_offsets(), std::int64_t(offs.size() * sizeof(std::int64_t)));
-    fetch(cols.data(), c.get_column_indices(), std::int64_t(cols.size() * sizeof(std::int64_t)));
-    csr_args<Float> a;
-    if (c.is_device()) {
-        fetch(vals.data(), c.get_data(), std::int64_t(vals.size() * sizeof(Float)));
-    } else {
-        csr_of<Float>(q, t, a);
-        std::copy(a.values, a.values + a.nnz, vals.begin());
-    }
-    std::vector<Float> dense(std::size_t(n * d), Float(0));
-    for (std::int64_t i = 0; i < n; ++i)
-        for (std::int64_t p = offs[i] - base; p < offs[i + 1] - base; ++p) {
-            if (p < 0 || p >= std::int64_t(cols.size()))
-                throw invalid_argument("Row offsets of the sparse data must be non-decreasing, from the "
-                                       "indexing base to base + non-zero count");
-            const std::int64_t col = cols[p] - base;
-            if (col < 0 || col >= d)
-                throw invalid_argument("Column indices of the sparse data must lie in [base, base + column count)");
-            dense[i * d + col] += vals[p];
-        }
-    array<Float> c0(k * d);
-    hip::check(abi_init_plus_plus<Float>(q.get_context(), dense.data(), n, d, k, 0u, c0.get_mutable_data()),
-               q.get_context());
-    return c0;
+// here the device k-means++ walks the CSR rows (the dense initialiser's result on
+// the rows' dense expansion, with its seed), host or device arrays as given.
+template <typename F>
+km_status abi_init_plus_plus_csr(km_ctx* ctx, const csr_args<F>& a, std::int64_t n, std::int64_t d,
+                                 std::int64_t k, std::uint32_t flags, F* C) {
+    if constexpr (std::is_same_v<F, float>)
+        return km_init_centroids_csr_f32(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, k,
+                                         KM_INIT_PLUS_PLUS_DENSE, 777u, flags, C);
+    else
+        return km_init_centroids_csr_f64(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, k,
+                                         KM_INIT_PLUS_PLUS_DENSE, 777u, flags, C);
 }
 
 template <typename Float>
@@ -647,11 +614,8 @@ train_result<> train_csr_on(hip::queue& q, const descriptor<Float, method::lloyd
         if (has_c0) {
             c0 = to_device<Float>(q, in.get_initial_centroids());
         } else {
-            const array<Float> h = csr_plus_plus<Float>(q, xt, k);
             auto dc = device_array<Float>(q, k * d);
-            hip::check(km_memcpy(ctx, dc.get(), h.get_data(), std::int64_t(sizeof(Float)) * k * d,
-                                 KM_COPY_HOST_TO_DEVICE),
-                       ctx);
+            hip::check(abi_init_plus_plus_csr<Float>(ctx, a, n, d, k, KM_FLAG_DEVICE_POINTERS, dc.get()), ctx);
             c0 = dc;
         }
         auto c = device_array<Float>(q, k * d);
@@ -663,8 +627,13 @@ train_result<> train_csr_on(hip::queue& q, const descriptor<Float, method::lloyd
         r.set_model(model<>().set_centroids(homogen_table::wrap_device<Float>(c, k, d, q.get_device())));
         r.set_responses(homogen_table::wrap_device<std::int32_t>(labels, n, 1, q.get_device()));
     } else {
-        const array<Float> c0 = has_c0 ? row_accessor<const Float>(in.get_initial_centroids()).pull(q)
-                                       : csr_plus_plus<Float>(q, xt, k);
+        array<Float> c0;
+        if (has_c0) {
+            c0 = row_accessor<const Float>(in.get_initial_centroids()).pull(q);
+        } else {
+            c0 = array<Float>(k * d);
+            hip::check(abi_init_plus_plus_csr<Float>(ctx, a, n, d, k, 0u, c0.get_mutable_data()), ctx);
+        }
         array<Float> c(k * d);
         array<std::int32_t> labels(n);
         hip::check(abi_train_csr<Float>(ctx, a.values, a.cols, a.offs, n, d, a.nnz, a.indexing, c0.get_data(), k,

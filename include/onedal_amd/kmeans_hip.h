@@ -213,6 +213,23 @@ km_status km_init_centroids_f32(km_ctx* ctx, const float* X, int64_t n, int64_t 
 km_status km_init_centroids_f64(km_ctx* ctx, const double* X, int64_t n, int64_t d, int64_t k,
                                 int32_t method, uint64_t seed, uint32_t flags, double* C_out);
 
+/* C_out (k x d) = initial centroids of CSR rows (arrays as for km_lloyd_train_csr_*)
+ * by a KM_INIT_* method applied to the rows' dense expansion, without forming it:
+ * the result equals km_init_centroids_* on the expanded matrix bit for bit (a
+ * repeated column holds its values added in stored order).  Replaces the start
+ * of a lloyd_csr training without initial centroids, which the reference runs
+ * with DAAL's k-means++ on the host (train_kernel_lloyd_csr_dpc.cpp:41-68,
+ * kmeans/detail/train_init_centroids.hpp); preconditions and messages as
+ * km_init_centroids_*, plus the CSR structure checks. */
+km_status km_init_centroids_csr_f32(km_ctx* ctx, const float* values, const int64_t* col_indices,
+                                    const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                    int32_t indexing, int64_t k, int32_t method, uint64_t seed,
+                                    uint32_t flags, float* C_out);
+km_status km_init_centroids_csr_f64(km_ctx* ctx, const double* values, const int64_t* col_indices,
+                                    const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz,
+                                    int32_t indexing, int64_t k, int32_t method, uint64_t seed,
+                                    uint32_t flags, double* C_out);
+
 /* One instance of a batched Lloyd training; all pointers are device memory. */
 typedef struct {
     const float* X;      /* n x d row-major points */

@@ -31,7 +31,9 @@ namespace km {
 
 // Structure check: row offsets non-decreasing from `base` to base + nnz, every
 // column index in [base, base + d).  Each thread reads a row's entries only after
-// its own offsets passed the check.  flag bit 1: bad offsets, bit 2: bad column.
+// its own offsets passed the check.  flag bit 1: bad offsets, bit 2: bad column,
+// bit 4: a row whose column indices do not increase strictly (unsorted or
+// repeated; allowed, reported for the initialisation's feature-order walk).
 __global__ __launch_bounds__(256) void csr_check_kernel(const int64_t* __restrict__ cols,
                                                         const int64_t* __restrict__ offs,
                                                         int64_t base, int64_t n, int64_t d,
@@ -46,13 +48,18 @@ __global__ __launch_bounds__(256) void csr_check_kernel(const int64_t* __restric
             atomicOr(flag, 1);
             continue;
         }
+        int64_t prev = -1;
+        bool inc = true;
         for (int64_t p = o0; p < o1; ++p) {
             const int64_t c = cols[p] - base;
             if (c < 0 || c >= d) {
                 atomicOr(flag, 2);
                 break;
             }
+            inc = inc && c > prev;
+            prev = c;
         }
+        if (!inc) atomicOr(flag, 4);
     }
 }
 

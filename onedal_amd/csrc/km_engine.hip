@@ -1015,11 +1015,12 @@ struct csr_view {
 };
 
 // device view of the caller's CSR arrays (uploaded unless they are device
-// pointers), after the structure check
+// pointers), after the structure check; *increasing (if given) tells whether
+// every row's column indices increase strictly
 template <typename F>
 static csr_view<F> csr_prepare(km_ctx* ctx, const F* vals, const int64_t* cols,
                                const int64_t* offs, int64_t n, int64_t d, int64_t nnz,
-                               int32_t indexing, bool dev) {
+                               int32_t indexing, bool dev, bool* increasing = nullptr) {
     csr_view<F> v{vals, cols, offs, indexing == KM_SPARSE_ONE_BASED ? 1 : 0, n, d, nnz};
     if (!dev) {
         F* vb = ctx->csr_vals.get<F>(std::max<int64_t>(nnz, 1));
@@ -1054,6 +1055,7 @@ static csr_view<F> csr_prepare(km_ctx* ctx, const F* vals, const int64_t* cols,
     if (pin[0] & 2)
         throw status_error(KM_INVALID_ARGUMENT,
                            "Column indices of the sparse data must lie in [base, base + column count)");
+    if (increasing) *increasing = (pin[0] & 4) == 0;
     return v;
 }
 
@@ -1864,9 +1866,25 @@ static void batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
 // plus_plus_dense methods.  Preconditions of kmeans_init/detail/compute_ops.hpp:
 // 40-56 (checked in that order).
 // ---------------------------------------------------------------------------
-template <typename F>
-static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_t k, int method,
-                           uint64_t seed, uint32_t flags, F* C_out) {
+// k distinct row ids of [0, n): a partial Fisher-Yates shuffle driven by the
+// counter-based generator (random_dense)
+static std::vector<int64_t> init_random_rows(uint64_t seed, int64_t n, int64_t k) {
+    std::vector<int64_t> ids(k);
+    std::unordered_map<int64_t, int64_t> moved;
+    for (int64_t i = 0; i < k; ++i) {
+        const int64_t j = i + int64_t(rng_index(seed, RNG_STREAM_RANDOM, uint64_t(i), uint64_t(n - i)));
+        auto at = [&](int64_t p) {
+            auto it = moved.find(p);
+            return it == moved.end() ? p : it->second;
+        };
+        const int64_t vi = at(i), vj = at(j);
+        ids[i] = vj;
+        moved[j] = vi;
+    }
+    return ids;
+}
+
+static void init_check_args(const void* X, int64_t n, int64_t d, int64_t k, int method, const void* C_out) {
     if (X == nullptr || n <= 0 || d <= 0) throw status_error(KM_DOMAIN_ERROR, "Input data is empty");
     if (k <= 0) throw status_error(KM_DOMAIN_ERROR, "Cluster count is lower than or equal to zero");
     if (k >= n) throw status_error(KM_DOMAIN_ERROR, "Cluster count exceeds data row count");
@@ -1877,6 +1895,12 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
         throw status_error(KM_INVALID_ARGUMENT, "unknown initialisation method");
     if (d > std::numeric_limits<int32_t>::max())
         throw status_error(KM_DOMAIN_ERROR, "Column count is greater than max int32 value");
+}
+
+template <typename F>
+static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_t k, int method,
+                           uint64_t seed, uint32_t flags, F* C_out) {
+    init_check_args(X, n, d, k, method, C_out);
     const bool dev = (flags & KM_FLAG_DEVICE_POINTERS) != 0;
     const F* Xd = X;
     F* Cd = C_out;
@@ -1895,18 +1919,7 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
     } else if (method == KM_INIT_RANDOM_DENSE) {
         // k distinct rows by a partial Fisher-Yates shuffle of [0, n) on the host,
         // gathered on the device (compute_kernel_distr_random_dense_dpc.cpp:33-66)
-        std::vector<int64_t> ids(k);
-        std::unordered_map<int64_t, int64_t> moved;
-        for (int64_t i = 0; i < k; ++i) {
-            const int64_t j = i + int64_t(rng_index(seed, RNG_STREAM_RANDOM, uint64_t(i), uint64_t(n - i)));
-            auto at = [&](int64_t p) {
-                auto it = moved.find(p);
-                return it == moved.end() ? p : it->second;
-            };
-            const int64_t vi = at(i), vj = at(j);
-            ids[i] = vj;
-            moved[j] = vi;
-        }
+        const std::vector<int64_t> ids = init_random_rows(seed, n, k);
         int64_t* dids = ctx->init_ids.get<int64_t>(k);
         KM_HIP(hipMemcpyAsync(dids, ids.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(init_gather_rows_kernel<F>, dim3(grid_for(ctx, k, 1, 4)), dim3(64), 0, ctx->stream, Xd,
@@ -1936,6 +1949,128 @@ static void init_centroids(km_ctx* ctx, const F* X, int64_t n, int64_t d, int64_
             KM_LAUNCH_CHECK();
             hipLaunchKernelGGL(init_copy_row_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, Xd, di, chosen,
                                int64_t(0), Cd, t);
+            KM_LAUNCH_CHECK();
+        }
+    }
+    if (!dev) KM_HIP(hipMemcpyAsync(C_out, Cd, sizeof(F) * k * d, hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// Rows of a CSR matrix with each row's column indices strictly increasing and
+// zero-based: entries sorted by column (stable), the values of a repeated column
+// added in stored order onto a zero, as the row's dense expansion holds them.
+template <typename F>
+static void csr_merge_rows(const F* vals, const int64_t* cols, const int64_t* offs, int64_t base, int64_t n,
+                           std::vector<F>& mv, std::vector<int64_t>& mc, std::vector<int64_t>& mo) {
+    mv.clear();
+    mc.clear();
+    mo.assign(size_t(n + 1), 0);
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t o0 = offs[i] - base, o1 = offs[i + 1] - base;
+        idx.resize(size_t(o1 - o0));
+        for (int64_t p = o0; p < o1; ++p) idx[size_t(p - o0)] = p;
+        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return cols[a] < cols[b]; });
+        for (size_t j = 0; j < idx.size();) {
+            const int64_t col = cols[idx[j]];
+            F v = F(0);
+            for (; j < idx.size() && cols[idx[j]] == col; ++j) v = v + vals[idx[j]];
+            mc.push_back(col - base);
+            mv.push_back(v);
+        }
+        mo[size_t(i + 1)] = int64_t(mc.size());
+    }
+}
+
+// Initial centroids of CSR rows (km_init_centroids_csr_*): the dense methods
+// applied to the rows' dense expansion, which is never formed.  The reference
+// starts a lloyd_csr training without initial centroids with DAAL's k-means++
+// on the host (train_kernel_lloyd_csr_dpc.cpp:41-68).  Every step is the dense
+// driver's with the CSR row walk in place of the row read, so the centroids
+// equal init_centroids on the expanded matrix bit for bit.
+template <typename F>
+static void init_centroids_csr(km_ctx* ctx, const F* vals, const int64_t* cols, const int64_t* offs,
+                               int64_t n, int64_t d, int64_t nnz, int32_t indexing, int64_t k, int method,
+                               uint64_t seed, uint32_t flags, F* C_out) {
+    init_check_args(offs, n, d, k, method, C_out);
+    const bool dev = (flags & KM_FLAG_DEVICE_POINTERS) != 0;
+    bool increasing = true;
+    csr_view<F> x = csr_prepare<F>(ctx, vals, cols, offs, n, d, nnz, indexing, dev, &increasing);
+    // merged rows (only when a row has unsorted or repeated columns), kept alive
+    // until the stream has consumed them
+    std::vector<F> mv;
+    std::vector<int64_t> mc, mo;
+    if (!increasing) {
+        std::vector<F> hv;
+        std::vector<int64_t> hc, ho;
+        const F* sv = vals;
+        const int64_t* sc = cols;
+        const int64_t* so = offs;
+        if (dev) {
+            hv.resize(size_t(nnz));
+            hc.resize(size_t(nnz));
+            ho.resize(size_t(n + 1));
+            if (nnz > 0) {
+                KM_HIP(hipMemcpyAsync(hv.data(), vals, sizeof(F) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+                KM_HIP(hipMemcpyAsync(hc.data(), cols, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+            }
+            KM_HIP(hipMemcpyAsync(ho.data(), offs, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+            KM_HIP(hipStreamSynchronize(ctx->stream));
+            sv = hv.data();
+            sc = hc.data();
+            so = ho.data();
+        }
+        csr_merge_rows<F>(sv, sc, so, x.base, n, mv, mc, mo);
+        const int64_t mnz = int64_t(mc.size());
+        F* vb = ctx->csr_vals.get<F>(std::max<int64_t>(mnz, 1));
+        int64_t* cb = ctx->csr_cols.get<int64_t>(std::max<int64_t>(mnz, 1));
+        int64_t* ob = ctx->csr_offs.get<int64_t>(n + 1);
+        if (mnz > 0) {
+            KM_HIP(hipMemcpyAsync(vb, mv.data(), sizeof(F) * mnz, hipMemcpyHostToDevice, ctx->stream));
+            KM_HIP(hipMemcpyAsync(cb, mc.data(), sizeof(int64_t) * mnz, hipMemcpyHostToDevice, ctx->stream));
+        }
+        KM_HIP(hipMemcpyAsync(ob, mo.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream));
+        KM_HIP(hipStreamSynchronize(ctx->stream));
+        x = csr_view<F>{vb, cb, ob, 0, n, d, mnz};
+    }
+    const int di = int(d);
+    F* Cd = dev ? C_out : ctx->c_out_dev.get<F>(k * d);
+    std::vector<int64_t> ids;
+    if (method == KM_INIT_DENSE || method == KM_INIT_RANDOM_DENSE) {
+        const int64_t* dids = nullptr;
+        if (method == KM_INIT_RANDOM_DENSE) {
+            ids = init_random_rows(seed, n, k);
+            int64_t* b = ctx->init_ids.get<int64_t>(k);
+            KM_HIP(hipMemcpyAsync(b, ids.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, ctx->stream));
+            dids = b;
+        }
+        hipLaunchKernelGGL(init_expand_csr_rows_kernel<F>, dim3(grid_for(ctx, k, 1, 4)), dim3(256), 0,
+                           ctx->stream, x.vals, x.cols, x.offs, x.base, di, dids, nullptr, int64_t(0), k, Cd,
+                           int64_t(0));
+        KM_LAUNCH_CHECK();
+        KM_HIP(hipStreamSynchronize(ctx->stream));  // ids is read by the expansion above
+    } else {
+        double* D2 = ctx->init_d2.get<double>(n);
+        const int64_t nchunks = (n + OBJ_CHUNK - 1) / OBJ_CHUNK;
+        double* chunks = ctx->init_chunks.get<double>(nchunks);
+        int64_t* chosen = ctx->init_chosen.get<int64_t>(1);
+        const int64_t first = int64_t(rng_index(seed, RNG_STREAM_PLUS_PLUS, 0, uint64_t(n)));
+        hipLaunchKernelGGL(init_expand_csr_rows_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, x.vals, x.cols,
+                           x.offs, x.base, di, nullptr, nullptr, first, int64_t(1), Cd, int64_t(0));
+        KM_LAUNCH_CHECK();
+        const int d2_grid = grid_for(ctx, n, 256, 8);
+        const int ch_grid = grid_for(ctx, nchunks, 1, 8);
+        for (int64_t t = 1; t < k; ++t) {
+            hipLaunchKernelGGL(init_d2_csr_kernel<F>, dim3(d2_grid), dim3(256), 0, ctx->stream, x.vals, x.cols,
+                               x.offs, x.base, n, di, Cd + (t - 1) * d, D2, t == 1 ? 1 : 0);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(init_chunk_sums_kernel, dim3(ch_grid), dim3(256), 0, ctx->stream, D2, n, chunks);
+            KM_LAUNCH_CHECK();
+            const double u = rng_uniform01(seed, RNG_STREAM_PLUS_PLUS, uint64_t(t));
+            hipLaunchKernelGGL(init_select_kernel, dim3(1), dim3(256), 0, ctx->stream, D2, n, chunks, u, chosen);
+            KM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(init_expand_csr_rows_kernel<F>, dim3(1), dim3(256), 0, ctx->stream, x.vals,
+                               x.cols, x.offs, x.base, di, nullptr, chosen, int64_t(0), int64_t(1), Cd, t);
             KM_LAUNCH_CHECK();
         }
     }
@@ -2253,6 +2388,21 @@ km_status km_init_centroids_f64(km_ctx* ctx, const double* X, int64_t n, int64_t
                                 int32_t method, uint64_t seed, uint32_t flags, double* C_out) {
     return km::guarded(ctx, [&] { km::init_centroids<double>(ctx, X, n, d, k, method, seed, flags, C_out); });
 }
+
+#define KM_INIT_CSR_ABI(SUF, F)                                                                       \
+    km_status km_init_centroids_csr_##SUF(km_ctx* ctx, const F* values, const int64_t* col_indices,      \
+                                          const int64_t* row_offsets, int64_t n, int64_t d, int64_t nnz, \
+                                          int32_t indexing, int64_t k, int32_t method, uint64_t seed,    \
+                                          uint32_t flags, F* C_out) {                                    \
+        return km::guarded(ctx, [&] {                                                                     \
+            km::check_csr_args(values, col_indices, row_offsets, nnz, indexing);                          \
+            km::init_centroids_csr<F>(ctx, values, col_indices, row_offsets, n, d, nnz, indexing, k,      \
+                                      method, seed, flags, C_out);                                        \
+        });                                                                                               \
+    }
+KM_INIT_CSR_ABI(f32, float)
+KM_INIT_CSR_ABI(f64, double)
+#undef KM_INIT_CSR_ABI
 
 km_status km_batch_autotune(km_ctx* ctx, int32_t* table_out, int64_t table_len) {
     return km::guarded(ctx, [&] {

@@ -248,4 +248,71 @@ __global__ void init_gather_rows_kernel(const F* __restrict__ X, int d, const in
         for (int f = threadIdx.x; f < d; f += blockDim.x) C[i * d + f] = X[ids[i] * d + f];
 }
 
+// ---------------------------------------------------------------------------
+// CSR rows.  The kernels below take rows whose column indices (minus `base`)
+// increase strictly within each row; the driver merges unsorted or repeated
+// columns first.  A row then stands for its dense expansion: x_f = F(0) + v at
+// a stored column f (the expansion adds each stored value to a zero), 0
+// elsewhere.
+// ---------------------------------------------------------------------------
+constexpr int INIT_CSR_FT = 1024;  // centroid features staged per pass
+
+// init_d2_kernel on the dense expansion, without forming it: each thread walks
+// its row's features in order (a missing feature contributes (0 - c_f)^2), so
+// every sub, mul and add is the dense kernel's and D2 agrees bit for bit.
+template <typename F>
+__global__ __launch_bounds__(256) void init_d2_csr_kernel(const F* __restrict__ vals,
+                                                          const int64_t* __restrict__ cols,
+                                                          const int64_t* __restrict__ offs, int64_t base,
+                                                          int64_t n, int d, const F* __restrict__ c,
+                                                          double* __restrict__ D2, int first) {
+    __shared__ double cs[INIT_CSR_FT];
+    const int t = threadIdx.x;
+    for (int64_t r0 = int64_t(blockIdx.x) * 256; r0 < n; r0 += int64_t(gridDim.x) * 256) {
+        const int64_t r = r0 + t;
+        const bool ok = r < n;
+        int64_t p = ok ? offs[r] - base : 0;
+        const int64_t e = ok ? offs[r + 1] - base : 0;
+        int64_t nc = p < e ? cols[p] - base : INT64_MAX;  // next stored column
+        double acc = 0.0;
+        for (int f0 = 0; f0 < d; f0 += INIT_CSR_FT) {
+            const int ft = min(INIT_CSR_FT, d - f0);
+            __syncthreads();
+            for (int q = t; q < ft; q += 256) cs[q] = double(c[f0 + q]);
+            __syncthreads();
+            if (ok) {
+                for (int q = 0; q < ft; ++q) {
+                    double xv = 0.0;
+                    if (nc == int64_t(f0 + q)) {
+                        xv = double(F(0) + vals[p]);
+                        ++p;
+                        nc = p < e ? cols[p] - base : INT64_MAX;
+                    }
+                    const double diff = xv - cs[q];
+                    const double sq = diff * diff;
+                    acc = acc + sq;
+                }
+            }
+        }
+        if (ok) D2[r] = first ? acc : fmin(D2[r], acc);
+    }
+}
+
+// C[row_out + i] = dense expansion of row src_i for i < count: src_i = ids[i],
+// or *chosen (count 1), or first + i; one workgroup per output row
+template <typename F>
+__global__ void init_expand_csr_rows_kernel(const F* __restrict__ vals, const int64_t* __restrict__ cols,
+                                            const int64_t* __restrict__ offs, int64_t base, int d,
+                                            const int64_t* __restrict__ ids, const int64_t* __restrict__ chosen,
+                                            int64_t first, int64_t count, F* __restrict__ C, int64_t row_out) {
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int64_t src = ids ? ids[i] : chosen ? chosen[0] : first + i;
+        F* dst = C + (row_out + i) * int64_t(d);
+        for (int f = threadIdx.x; f < d; f += blockDim.x) dst[f] = F(0);
+        __syncthreads();  // zeros before the stored values (columns are distinct)
+        const int64_t o0 = offs[src] - base, o1 = offs[src + 1] - base;
+        for (int64_t p = o0 + threadIdx.x; p < o1; p += blockDim.x) dst[cols[p] - base] = F(0) + vals[p];
+    }
+}
+
 }  // namespace km

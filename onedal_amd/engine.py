@@ -40,7 +40,8 @@ EXPORTED_SYMBOLS = [
     "km_context_create", "km_context_destroy", "km_last_error", "km_context_set_option",
     "km_get_stats", "km_lloyd_train_f32", "km_lloyd_train_f64", "km_lloyd_infer_f32",
     "km_lloyd_infer_f64", "km_lloyd_infer_f32_noalloc", "km_device_alloc", "km_device_free", "km_memcpy",
-    "km_init_centroids_f32", "km_init_centroids_f64",
+    "km_init_centroids_f32", "km_init_centroids_f64", "km_init_centroids_csr_f32",
+    "km_init_centroids_csr_f64",
     "km_lloyd_train_csr_f32", "km_lloyd_train_csr_f64", "km_lloyd_infer_csr_f32",
     "km_lloyd_infer_csr_f64", "km_measure_bf16_mfma_peak",
     "km_batch_train_f32", "km_batch_autotune", "km_batch_step_f32",
@@ -172,6 +173,10 @@ def load_library():
     for sfx in ("f32", "f64"):
         f = getattr(L, "km_init_centroids_" + sfx)
         f.argtypes = [P, P, I64, I64, I64, ctypes.c_int32, ctypes.c_uint64, U32, P]
+        f.restype = I
+        f = getattr(L, "km_init_centroids_csr_" + sfx)
+        f.argtypes = [P, P, P, P, I64, I64, I64, ctypes.c_int32, I64, ctypes.c_int32,
+                      ctypes.c_uint64, U32, P]
         f.restype = I
     L.km_measure_bf16_mfma_peak.argtypes = [P, I64, ctypes.POINTER(D), ctypes.POINTER(D)]
     L.km_measure_bf16_mfma_peak.restype = I
@@ -512,6 +517,34 @@ class Engine:
         st = getattr(self.lib, "km_init_centroids_" + sfx)(
             self.ctx, x.ctypes.data, x.shape[0], x.shape[1], int(k), int(method),
             ctypes.c_uint64(seed & (2**64 - 1)), 0, c.ctypes.data)
+        self._check(st)
+        return c
+
+    def init_centroids_csr(self, values, cols, offs, d, k, method, seed=777, base=0):
+        """Initial centroids (KM_INIT_* method) of CSR rows, equal to init_centroids
+        on their dense expansion; host arrays in, numpy out, or CUDA tensors in,
+        a CUDA tensor out."""
+        dev = _check_csr(values, cols, offs)
+        n = int(offs.shape[0]) - 1
+        nnz = int(values.shape[0])
+        if dev:
+            import torch
+
+            sfx = _sfx(str(values.dtype).replace("torch.", ""))
+            c = torch.empty((k, d), dtype=values.dtype, device=values.device)
+            ptrs = [values.data_ptr(), cols.data_ptr(), offs.data_ptr(), c.data_ptr()]
+            flags = KM_FLAG_DEVICE_POINTERS
+        else:
+            values = np.ascontiguousarray(values)
+            sfx = _sfx(values.dtype)
+            cols = np.ascontiguousarray(cols, dtype=np.int64)
+            offs = np.ascontiguousarray(offs, dtype=np.int64)
+            c = np.empty((k, d), values.dtype)
+            ptrs = [values.ctypes.data, cols.ctypes.data, offs.ctypes.data, c.ctypes.data]
+            flags = 0
+        st = getattr(self.lib, "km_init_centroids_csr_" + sfx)(
+            self.ctx, ptrs[0], ptrs[1], ptrs[2], n, int(d), nnz, int(base), int(k), int(method),
+            ctypes.c_uint64(seed & (2**64 - 1)), flags, ptrs[3])
         self._check(st)
         return c
 

@@ -349,6 +349,25 @@ static void csr_checks(dal::hip::queue& q) {
         bool in_range = true;
         for (int64_t i = 0; i < n; ++i) in_range = in_range && ll[i] >= 0 && ll[i] < k;
         CHECK(in_range);
+        // the default start of a CSR training is compute<plus_plus_dense> (seed 777)
+        // on the rows' dense expansion, for host and device tables alike
+        namespace ki = oneapi::dal::kmeans_init;
+        std::vector<F> xd(size_t(n * d), F(0));
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t p = ro[i]; p < ro[i + 1]; ++p) xd[size_t(i * d + ci[p])] += v[p];
+        const auto c0 = dal::compute(q, ki::descriptor<F, ki::method::plus_plus_dense>(k),
+                                     dal::homogen_table::wrap(xd.data(), n, d))
+                            .get_centroids();
+        const auto er = dal::train(q, desc, hx, c0);
+        const auto dr0 = dal::train(q, desc, dx);
+        CHECK(er.get_objective_function_value() == hr.get_objective_function_value());
+        CHECK(dr0.get_objective_function_value() == hr.get_objective_function_value());
+        const auto ec = dal::row_accessor<const F>(er.get_model().get_centroids()).pull(q);
+        const auto hc = dal::row_accessor<const F>(hr.get_model().get_centroids()).pull(q);
+        const auto d0c = dal::row_accessor<const F>(dr0.get_model().get_centroids()).pull(q);
+        bool same_c = true;
+        for (int64_t i = 0; i < k * d; ++i) same_c = same_c && ec[i] == hc[i] && d0c[i] == hc[i];
+        CHECK(same_c);
     }
     CHECK(km_device_free(q.get_context(), dv) == KM_OK);
     CHECK(km_device_free(q.get_context(), dci) == KM_OK);
