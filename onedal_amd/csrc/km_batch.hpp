@@ -560,8 +560,17 @@ __global__ __launch_bounds__(256) void batch_count_kernel(const batch_inst* __re
         if (lds)
             for (int j = threadIdx.x; j < in.k; j += 256) hist[j] = 0;
         __syncthreads();
-        for (int r = threadIdx.x; r < nr; r += 256) {
-            const int lab = labels[in.row0 + r0 + r];
+        // every label of the thread loaded before the first atomic
+        constexpr int RPT = BATCH_CNT_ROWS / 256;
+        int lv[RPT];
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+            const int r = threadIdx.x + 256 * u;
+            lv[u] = r < nr ? labels[in.row0 + r0 + r] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+            const int lab = lv[u];
             if (lab >= 0 && lab < in.k) {
                 if (lds) atomicAdd(&hist[lab], 1);
                 else atomicAdd(&counts[in.k0 + lab], 1);
@@ -617,6 +626,17 @@ __global__ __launch_bounds__(256) void sort_group_sums_kernel(const int32_t* __r
     }
     for (; lb < b1; ++lb) s += bhist[lb * k + j];
     gsum[e] = s;
+}
+
+// counts[j] = label j's count over every block (the column sums of gsum), in
+// place of one global atomic per (block, label)
+__global__ __launch_bounds__(256) void sort_counts_kernel(const int32_t* __restrict__ gsum, int k, int G,
+                                                          int32_t* __restrict__ counts) {
+    const int j = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (j >= k) return;
+    int32_t s = 0;
+    for (int g = 0; g < G; ++g) s += gsum[int64_t(g) * k + j];
+    counts[j] = s;
 }
 
 __global__ __launch_bounds__(256) void sort_block_bases_kernel(int32_t* __restrict__ bhist, int64_t nb, int k,

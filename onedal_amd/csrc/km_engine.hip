@@ -585,14 +585,22 @@ static label_sort label_sort_setup(km_ctx* ctx, int64_t n, int k) {
     return ls;
 }
 
-// counts (zeroed first) of the labels
+// counts of the labels (with the counting sort: the block histograms, their
+// group sums, and the counts as the column sums of those)
 static void label_counts(km_ctx* ctx, const label_sort& ls, const int32_t* labels, int32_t* counts) {
-    fill<int32_t>(ctx, counts, ls.k, 0);
     if (ls.csort) {
         const int grid = int(std::min<int64_t>(ls.nb, int64_t(ctx->cu_count) * 8));
         hipLaunchKernelGGL(batch_count_kernel, dim3(grid), dim3(256), 0, ctx->stream, ls.I, 1, ls.pre, labels,
-                           counts, ls.bhist);
+                           nullptr, ls.bhist);
+        KM_LAUNCH_CHECK();
+        const int g2 = int((int64_t(ls.G) * ls.k + 255) / 256);
+        hipLaunchKernelGGL(sort_group_sums_kernel, dim3(g2), dim3(256), 0, ctx->stream, ls.bhist, ls.nb, ls.k, ls.G,
+                           ls.GB, ls.gsum);
+        KM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(sort_counts_kernel, dim3((ls.k + 255) / 256), dim3(256), 0, ctx->stream, ls.gsum, ls.k,
+                           ls.G, counts);
     } else {
+        fill<int32_t>(ctx, counts, ls.k, 0);
         const bool lds = ls.k <= 16384;
         hipLaunchKernelGGL(count_kernel, dim3(grid_for(ctx, ls.n, 256, 4)), dim3(256),
                            lds ? ls.k * sizeof(int32_t) : 0, ctx->stream, labels, ls.n, ls.k, counts, lds ? 1 : 0);
@@ -607,11 +615,8 @@ static void label_group(km_ctx* ctx, const label_sort& ls, const int32_t* labels
         sort_by_label(ctx, labels, ls.n, ls.k, perm);
         return;
     }
-    const int64_t gk = int64_t(ls.G) * ls.k;
-    const int g2 = int((gk + 255) / 256);
-    hipLaunchKernelGGL(sort_group_sums_kernel, dim3(g2), dim3(256), 0, ctx->stream, ls.bhist, ls.nb, ls.k, ls.G,
-                       ls.GB, ls.gsum);
-    KM_LAUNCH_CHECK();
+    // the group sums are label_counts' (bhist is unchanged since)
+    const int g2 = int((int64_t(ls.G) * ls.k + 255) / 256);
     hipLaunchKernelGGL(sort_block_bases_kernel, dim3(g2), dim3(256), 0, ctx->stream, ls.bhist, ls.nb, ls.k, ls.G,
                        ls.GB, ls.gsum, offsets);
     KM_LAUNCH_CHECK();
