@@ -744,17 +744,52 @@ __global__ __launch_bounds__(256) void compact_lists_kernel(seg_lists S, int nse
     bc = __shfl(bc, 0);
     bl = __shfl(bl, 0);
     const int64_t base = int64_t(seg) * S.cap;
-    for (int i = lane; i < nc; i += 64) {
-        const int64_t src = base + i, dst = int64_t(bc) + i;
-        const int cnt = S.cand_count[src];
-        L.cand_rows[dst] = S.cand_rows[src];
-        L.cand_count[dst] = cnt;
-        for (int c = 0; c < cnt; ++c) L.cand_ids[dst * MAXC + c] = S.cand_ids[src * MAXC + c];
+    // U entries per lane in flight; an entry's MAXC id slots move as two 16-byte
+    // vectors (entries are 32-byte aligned; slots past its count are never read)
+    constexpr int U = 4;
+    static_assert(MAXC == 8, "an entry's ids are two 16-byte vectors");
+    for (int i0 = lane; i0 < nc; i0 += 64 * U) {
+        int32_t row[U], cnt[U];
+        uintx4 id0[U], id1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u;
+            const int64_t src = base + (i < nc ? i : 0);
+            row[u] = S.cand_rows[src];
+            cnt[u] = S.cand_count[src];
+            id0[u] = *reinterpret_cast<const uintx4*>(S.cand_ids + src * MAXC);
+            id1[u] = *reinterpret_cast<const uintx4*>(S.cand_ids + src * MAXC + 4);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u;
+            if (i < nc) {
+                const int64_t dst = int64_t(bc) + i;
+                L.cand_rows[dst] = row[u];
+                L.cand_count[dst] = cnt[u];
+                *reinterpret_cast<uintx4*>(L.cand_ids + dst * MAXC) = id0[u];
+                *reinterpret_cast<uintx4*>(L.cand_ids + dst * MAXC + 4) = id1[u];
+            }
+        }
     }
-    for (int i = lane; i < nl; i += 64) {
-        const int64_t src = base + i, dst = int64_t(bl) + i;
-        L.coll_rows[dst] = S.coll_rows[src];
-        L.coll_T[dst] = S.coll_T[src];
+    for (int i0 = lane; i0 < nl; i0 += 64 * U) {
+        int32_t row[U];
+        double T[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u;
+            const int64_t src = base + (i < nl ? i : 0);
+            row[u] = S.coll_rows[src];
+            T[u] = S.coll_T[src];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 64 * u;
+            if (i < nl) {
+                L.coll_rows[int64_t(bl) + i] = row[u];
+                L.coll_T[int64_t(bl) + i] = T[u];
+            }
+        }
     }
 }
 

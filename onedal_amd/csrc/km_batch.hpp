@@ -635,6 +635,7 @@ __global__ __launch_bounds__(256) void sort_counts_kernel(const int32_t* __restr
     const int j = int(blockIdx.x) * 256 + int(threadIdx.x);
     if (j >= k) return;
     int32_t s = 0;
+#pragma unroll 16
     for (int g = 0; g < G; ++g) s += gsum[int64_t(g) * k + j];
     counts[j] = s;
 }
@@ -646,9 +647,22 @@ __global__ __launch_bounds__(256) void sort_block_bases_kernel(int32_t* __restri
     if (e >= int64_t(G) * k) return;
     const int g = int(e / k), j = int(e - int64_t(g) * k);
     int32_t base = int32_t(offsets[j]);
+#pragma unroll 16
     for (int q = 0; q < g; ++q) base += gsum[int64_t(q) * k + j];
     const int64_t b0 = int64_t(g) * GB, b1 = min<int64_t>(nb, b0 + GB);
-    for (int64_t lb = b0; lb < b1; ++lb) {
+    // eight blocks' counts loaded before their destinations are written
+    int64_t lb = b0;
+    for (; lb + 8 <= b1; lb += 8) {
+        int32_t c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = bhist[(lb + u) * k + j];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            bhist[(lb + u) * k + j] = base;
+            base += c[u];
+        }
+    }
+    for (; lb < b1; ++lb) {
         const int32_t c = bhist[lb * k + j];
         bhist[lb * k + j] = base;
         base += c;

@@ -375,12 +375,12 @@ __global__ __launch_bounds__(256) void finalize_centroids_kernel(
         int q = chunk_base[j];
         const int q1 = chunk_base[j + 1];
         // loads issued ahead, adds kept in chunk order
-        for (; q + 8 <= q1; q += 8) {
-            double v[8];
+        for (; q + 16 <= q1; q += 16) {
+            double v[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = part[int64_t(q + u) * d + f];
+            for (int u = 0; u < 16; ++u) v[u] = part[int64_t(q + u) * d + f];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) s = s + v[u];
+            for (int u = 0; u < 16; ++u) s = s + v[u];
         }
         for (; q < q1; ++q) s = s + part[int64_t(q) * d + f];
         Cnext[e] = F(s / double(cnt));
