@@ -250,7 +250,7 @@ __global__ void init_gather_rows_kernel(const F* __restrict__ X, int d, const in
 
 // ---------------------------------------------------------------------------
 // CSR rows.  The kernels below take rows whose column indices (minus `base`)
-// increase strictly within each row; the driver merges unsorted or repeated
+// increase strictly within each row; init_centroids_csr merges unsorted or repeated
 // columns first.  A row then stands for its dense expansion: x_f = F(0) + v at
 // a stored column f (the expansion adds each stored value to a zero), 0
 // elsewhere.

@@ -1,4 +1,4 @@
-// kmeans_cpu_fast.cpp -- CPU BASELINE, not the product and not the bit-exact
+// kmeans_cpu_fast.cpp -- CPU baseline, not the product and not the bit-exact
 // oracle: the reference's CPU Lloyd algorithm (DAAL,
 // daal/src/algorithms/kmeans/kmeans_lloyd_impl.i:136-246 and
 // kmeans_lloyd_batch_impl.i:117-213) with its X.C^T product written as a
