@@ -170,3 +170,30 @@ def test_init_csr_preconditions(engine):
     with pytest.raises(InvalidArgument, match="Column indices"):
         engine.init_centroids_csr(vals, bad, offs, 4, 3, 2)
     assert engine.init_centroids_csr(vals, cols, offs, 4, 3, 2).shape == (3, 4)
+
+
+def test_init_csr_edge_shapes(engine, oracle):
+    """k = 1, d = 1, every row empty (all D^2 zero after the first pick), a
+    single stored entry, and nnz = 0 with device arrays."""
+    import torch
+
+    cases = []
+    # every row empty: the expansion is all zeros
+    cases.append((np.zeros(0, np.float32), np.zeros(0, np.int64), np.zeros(51, np.int64), 7, 5))
+    # one stored entry in the whole matrix
+    cases.append((np.array([3.5], np.float64), np.array([2], np.int64),
+                  np.array([0] * 20 + [1] * 31, np.int64), 4, 3))
+    # d = 1, k = 1
+    v = np.arange(1, 41, dtype=np.float32)
+    cases.append((v, np.zeros(40, np.int64), np.arange(41, dtype=np.int64), 1, 1))
+    for vals, cols, offs, d, k in cases:
+        x = _expand(vals, cols, offs, d)
+        for method in (0, 1, 2):
+            want = oracle.init_centroids(x, k, method, 5)
+            got = engine.init_centroids_csr(vals, cols, offs, d, k, method, 5)
+            assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (d, k, method)
+    vals, cols, offs, d, k = cases[0]
+    gd = engine.init_centroids_csr(torch.from_numpy(vals).cuda(), torch.from_numpy(cols).cuda(),
+                                   torch.from_numpy(offs).cuda(), d, k, 2, 5)
+    torch.cuda.synchronize()
+    assert np.array_equal(gd.cpu().numpy(), oracle.init_centroids(_expand(vals, cols, offs, d), k, 2, 5))
