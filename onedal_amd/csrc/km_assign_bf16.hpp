@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void centroid_prep_bf16_kernel(
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     for (int j = k + wave; j < kp; j += nwaves)
         for (int f = lane; f < dpb; f += 64) Cb[int64_t(j) * dpb + f] = 0;
+    unsigned long long b0 = 0, b1 = 0, b2 = 0;  // lane 0: this wave's maxima
     for (int j = wave; j < k; j += nwaves) {
         double s = 0.0, sh = 0.0, se = 0.0;
         for (int f = lane; f < dpb; f += 64) {
@@ -213,10 +214,25 @@ __global__ __launch_bounds__(256) void centroid_prep_bf16_kernel(
         if (lane == 0) {
             cn[j] = float(s);
             const double up = 1.0 + 1e-12;
-            atomicMax(&max_bits[0], (unsigned long long)__double_as_longlong(sqrt(s) * up));
-            atomicMax(&max_bits[1], (unsigned long long)__double_as_longlong(sqrt(sh) * up));
-            atomicMax(&max_bits[2], (unsigned long long)__double_as_longlong(sqrt(se) * up + 1e-30));
+            b0 = max(b0, (unsigned long long)__double_as_longlong(sqrt(s) * up));
+            b1 = max(b1, (unsigned long long)__double_as_longlong(sqrt(sh) * up));
+            b2 = max(b2, (unsigned long long)__double_as_longlong(sqrt(se) * up + 1e-30));
         }
+    }
+    // one atomic per bound and workgroup (the maxima of non-negative doubles, or
+    // NaN, taken on their bit patterns as before)
+    __shared__ unsigned long long wb[4][3];
+    const int w = threadIdx.x >> 6;
+    if (lane == 0) {
+        wb[w][0] = b0;
+        wb[w][1] = b1;
+        wb[w][2] = b2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int q = threadIdx.x;
+        const unsigned long long m = max(max(wb[0][q], wb[1][q]), max(wb[2][q], wb[3][q]));
+        if (m) atomicMax(&max_bits[q], m);
     }
 }
 

@@ -273,7 +273,8 @@ static void centroid_prep(km_ctx* ctx, work<F>& w, const F* C, int path) {
         if (path == 3) {
             fill<unsigned long long>(ctx, w.cmax_bits, 3, 0ull);
             const int kp = bf16_padded_k(w.k);
-            const int grid = grid_for(ctx, int64_t(kp) * 64, 256, 4);
+            // four centroids per wave, so few workgroups add to the three maxima
+            const int grid = grid_for(ctx, int64_t(kp) * 16, 256, 4);
             hipLaunchKernelGGL(centroid_prep_bf16_kernel, dim3(grid), dim3(256), 0, ctx->stream, C,
                                w.k, kp, w.d, w.dpad, w.Cb, w.cn, w.cmax_bits);
             KM_LAUNCH_CHECK();
