@@ -749,16 +749,35 @@ __global__ __launch_bounds__(256, 2) void assign_bf16_kernel(
 // each entry re-scores its own row).
 __global__ __launch_bounds__(256) void compact_lists_kernel(seg_lists S, int nseg, refine_lists L) {
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // one atomic per list and workgroup: thread 0 reserves the four segments' runs
+    // (segments past nseg count zero)
+    __shared__ int wbase[4][2];
+    if (threadIdx.x == 0) {
+        const int s0 = blockIdx.x * 4;
+        int c[4], l[4], tc = 0, tl = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            c[q] = s0 + q < nseg ? S.counts[2 * (s0 + q)] : 0;
+            l[q] = s0 + q < nseg ? S.counts[2 * (s0 + q) + 1] : 0;
+            tc += c[q];
+            tl += l[q];
+        }
+        int bc = tc ? atomicAdd(&L.counters[0], tc) : 0;
+        int bl = tl ? atomicAdd(&L.counters[2], tl) : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            wbase[q][0] = bc;
+            wbase[q][1] = bl;
+            bc += c[q];
+            bl += l[q];
+        }
+    }
+    __syncthreads();
     if (seg >= nseg) return;
     const int nc = S.counts[2 * seg], nl = S.counts[2 * seg + 1];
-    int bc = 0, bl = 0;
-    if (lane == 0) {
-        if (nc) bc = atomicAdd(&L.counters[0], nc);
-        if (nl) bl = atomicAdd(&L.counters[2], nl);
-    }
-    bc = __shfl(bc, 0);
-    bl = __shfl(bl, 0);
+    const int bc = wbase[wave][0], bl = wbase[wave][1];
     const int64_t base = int64_t(seg) * S.cap;
     // U entries per lane in flight; an entry's MAXC id slots move as two 16-byte
     // vectors (entries are 32-byte aligned; slots past its count are never read)
